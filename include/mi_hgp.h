@@ -1,0 +1,124 @@
+/*
+ * mi_hgp.h -- C ABI of the MI355X-native Head-Gordon/Pople two-electron
+ * integral engine (libmi_hgp.so).
+ *
+ * Drop-in boundary for MolecularIntegrals.jl's HGP path:
+ *   all_twoe_ints_chrr(bfs::Basis)::Vector{Float64}     src/HGP.jl:22-34
+ *   all_twoe_ints(bfs, ERI=coulomb)::Vector{Float64}    src/ERI.jl:104-112
+ * Same basis in, same 8-fold-unique (ij|kl) array out, in either of the two
+ * index orders those functions write (MI_ORDER_CHRR: enumeration of
+ * iiterator, src/Basis.jl:203; MI_ORDER_IINDEX: iindex, src/Utils.jl:19).
+ * Slots here are 0-based (the Julia slot minus one).
+ *
+ * Values are the package's tested definition, the normalised contracted
+ * coulomb(bfs[i],bfs[j],bfs[k],bfs[l]) (src/ERI.jl:102, src/Basis.jl:86-98),
+ * evaluated with HGP recurrences.  Plain calls (precision MI_PREC_FP64,
+ * schwarz_cut 0) are bitwise deterministic.
+ *
+ * Plain pointers and sizes only; no exception crosses the ABI.  Every call
+ * returns an MI_* status; mi_last_error() describes the last failure of a
+ * context.  One context per host thread.
+ */
+#ifndef MI_HGP_H
+#define MI_HGP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MI_OK 0
+#define MI_EINVAL 1        /* malformed basis / options / sizes            */
+#define MI_EUNSUPPORTED 2  /* angular momentum beyond the compiled tables  */
+#define MI_EDEVICE 3       /* HIP runtime failure                          */
+#define MI_ENOMEM 4        /* device allocation failed / budget exceeded   */
+#define MI_ESIZE 5         /* output length does not match the basis       */
+
+/* output orders */
+#define MI_ORDER_IINDEX 0  /* all_twoe_ints:      slot = iindex(i,j,k,l) - 1   */
+#define MI_ORDER_CHRR 1    /* all_twoe_ints_chrr: slot = enumeration index - 1 */
+
+/* precision modes */
+#define MI_PREC_FP64 0     /* everything in FP64                               */
+#define MI_PREC_MIXED 1    /* FP32 VRR for quartets whose Schwarz bound < tau  */
+
+typedef struct mi_ctx mi_ctx;
+typedef struct mi_plan mi_plan;
+
+/*
+ * Flattened Basis (MolecularIntegrals.Basis, src/Basis.jl:172-184):
+ *  shells in bfs.shells order; functions in bfs.cgbfs order, i.e. shell by
+ *  shell, each shell in shell_indices[L] order (src/Basis.jl:127-136).
+ *  prim_coef[p]  raw Shell.coefs
+ *  prim_norm[p]  PGBF norm of primitive p of the shell's first component
+ *                (cgbf.pgbfs[p].norm, src/Basis.jl:29-33)
+ *  bf_scale[i]   cgbfs[i].norm * cgbfs[i].pgbfs[1].norm / (first component's
+ *                pgbfs[1].norm)
+ *  With these, (ij|kl) = s_i s_j s_k s_l sum c'_p c'_q c'_r c'_s [pq|rs]
+ *  where c'_p = prim_coef[p]*prim_norm[p], which equals coulomb() on CGBFs.
+ */
+typedef struct {
+    int32_t nshell;
+    const double *shell_xyz;        /* [3*nshell], bohr                  */
+    const int32_t *shell_l;         /* [nshell], 0..MI_LMAX              */
+    const int32_t *shell_prim_off;  /* [nshell+1]                        */
+    const double *prim_exp;         /* [nprim]                           */
+    const double *prim_coef;        /* [nprim]                           */
+    const double *prim_norm;        /* [nprim]                           */
+    int32_t nbf;                    /* sum over shells of (L+1)(L+2)/2   */
+    const double *bf_scale;         /* [nbf]                             */
+} mi_basis;
+
+typedef struct {
+    int32_t order;       /* MI_ORDER_*                                         */
+    int32_t precision;   /* MI_PREC_*                                          */
+    double tau;          /* MIXED: quartets with Q_ab*Q_cd < tau use FP32 VRR  */
+    double schwarz_cut;  /* 0: compute every quartet (plain); >0: skip below   */
+    void *stream;        /* hipStream_t for *_device calls; NULL = default     */
+} mi_opts;
+
+/* Context: device buffers, Boys grid and op tables for one GPU.
+ * Replaces nothing in the reference (Julia keeps no device state). */
+int mi_create(int device, mi_ctx **out);
+void mi_destroy(mi_ctx *ctx);
+const char *mi_last_error(const mi_ctx *ctx);
+int32_t mi_lmax(void);                   /* largest supported shell L      */
+int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
+
+/* One-shot drop-in: host basis in, host array out (caller-owned, out_len
+ * must be mi_n_unique(nbf)).  Replaces all_twoe_ints_chrr (order CHRR,
+ * src/HGP.jl:22) and all_twoe_ints (order IINDEX, src/ERI.jl:104). */
+int mi_eri_unique(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, double *out, int64_t out_len);
+
+/* Plan / execute split: the plan holds the basis topology on the device
+ * (shell pairs sorted by class, tasks); execute recomputes shell-pair data
+ * and all ERIs into a caller-owned DEVICE buffer on opts->stream, without
+ * host synchronisation.  The analogue of the reference's eri_fetcher
+ * (src/Basis.jl:201-212) plus the threaded loop of src/HGP.jl:27-32. */
+int mi_plan_create(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, mi_plan **out);
+int mi_plan_execute(mi_plan *plan, double *out_device, int64_t out_len, void *stream);
+void mi_plan_destroy(mi_plan *plan);
+/* Plan statistics: [0]=nbf [1]=shell pairs [2]=primitive pairs
+ * [3]=shell quartets [4]=primitive quartets [5]=unique ERIs
+ * [6]=kernel launches per execute. */
+int mi_plan_stats(const mi_plan *plan, int64_t *stats, int32_t nstats);
+
+/* Instrumentation (no counterpart in the reference, whose timing lives in
+ * test/timing.jl).  A plan executes as launch -1 (shell-pair kernel) then
+ * launches 0..n-1, one per quartet class present. */
+int mi_plan_nlaunch(const mi_plan *plan);
+/* cls4 = (la,lb,lc,ld); counts3 = (shell quartets, primitive quartets,
+ * blocks).  Returns 1 for a register-kernel class, 0 for the LDS kernel,
+ * MI_EINVAL (negative never) on a bad index -- check with the count first. */
+int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *counts3);
+/* One execute with HIP events between launches on `stream`; synchronises and
+ * writes nlaunch+1 durations in ms (ms[0] = shell-pair kernel). */
+int mi_plan_profile(mi_plan *plan, double *out_device, int64_t out_len, void *stream, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MI_HGP_H */

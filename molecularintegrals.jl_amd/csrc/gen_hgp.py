@@ -1,0 +1,343 @@
+#!/usr/bin/env python3
+"""Generator for the HGP ERI device code (gfx950).
+
+It symbolically runs the reference's recurrences and emits HIP source:
+
+* VRR -- HGP eq. 6 exactly as ``vrr!`` applies it (rpmuller/MolecularIntegrals.jl
+  src/HGP.jl:135-242): the raised index is lowered along
+  ``shift_direction`` (first argmax of the Cartesian powers,
+  src/Basis.jl:346-354), a-build first with c = 0, then the c-build.
+* HRR -- ``chrr`` (src/HGP.jl:83-116): (a,b+1_j| = (a+1_j,b| + AB_j (a,b|,
+  then (ab|c,d+1_j) = (ab|c+1_j,d) + CD_j (ab|c,d), j = shift_direction.
+
+Only the entries the target block needs are generated (the reference fills the
+whole cumulative tables).  Two outputs:
+
+``hgp_rkernels.inc``  one straight-line kernel per "register" class: a lane
+    group of G lanes owns a shell quartet, splits its primitive quartets,
+    keeps VRR intermediates and contracted (e0|f0) in VGPRs, reduces across the
+    group with cross-lane shuffles, runs HRR in registers and scatters.
+``hgp_wtables.inc``   op tables for every class, consumed by the generic
+    wave-cooperative kernel (one wavefront per shell quartet, VRR/HRR levels
+    staged in LDS, lanes parallel within a level).
+
+Cartesian component order: src/Basis.jl:269-279 for L<=4 (f: xyy, xzz, xyz),
+descending I then J for L>=5 (HRR intermediates only).
+"""
+from __future__ import annotations
+
+import argparse
+import functools
+import os
+import sys
+from collections import OrderedDict
+
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+from mi_amd.hgp_tables import (AO2M, LMAX_SHELL, HrrGraph, VrrGraph, cname, lval, nao, ncart,  # noqa: E402
+                               quartet_classes, sdir, shell_aos, sidx, sidx_plus)
+
+# ---------------------------------------------------------------- R emitter
+
+XYZ = "xyz"
+
+
+def vname(e, f, m):
+    return f"v{e}_{f}_{m}"
+
+
+def emit_rkernel(c):
+    la, lb, lc, ld = c
+    g = VrrGraph(*c)
+    h = HrrGraph(*c)
+    amax, cmax, mmax = g.amax, g.cmax, g.mmax
+    name = cname(c)
+    L = []
+    w = L.append
+    w(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}): amax={amax} cmax={cmax} mmax={mmax}, "
+      f"{g.count_entries()} VRR entries, {len(g.targets)} contracted (e0|f0), {len(h.nodes)} HRR nodes")
+    w(f"__global__ void __launch_bounds__(256) hgp_r_{name}(const EriArgs args) {{")
+    w(f"  __shared__ double sboys[BOYS_NROW * 8];")
+    w(f"  stage_boys_slice(sboys, args.boys, {mmax - 1});")
+    w(f"  QuartetCtx qc;")
+    w(f"  if (!decode_quartet(args, qc)) return;")
+    w(f"  const PairTopo P = args.topo[qc.bra];")
+    w(f"  const PairTopo Q = args.topo[qc.ket];")
+    for (e, f) in g.targets:
+        w(f"  double acc{e}_{f} = 0.0;")
+    w(f"  const int nu = P.npp, nv = Q.npp, ntot = nu * nv;")
+    w(f"  const int G = qc.G;")
+    w(f"  int u = qc.g / nv, v = qc.g - (qc.g / nv) * nv;")
+    w(f"  const int du = G / nv, dv = G - du * nv;")
+    w(f"  for (int t = qc.g; t < ntot; t += G) {{")
+    w(f"    const PrimPair bp = args.pp[P.ppoff + u];")
+    w(f"    const PrimPair kp = args.pp[Q.ppoff + v];")
+    w(f"    v += dv; u += du; if (v >= nv) {{ v -= nv; ++u; }}")
+    w(f"    const double zeta = bp.zeta, eta = kp.zeta;")
+    w(f"    const double inv_s = 1.0 / (zeta + eta);")
+    w(f"    const double PQx = bp.Px - kp.Px, PQy = bp.Py - kp.Py, PQz = bp.Pz - kp.Pz;")
+    w(f"    const double T = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    w(f"    const double pref = bp.K * kp.K * sqrt(inv_s);")
+    w(f"    double F[{mmax}];")
+    w(f"    boys_eval<{mmax}>(T, sboys, F);")
+    uses = set()
+    body = []
+    for (e, f) in g.nodes:
+        rec = g.recurrence(e, f)
+        for m in range(g.maxm[(e, f)] + 1):
+            if rec[0] == "F":
+                body.append(f"    const double {vname(e, f, m)} = pref * F[{m}];")
+            elif rec[0] == "A":
+                _, i, e1, e2, n1 = rec
+                x = XYZ[i]
+                uses.update({f"PA{x}", f"WP{x}"})
+                expr = f"PA{x} * {vname(e1, 0, m)} + WP{x} * {vname(e1, 0, m + 1)}"
+                if e2 >= 0:
+                    uses.update({"oo2z", "eta_s"})
+                    coef = "oo2z" if n1 == 1 else f"({float(n1)} * oo2z)"
+                    expr += f" + {coef} * ({vname(e2, 0, m)} - eta_s * {vname(e2, 0, m + 1)})"
+                body.append(f"    const double {vname(e, f, m)} = {expr};")
+            else:
+                _, i, f1, f2, e1, ne, nf = rec
+                x = XYZ[i]
+                uses.update({f"QC{x}", f"WQ{x}"})
+                expr = f"QC{x} * {vname(e, f1, m)} + WQ{x} * {vname(e, f1, m + 1)}"
+                if e1 >= 0:
+                    uses.add("oo2s")
+                    coef = "oo2s" if ne == 1 else f"({float(ne)} * oo2s)"
+                    expr += f" + {coef} * {vname(e1, f1, m + 1)}"
+                if f2 >= 0:
+                    uses.update({"oo2e", "zeta_s"})
+                    coef = "oo2e" if nf == 1 else f"({float(nf)} * oo2e)"
+                    expr += f" + {coef} * ({vname(e, f2, m)} - zeta_s * {vname(e, f2, m + 1)})"
+                body.append(f"    const double {vname(e, f, m)} = {expr};")
+    for x in XYZ:
+        X = x
+        if f"PA{x}" in uses:
+            w(f"    const double PA{x} = bp.PA{X};")
+        if f"WP{x}" in uses:
+            w(f"    const double WP{x} = -eta * inv_s * PQ{X};")
+        if f"QC{x}" in uses:
+            w(f"    const double QC{x} = kp.PA{X};")
+        if f"WQ{x}" in uses:
+            w(f"    const double WQ{x} = zeta * inv_s * PQ{X};")
+    if "oo2z" in uses:
+        w("    const double oo2z = bp.oo2z;")
+    if "oo2e" in uses:
+        w("    const double oo2e = kp.oo2z;")
+    if "oo2s" in uses:
+        w("    const double oo2s = 0.5 * inv_s;")
+    if "eta_s" in uses:
+        w("    const double eta_s = eta * inv_s;")
+    if "zeta_s" in uses:
+        w("    const double zeta_s = zeta * inv_s;")
+    L.extend(body)
+    for (e, f) in g.targets:
+        w(f"    acc{e}_{f} += {vname(e, f, 0)};")
+    w("  }")
+    # cross-lane reduction within the lane group (fixed butterfly: deterministic)
+    w("  for (int s = G >> 1; s > 0; s >>= 1) {")
+    for (e, f) in g.targets:
+        w(f"    acc{e}_{f} += __shfl_xor(acc{e}_{f}, s, 64);")
+    w("  }")
+    # HRR
+    if lb or ld:
+        w("  const PairGeom PG = args.geom[qc.bra];")
+        w("  const PairGeom QG = args.geom[qc.ket];")
+    for n in h.nodes:
+        kind, s1, s2, j = h.deps(n)
+        a, b, cc, d = n
+        nm = f"h{a}_{b}_{cc}_{d}"
+        if kind == "acc":
+            w(f"  const double {nm} = acc{a}_{cc};")
+        elif kind == "bra":
+            w(f"  const double {nm} = h{s1[0]}_{s1[1]}_{s1[2]}_{s1[3]} + PG.AB{XYZ[j]} * h{s2[0]}_{s2[1]}_{s2[2]}_{s2[3]};")
+        else:
+            w(f"  const double {nm} = h{s1[0]}_{s1[1]}_{s1[2]}_{s1[3]} + QG.AB{XYZ[j]} * h{s2[0]}_{s2[1]}_{s2[2]}_{s2[3]};")
+    # scales
+    w(f"  double sA[{ncart(la)}], sB[{ncart(lb)}], sC[{ncart(lc)}], sD[{ncart(ld)}];")
+    w(f"  load_scales<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, P, Q, sA, sB, sC, sD);")
+    w("  const ScatterCtx sc = make_scatter(args, qc, P, Q);")
+    idx = 0
+    for ma, a in enumerate(shell_aos(la)):
+        for mb, b in enumerate(shell_aos(lb)):
+            for mc, cc in enumerate(shell_aos(lc)):
+                for md, d in enumerate(shell_aos(ld)):
+                    w(f"  scatter_one(sc, {idx}, {ma}, {mb}, {mc}, {md}, "
+                      f"h{a}_{b}_{cc}_{d} * (sA[{ma}] * sB[{mb}]) * (sC[{mc}] * sD[{md}]));")
+                    idx += 1
+    w("}")
+    return "\n".join(L), g, h
+
+
+# ---------------------------------------------------------------- W tables
+
+
+def w_tables(c):
+    """Op tables for the wave-cooperative kernel.  Returns (desc, vitems,
+    accmap, hitems, outs)."""
+    la, lb, lc, ld = c
+    g = VrrGraph(*c)
+    h = HrrGraph(*c)
+    # VRR LDS layout: contiguous m per (e,f)
+    base = {}
+    off = 0
+    for ef in g.nodes:
+        base[ef] = off
+        off += g.maxm[ef] + 1
+    nvrr = off
+    # levels
+    levels = sorted(set(g.level(*ef) for ef in g.nodes if ef != (0, 0)))
+    vitems = []
+    vlev = []
+    for lev in levels:
+        vlev.append(len(vitems))
+        for ef in g.nodes:
+            if ef == (0, 0) or g.level(*ef) != lev:
+                continue
+            e, f = ef
+            rec = g.recurrence(e, f)
+            for m in range(g.maxm[ef] + 1):
+                t = base[ef] + m
+                if rec[0] == "A":
+                    _, i, e1, e2, n1 = rec
+                    s1 = base[(e1, 0)] + m
+                    has2 = e2 >= 0
+                    s2 = base[(e2, 0)] + m if has2 else 0
+                    vitems.append((t | (s1 << 16), s2, i | (n1 << 2) | (1 << 14) | (int(has2) << 16), 0))
+                else:
+                    _, i, f1, f2, e1, ne, nf = rec
+                    s1 = base[(e, f1)] + m
+                    has3 = e1 >= 0
+                    s3 = base[(e1, f1)] + m if has3 else 0
+                    has2 = f2 >= 0
+                    s2 = base[(e, f2)] + m if has2 else 0
+                    vitems.append((t | (s1 << 16), s2 | (s3 << 16),
+                                   i | (ne << 2) | (nf << 8) | (2 << 14) | (int(has2) << 16) | (int(has3) << 17), 0))
+    vlev.append(len(vitems))
+    # HRR layout
+    hbase = {}
+    hoff = 0
+    for n in h.nodes:
+        hbase[n] = hoff
+        hoff += 1
+    nhrr = hoff
+    accmap = []
+    for n in h.nodes:
+        if h.deps(n)[0] == "acc":
+            a, b, cc, d = n
+            accmap.append((base[(a, cc)], hbase[n]))
+    hlevels = sorted(set(h.level(n) for n in h.nodes if h.level(n) > 0))
+    hitems = []
+    hlev = []
+    for lev in hlevels:
+        hlev.append(len(hitems))
+        for n in h.nodes:
+            if h.level(n) != lev:
+                continue
+            kind, s1, s2, j = h.deps(n)
+            hitems.append((hbase[n] | (hbase[s1] << 16), hbase[s2] | (j << 16) | ((1 if kind == "ket" else 0) << 18)))
+    hlev.append(len(hitems))
+    outs = []
+    for ma, a in enumerate(shell_aos(la)):
+        for mb, b in enumerate(shell_aos(lb)):
+            for mc, cc in enumerate(shell_aos(lc)):
+                for md, d in enumerate(shell_aos(ld)):
+                    outs.append(hbase[(a, b, cc, d)] | (ma << 16) | (mb << 20) | (mc << 24) | (md << 28))
+    desc = dict(la=la, lb=lb, lc=lc, ld=ld, mmax=g.mmax, nvrr=nvrr, nhrr=nhrr, base00=base[(0, 0)],
+                vlev=vlev, hlev=hlev, nacc=len(accmap), nout=len(outs))
+    return desc, vitems, accmap, hitems, outs
+
+
+# ---------------------------------------------------------------- driver
+
+# Classes compiled as straight-line register kernels.  The rest use the
+# table-driven wave-cooperative kernel.  Chosen by contracted-accumulator count
+# and VRR size so the register kernels stay within the VGPR budget.
+def is_register_class(c):
+    g = VrrGraph(*c)
+    return len(g.targets) <= 100 and g.count_entries() <= 320
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--lmax", type=int, default=LMAX_SHELL)
+    args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+    classes = quartet_classes(args.lmax)
+    rcls = [c for c in classes if is_register_class(c)]
+    # register kernels
+    parts = ["// Generated by gen_hgp.py -- do not edit.", "#pragma once", ""]
+    for c in rcls:
+        code, _, _ = emit_rkernel(c)
+        parts.append(code)
+        parts.append("")
+    # dispatch table (class id -> register kernel or null)
+    parts.append("static const void* const kRegisterKernels[] = {")
+    for c in classes:
+        parts.append(f"  {'(const void*)hgp_r_' + cname(c) if c in rcls else 'nullptr'},  // {cname(c)}")
+    parts.append("};")
+    with open(os.path.join(args.out, "hgp_rkernels.inc"), "w") as f:
+        f.write("\n".join(parts) + "\n")
+    # W tables
+    descs, V, A, H, O = [], [], [], [], []
+    for c in classes:
+        d, vi, am, hi, ou = w_tables(c)
+        d["voff"], d["aoff"], d["hoff"], d["ooff"] = len(V), len(A), len(H), len(O)
+        V += vi
+        A += am
+        H += hi
+        O += ou
+        descs.append(d)
+    maxlev = max(len(d["vlev"]) for d in descs)
+    maxhlev = max(len(d["hlev"]) for d in descs)
+    t = ["// Generated by gen_hgp.py -- do not edit.", "#pragma once", ""]
+    t.append(f"#define HGP_LMAX_SHELL {args.lmax}")
+    t.append(f"#define HGP_NCLASS {len(classes)}")
+    t.append(f"#define HGP_MAX_VLEV {maxlev}")
+    t.append(f"#define HGP_MAX_HLEV {maxhlev}")
+    t.append("static const WClassDesc kWClassDesc[HGP_NCLASS] = {")
+    for c, d in zip(classes, descs):
+        vl = list(d["vlev"]) + [d["vlev"][-1]] * (maxlev - len(d["vlev"]))
+        hl = list(d["hlev"]) + [d["hlev"][-1]] * (maxhlev - len(d["hlev"]))
+        t.append(f"  {{{d['la']}, {d['lb']}, {d['lc']}, {d['ld']}, {d['mmax']}, {d['nvrr']}, {d['nhrr']}, "
+                 f"{d['base00']}, {len(d['vlev']) - 1}, {len(d['hlev']) - 1}, {d['nacc']}, {d['nout']}, "
+                 f"{d['voff']}, {d['aoff']}, {d['hoff']}, {d['ooff']}, "
+                 f"{{{', '.join(map(str, vl))}}}, {{{', '.join(map(str, hl))}}}, "
+                 f"{1 if c in rcls else 0}}},  // {cname(c)}")
+    t.append("};")
+
+    def arr(name, typ, rows, width):
+        t.append(f"static const {typ} {name}[] = {{")
+        line = []
+        for r in rows:
+            if width == 1:
+                line.append(f"{r}u")
+            else:
+                line.append("{" + ", ".join(f"{x}u" for x in r) + "}")
+            if len(line) == 8:
+                t.append("  " + ", ".join(line) + ",")
+                line = []
+        if line:
+            t.append("  " + ", ".join(line) + ",")
+        if not rows:
+            t.append("  " + ("0u" if width == 1 else "{" + ", ".join(["0u"] * width) + "}"))
+        t.append("};")
+
+    arr("kWVItems", "uint4_t", V, 4)
+    arr("kWAccMap", "uint2_t", A, 2)
+    arr("kWHItems", "uint2_t", H, 2)
+    arr("kWOut", "uint32_t", O, 1)
+    t.append(f"static const int kWVItemsN = {len(V)};")
+    t.append(f"static const int kWAccMapN = {len(A)};")
+    t.append(f"static const int kWHItemsN = {len(H)};")
+    t.append(f"static const int kWOutN = {len(O)};")
+    with open(os.path.join(args.out, "hgp_wtables.inc"), "w") as f:
+        f.write("\n".join(t) + "\n")
+    print(f"gen_hgp: {len(classes)} classes, {len(rcls)} register kernels: "
+          + " ".join(cname(c) for c in rcls))
+
+
+if __name__ == "__main__":
+    main()

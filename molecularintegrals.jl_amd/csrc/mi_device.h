@@ -1,0 +1,248 @@
+// mi_device.h -- data layout and device helpers of the HGP ERI engine (gfx950).
+//
+// HBM layout (one plan = one basis):
+//   PairTopo[npair]   shell-pair records sorted by (pair class, contraction
+//                     bucket); a pair's shell A has L_A >= L_B.
+//   PairGeom[npair]   AB = A - B, written by the pair kernel.
+//   PrimPair[npp]     per primitive pair: P, zeta, PA = P - A, the folded
+//                     prefactor K and 1/(2 zeta); 96-byte rows so one lane
+//                     gathers a whole record with six 16-byte loads.
+//   boys[mtop][481][8] Boys grid: row k (T_k = k/16) holds F_{mtop+j}(T_k)/j!,
+//                     j = 0..6, and exp(-T_k).
+//   out[]             caller-owned unique ERI array (iindex or chrr order).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BOYS_NROW 481          // T_k = k/16, k = 0..480 covers T < 30
+#define BOYS_TMAX 30.0         // src/Boys.jl:19 (Tmax = 30)
+#define BOYS_MTOP_MAX 16       // top m of the largest class supported by the table
+#define BOYS_MCOL 23           // F_m columns computed on the host (m = 0..22)
+
+struct uint4_t { uint32_t x, y, z, w; };
+struct uint2_t { uint32_t x, y; };
+
+struct PairTopo {
+  int32_t shA, shB;   // canonical shells (L_A >= L_B)
+  int32_t bfA, bfB;   // first function of each shell (0-based)
+  int32_t ppoff, npp; // primitive-pair range [ppoff, ppoff+npp)
+  int32_t nprimB;     // primitives of shell B (row length of the pair's prim block)
+  int32_t pidx;       // canonical pair index tri(I,J) of the original shells
+};
+
+struct PairGeom {
+  double ABx, ABy, ABz, Q;  // Q: Schwarz bound sqrt(max |(ab|ab)|), 0 until computed
+};
+
+struct __attribute__((aligned(16))) PrimPair {
+  double Px, Py, Pz, zeta;
+  double PAx, PAy, PAz, K;   // K = sqrt(2) pi^{5/4} c'_a c'_b exp(-ab|AB|^2/zeta) / zeta
+  double oo2z, pad0, pad1, pad2;
+};
+
+struct TaskDesc {
+  int32_t braStart, nBra, ketStart, nKet;
+  int32_t same;        // bra list == ket list: triangular quartet space
+  int32_t G;           // lanes per quartet (power of two, <= 64)
+  int32_t blockStart;  // first block of this task in its launch
+  int32_t cls;         // quartet class id
+  int64_t nq;          // quartets in the task
+  int64_t pad;
+};
+
+struct EriArgs {
+  const PairTopo* topo;
+  const PairGeom* geom;
+  const PrimPair* pp;
+  const double* bf_scale;
+  const double* boys;
+  double* out;
+  const TaskDesc* tasks;
+  int64_t M;           // nbf(nbf+1)/2
+  int32_t order;       // 0 iindex, 1 chrr
+  int32_t ntasks;
+};
+
+struct QuartetCtx {
+  int bra, ket;
+  int G, g;
+  bool same_pair;
+};
+
+struct ScatterCtx {
+  double* out;
+  int64_t M;
+  int order;
+  int fa0, fb0, fc0, fd0;
+  bool braSame, ketSame, quartetSame;
+  int gmask, g;
+};
+
+// Wave-cooperative class descriptor (tables generated by gen_hgp.py).
+struct WClassDesc {
+  int la, lb, lc, ld, mmax;
+  int nvrr, nhrr, base00;
+  int nvlev, nhlev, nacc, nout;
+  int voff, aoff, hoff, ooff;
+  int vlev[16];
+  int hlev[8];
+  int is_register;
+};
+
+// ---------------------------------------------------------------------------
+// Boys function F_m(T), m = 0..MMAX-1 (reference: farray, src/Boys.jl:19-27).
+// T < 30: the top order from a 7-term Taylor expansion about the nearest grid
+// point T_k = k/16 (|T - T_k| <= 1/32, truncation < 6e-15 relative), exp(-T) =
+// exp(-T_k) * exp(T_k - T) by a degree-8 series, then the downward recursion
+// F_m = (2T F_{m+1} + e^{-T}) / (2m+1).  T >= 30: the asymptotic form without
+// the e^{-T} term, as the reference's branch: F_0 = sqrt(pi/(4T)),
+// F_{m+1} = F_m (2m+1)/(2T).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double exp_small(double d) {
+  // e^d for |d| <= 1/32: Taylor to d^8 (remainder < 1e-19)
+  double r = 1.0 / 40320.0;
+  r = fma(r, d, 1.0 / 5040.0);
+  r = fma(r, d, 1.0 / 720.0);
+  r = fma(r, d, 1.0 / 120.0);
+  r = fma(r, d, 1.0 / 24.0);
+  r = fma(r, d, 1.0 / 6.0);
+  r = fma(r, d, 0.5);
+  r = fma(r, d, 1.0);
+  r = fma(r, d, 1.0);
+  return r;
+}
+
+template <int MMAX>
+__device__ __forceinline__ void boys_eval(double T, const double* __restrict__ tab, double (&F)[MMAX]) {
+  if (T < BOYS_TMAX) {
+    const int k = (int)(T * 16.0 + 0.5);
+    const double d = (double)k * 0.0625 - T;
+    const double* r = tab + k * 8;
+    double f = r[6];
+    f = fma(f, d, r[5]);
+    f = fma(f, d, r[4]);
+    f = fma(f, d, r[3]);
+    f = fma(f, d, r[2]);
+    f = fma(f, d, r[1]);
+    f = fma(f, d, r[0]);
+    F[MMAX - 1] = f;
+    if (MMAX > 1) {
+      const double eT = r[7] * exp_small(d);
+      const double t2 = T + T;
+#pragma unroll
+      for (int m = MMAX - 2; m >= 0; --m) F[m] = fma(t2, F[m + 1], eT) * (1.0 / (2 * m + 1));
+    }
+  } else {
+    const double rT = 1.0 / T;
+    F[0] = 0.88622692545275801365 * sqrt(rT);  // sqrt(pi)/2 * T^{-1/2}
+    const double o2T = 0.5 * rT;
+#pragma unroll
+    for (int m = 1; m < MMAX; ++m) F[m] = F[m - 1] * ((2 * m - 1) * o2T);
+  }
+}
+
+// Copy the Boys slice of one top order into LDS (all threads of the block).
+__device__ __forceinline__ void stage_boys_slice(double* lds, const double* __restrict__ boys, int mtop) {
+  const double2* src = reinterpret_cast<const double2*>(boys + (size_t)mtop * BOYS_NROW * 8);
+  double2* dst = reinterpret_cast<double2*>(lds);
+  for (int i = threadIdx.x; i < BOYS_NROW * 4; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Task/quartet decoding.  A launch covers one quartet class; blocks are
+// assigned to tasks by blockStart (tasks sorted by it).  A task pairs two
+// contiguous pair lists; same==1 means the triangular space p <= q.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_task(const EriArgs& a) {
+  int lo = 0, hi = a.ntasks - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (a.tasks[mid].blockStart <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, QuartetCtx& qc) {
+  if (T.same) {
+    int64_t q = (int64_t)((sqrt(8.0 * (double)qi + 1.0) - 1.0) * 0.5);
+    while (q * (q + 1) / 2 > qi) --q;
+    while ((q + 1) * (q + 2) / 2 <= qi) ++q;
+    const int64_t p = qi - q * (q + 1) / 2;
+    qc.bra = T.braStart + (int)p;
+    qc.ket = T.braStart + (int)q;
+  } else {
+    const int64_t p = qi / T.nKet;
+    const int64_t q = qi - p * T.nKet;
+    qc.bra = T.braStart + (int)p;
+    qc.ket = T.ketStart + (int)q;
+  }
+  qc.same_pair = (qc.bra == qc.ket);
+}
+
+// Lane-group decode for the register kernels: G lanes per quartet.
+__device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc) {
+  const TaskDesc T = a.tasks[find_task(a)];
+  const int G = T.G;
+  const int per_block = blockDim.x / G;
+  const int64_t qi = (int64_t)(blockIdx.x - T.blockStart) * per_block + threadIdx.x / G;
+  qc.G = G;
+  qc.g = threadIdx.x & (G - 1);
+  if (qi >= T.nq) return false;
+  decode_index(T, qi, qc);
+  return true;
+}
+
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void load_scales(const EriArgs& a, const PairTopo& P, const PairTopo& Q,
+                                            double (&sA)[NA], double (&sB)[NB], double (&sC)[NC], double (&sD)[ND]) {
+#pragma unroll
+  for (int i = 0; i < NA; ++i) sA[i] = a.bf_scale[P.bfA + i];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) sB[i] = a.bf_scale[P.bfB + i];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) sC[i] = a.bf_scale[Q.bfA + i];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) sD[i] = a.bf_scale[Q.bfB + i];
+}
+
+__device__ __forceinline__ ScatterCtx make_scatter(const EriArgs& a, const QuartetCtx& qc, const PairTopo& P,
+                                                   const PairTopo& Q) {
+  ScatterCtx s;
+  s.out = a.out;
+  s.M = a.M;
+  s.order = a.order;
+  s.fa0 = P.bfA;
+  s.fb0 = P.bfB;
+  s.fc0 = Q.bfA;
+  s.fd0 = Q.bfB;
+  s.braSame = (P.shA == P.shB);
+  s.ketSame = (Q.shA == Q.shB);
+  s.quartetSame = qc.same_pair;
+  s.gmask = qc.G - 1;
+  s.g = qc.g;
+  return s;
+}
+
+// Unique slot of the block element (fa fb|fc fd) or -1 when the element is a
+// duplicate inside this shell quartet.  Slot algebra: src/Utils.jl:14-19 and
+// the enumeration order of src/Basis.jl:203.
+__device__ __forceinline__ int64_t unique_slot(const ScatterCtx& s, int fa, int fb, int fc, int fd) {
+  if (s.braSame && fa < fb) return -1;
+  if (s.ketSame && fc < fd) return -1;
+  const int64_t i = (fa > fb ? fa : fb) + 1, j = (fa > fb ? fb : fa) + 1;
+  const int64_t k = (fc > fd ? fc : fd) + 1, l = (fc > fd ? fd : fc) + 1;
+  const int64_t ij = i * (i - 1) / 2 + j, kl = k * (k - 1) / 2 + l;
+  if (s.quartetSame && ij > kl) return -1;
+  const int64_t lo = ij < kl ? ij : kl, hi = ij < kl ? kl : ij;
+  if (s.order == 0) return hi * (hi - 1) / 2 + lo - 1;
+  return (lo - 1) * (s.M + 1) - lo * (lo - 1) / 2 + hi - lo;
+}
+
+__device__ __forceinline__ void scatter_one(const ScatterCtx& s, int idx, int ma, int mb, int mc, int md,
+                                            double val) {
+  if ((idx & s.gmask) != s.g) return;
+  const int64_t slot = unique_slot(s, s.fa0 + ma, s.fb0 + mb, s.fc0 + mc, s.fd0 + md);
+  if (slot >= 0) s.out[slot] = val;
+}

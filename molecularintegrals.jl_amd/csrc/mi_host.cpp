@@ -1,0 +1,501 @@
+// mi_host.cpp -- host side of libmi_hgp.so: C ABI, plans, Boys grid.
+//
+// Replaces the host half of the reference's HGP driver:
+//   all_twoe_ints_chrr  src/HGP.jl:22-34  (quartet loop, scatter)
+//   eri_fetcher         src/Basis.jl:201-212 (quartet -> slots map; here the
+//                       slot is a closed form evaluated in the kernels)
+// A plan sorts the shell pairs by (pair class, contraction bucket) so that a
+// "task" -- the cross product of two contiguous pair lists -- is one
+// angular-momentum class with similar primitive counts; each class is one
+// kernel launch over a table of tasks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mi_hgp.h"
+#include "mi_kernels.h"
+
+struct mi_ctx {
+  int device = 0;
+  double* d_boys = nullptr;
+  void* wtables = nullptr;
+  std::string err;
+  size_t lds_limit = 0;
+};
+
+namespace {
+
+int set_err(mi_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_err(ctx, MI_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Boys values F_m(T) = e^{-T}/2 sum_n T^n / prod_{k<=n} (m + 1/2 + k)
+// (power series of the lower incomplete gamma function, DLMF 8.7.1), in
+// long double; all terms are positive.
+long double boys_series(int m, long double T) {
+  long double a = m + 0.5L, term = 1.0L / a, sum = term;
+  for (int n = 1; n < 20000; ++n) {
+    term *= T / (a + n);
+    sum += term;
+    if (term < sum * 1e-22L) break;
+  }
+  return 0.5L * expl(-T) * sum;
+}
+
+// Device grid: for each top order mtop, rows k = 0..480 (T_k = k/16) of
+// { F_{mtop+j}(T_k)/j!  (j = 0..6),  exp(-T_k) }.
+std::vector<double> make_boys_table() {
+  std::vector<long double> F((size_t)BOYS_MCOL * BOYS_NROW);
+  for (int k = 0; k < BOYS_NROW; ++k)
+    for (int m = 0; m < BOYS_MCOL; ++m) F[(size_t)m * BOYS_NROW + k] = boys_series(m, (long double)k / 16.0L);
+  std::vector<double> tab((size_t)(BOYS_MTOP_MAX + 1) * BOYS_NROW * 8);
+  for (int mt = 0; mt <= BOYS_MTOP_MAX; ++mt)
+    for (int k = 0; k < BOYS_NROW; ++k) {
+      double* row = &tab[((size_t)mt * BOYS_NROW + k) * 8];
+      long double fact = 1.0L;
+      for (int j = 0; j < 7; ++j) {
+        if (j > 0) fact *= j;
+        row[j] = (double)(F[(size_t)(mt + j) * BOYS_NROW + k] / fact);
+      }
+      row[7] = (double)expl(-(long double)k / 16.0L);
+    }
+  return tab;
+}
+
+int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+int quartet_class(int X, int Y) { return X >= Y ? X * (X + 1) / 2 + Y : Y * (Y + 1) / 2 + X; }
+
+int kbucket(int K) {
+  if (K <= 1) return 0;
+  if (K <= 4) return 1;
+  if (K <= 9) return 2;
+  if (K <= 36) return 3;
+  return 4;
+}
+
+int pow2floor(double x) {
+  int g = 1;
+  while (g * 2 <= x && g < 64) g *= 2;
+  return g;
+}
+
+}  // namespace
+
+struct Launch {
+  int cls;
+  bool reg;
+  int task_off, ntasks, nblocks;
+  int64_t nq_shell, nq_prim;
+};
+
+struct mi_plan {
+  mi_ctx* ctx = nullptr;
+  int nshell = 0, nbf = 0, npair = 0, npp = 0;
+  int64_t nq_shell = 0, nq_prim = 0, n_unique = 0;
+  int order = MI_ORDER_CHRR;
+  // device buffers
+  double* d_shell_xyz = nullptr;
+  int32_t* d_shell_prim_off = nullptr;
+  double* d_prim_exp = nullptr;
+  double* d_prim_cn = nullptr;
+  double* d_bf_scale = nullptr;
+  PairTopo* d_topo = nullptr;
+  PairGeom* d_geom = nullptr;
+  PrimPair* d_pp = nullptr;
+  int32_t* d_pp_pair = nullptr;
+  TaskDesc* d_tasks = nullptr;
+  std::vector<Launch> launches;
+};
+
+namespace {
+
+int validate(mi_ctx* ctx, const mi_basis* b) {
+  if (!b) return set_err(ctx, MI_EINVAL, "basis is NULL");
+  if (b->nshell <= 0) return set_err(ctx, MI_EINVAL, "basis has no shells");
+  if (!b->shell_xyz || !b->shell_l || !b->shell_prim_off || !b->prim_exp || !b->prim_coef || !b->prim_norm ||
+      !b->bf_scale)
+    return set_err(ctx, MI_EINVAL, "basis has a NULL array");
+  if (b->shell_prim_off[0] != 0) return set_err(ctx, MI_EINVAL, "shell_prim_off[0] must be 0");
+  int nbf = 0;
+  for (int s = 0; s < b->nshell; ++s) {
+    const int L = b->shell_l[s];
+    if (L < 0) return set_err(ctx, MI_EINVAL, "shell %d has negative L", s);
+    if (L > mi::lmax_shell())
+      return set_err(ctx, MI_EUNSUPPORTED, "shell %d has L=%d; this build supports L<=%d", s, L, mi::lmax_shell());
+    const int np = b->shell_prim_off[s + 1] - b->shell_prim_off[s];
+    if (np <= 0) return set_err(ctx, MI_EINVAL, "shell %d has no primitives", s);
+    for (int p = b->shell_prim_off[s]; p < b->shell_prim_off[s + 1]; ++p)
+      if (!(b->prim_exp[p] > 0.0) || !std::isfinite(b->prim_exp[p]))
+        return set_err(ctx, MI_EINVAL, "primitive %d has a non-positive exponent", p);
+    nbf += ncart(L);
+  }
+  if (nbf != b->nbf) return set_err(ctx, MI_EINVAL, "nbf=%d but the shells hold %d functions", b->nbf, nbf);
+  return MI_OK;
+}
+
+template <typename T>
+int upload(mi_ctx* ctx, T** dst, const T* src, size_t n) {
+  if (n == 0) n = 1;
+  HIP_TRY(ctx, hipMalloc((void**)dst, sizeof(T) * n));
+  if (src) HIP_TRY(ctx, hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  return MI_OK;
+}
+
+void free_plan_buffers(mi_plan* p) {
+  hipFree(p->d_shell_xyz);
+  hipFree(p->d_shell_prim_off);
+  hipFree(p->d_prim_exp);
+  hipFree(p->d_prim_cn);
+  hipFree(p->d_bf_scale);
+  hipFree(p->d_topo);
+  hipFree(p->d_geom);
+  hipFree(p->d_pp);
+  hipFree(p->d_pp_pair);
+  hipFree(p->d_tasks);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mi_lmax(void) { return mi::lmax_shell(); }
+
+int64_t mi_n_unique(int32_t nbf) {
+  const int64_t n = nbf;
+  return n * (n + 1) * (n * n + n + 2) / 8;
+}
+
+const char* mi_last_error(const mi_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mi_create(int device, mi_ctx** out) {
+  if (!out) return MI_EINVAL;
+  *out = nullptr;
+  mi_ctx* c = new mi_ctx();
+  c->device = device;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0 || device < 0 || device >= ndev) {
+    delete c;
+    return MI_EDEVICE;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return MI_EDEVICE;
+  }
+  std::vector<double> tab = make_boys_table();
+  if (hipMalloc(&c->d_boys, sizeof(double) * tab.size()) != hipSuccess ||
+      hipMemcpy(c->d_boys, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      mi::upload_wtables(&c->wtables) != hipSuccess) {
+    hipFree(c->d_boys);
+    delete c;
+    return MI_EDEVICE;
+  }
+  size_t maxlds = 0;
+  for (int k = 0; k < mi::nclass(); ++k) maxlds = std::max(maxlds, mi::wclass_lds_bytes(k));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->lds_limit = prop.sharedMemPerBlock;
+  if (maxlds > 65536) mi::set_wave_kernel_lds_limit(maxlds);
+  *out = c;
+  return MI_OK;
+}
+
+void mi_destroy(mi_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipFree(ctx->d_boys);
+  mi::free_wtables(ctx->wtables);
+  delete ctx;
+}
+
+int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan** out) {
+  if (!ctx || !out) return MI_EINVAL;
+  *out = nullptr;
+  int rc = validate(ctx, b);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  mi_plan* P = new mi_plan();
+  P->ctx = ctx;
+  P->order = opts ? opts->order : MI_ORDER_CHRR;
+  if (P->order != MI_ORDER_CHRR && P->order != MI_ORDER_IINDEX) {
+    delete P;
+    return set_err(ctx, MI_EINVAL, "unknown order %d", P->order);
+  }
+  const int ns = b->nshell;
+  P->nshell = ns;
+  P->nbf = b->nbf;
+  P->n_unique = mi_n_unique(b->nbf);
+  std::vector<int> bfoff(ns + 1, 0), np(ns);
+  for (int s = 0; s < ns; ++s) {
+    bfoff[s + 1] = bfoff[s] + ncart(b->shell_l[s]);
+    np[s] = b->shell_prim_off[s + 1] - b->shell_prim_off[s];
+  }
+  // Shell pairs (I >= J), canonical shell order L_A >= L_B.
+  struct PairKey {
+    int pc, bucket, pidx, A, B, K;
+  };
+  std::vector<PairKey> keys;
+  keys.reserve((size_t)ns * (ns + 1) / 2);
+  for (int I = 0; I < ns; ++I)
+    for (int J = 0; J <= I; ++J) {
+      int A = I, B = J;
+      if (b->shell_l[J] > b->shell_l[I]) std::swap(A, B);
+      const int K = np[A] * np[B];
+      keys.push_back({pair_class(b->shell_l[A], b->shell_l[B]), kbucket(K), I * (I + 1) / 2 + J, A, B, K});
+    }
+  std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
+    if (x.pc != y.pc) return x.pc < y.pc;
+    if (x.bucket != y.bucket) return x.bucket < y.bucket;
+    return x.pidx < y.pidx;
+  });
+  const int npair = (int)keys.size();
+  std::vector<PairTopo> topo(npair);
+  std::vector<int32_t> pp_pair;
+  int ppoff = 0;
+  for (int r = 0; r < npair; ++r) {
+    const PairKey& k = keys[r];
+    topo[r] = {k.A, k.B, bfoff[k.A], bfoff[k.B], ppoff, k.K, np[k.B], k.pidx};
+    for (int t = 0; t < k.K; ++t) pp_pair.push_back(r);
+    ppoff += k.K;
+  }
+  P->npair = npair;
+  P->npp = ppoff;
+  // Contiguous lists of equal (pc, bucket).
+  struct List {
+    int start, n, pc, Ksum, Kmax;
+  };
+  std::vector<List> lists;
+  for (int r = 0; r < npair;) {
+    int e = r;
+    int Ksum = 0, Kmax = 0;
+    while (e < npair && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket) {
+      Ksum += keys[e].K;
+      Kmax = std::max(Kmax, keys[e].K);
+      ++e;
+    }
+    lists.push_back({r, e - r, keys[r].pc, Ksum, Kmax});
+    r = e;
+  }
+  // Tasks grouped by quartet class.
+  std::vector<std::vector<TaskDesc>> per_class(mi::nclass());
+  std::vector<int64_t> cls_shell(mi::nclass(), 0), cls_prim(mi::nclass(), 0);
+  int64_t nq_shell = 0, nq_prim = 0;
+  for (size_t x = 0; x < lists.size(); ++x)
+    for (size_t y = x; y < lists.size(); ++y) {
+      const List* bra = &lists[x];
+      const List* ket = &lists[y];
+      if (ket->pc > bra->pc) std::swap(bra, ket);
+      TaskDesc t{};
+      t.braStart = bra->start;
+      t.nBra = bra->n;
+      t.ketStart = ket->start;
+      t.nKet = ket->n;
+      t.same = (x == y);
+      t.cls = quartet_class(bra->pc, ket->pc);
+      t.nq = t.same ? (int64_t)bra->n * (bra->n + 1) / 2 : (int64_t)bra->n * ket->n;
+      const double kavg = ((double)bra->Ksum / bra->n) * ((double)ket->Ksum / ket->n);
+      int G = pow2floor(std::max(1.0, kavg / 4.0));
+      const WClassDesc& D = mi::wclass(t.cls);
+      if (D.nacc > 16) G = std::min(G, 4);
+      t.G = G;
+      int64_t tprim;
+      if (t.same) {
+        // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
+        int64_t s = 0, s2 = 0;
+        for (int r = bra->start; r < bra->start + bra->n; ++r) {
+          s += keys[r].K;
+          s2 += (int64_t)keys[r].K * keys[r].K;
+        }
+        tprim = (s * s + s2) / 2;
+      } else {
+        tprim = (int64_t)bra->Ksum * ket->Ksum;
+      }
+      nq_shell += t.nq;
+      nq_prim += tprim;
+      cls_shell[t.cls] += t.nq;
+      cls_prim[t.cls] += tprim;
+      per_class[t.cls].push_back(t);
+    }
+  P->nq_shell = nq_shell;
+  P->nq_prim = nq_prim;
+  std::vector<TaskDesc> all_tasks;
+  for (int c = 0; c < mi::nclass(); ++c) {
+    if (per_class[c].empty()) continue;
+    Launch L;
+    L.cls = c;
+    L.reg = mi::has_register_kernel(c);
+    L.task_off = (int)all_tasks.size();
+    L.ntasks = (int)per_class[c].size();
+    int blocks = 0;
+    for (TaskDesc& t : per_class[c]) {
+      t.blockStart = blocks;
+      const int64_t nb = L.reg ? (t.nq * t.G + 255) / 256 : t.nq;
+      blocks += (int)nb;
+      all_tasks.push_back(t);
+    }
+    L.nblocks = blocks;
+    L.nq_shell = cls_shell[c];
+    L.nq_prim = cls_prim[c];
+    P->launches.push_back(L);
+  }
+  // Device buffers.
+  std::vector<double> cn(b->shell_prim_off[ns]);
+  for (int p = 0; p < b->shell_prim_off[ns]; ++p) cn[p] = b->prim_coef[p] * b->prim_norm[p];
+  const int nprim = b->shell_prim_off[ns];
+  if ((rc = upload(ctx, &P->d_shell_xyz, b->shell_xyz, (size_t)3 * ns)) ||
+      (rc = upload(ctx, &P->d_shell_prim_off, b->shell_prim_off, (size_t)ns + 1)) ||
+      (rc = upload(ctx, &P->d_prim_exp, b->prim_exp, (size_t)nprim)) ||
+      (rc = upload(ctx, &P->d_prim_cn, cn.data(), (size_t)nprim)) ||
+      (rc = upload(ctx, &P->d_bf_scale, b->bf_scale, (size_t)b->nbf)) ||
+      (rc = upload(ctx, &P->d_topo, topo.data(), (size_t)npair)) ||
+      (rc = upload<PairGeom>(ctx, &P->d_geom, nullptr, (size_t)npair)) ||
+      (rc = upload<PrimPair>(ctx, &P->d_pp, nullptr, (size_t)P->npp)) ||
+      (rc = upload(ctx, &P->d_pp_pair, pp_pair.data(), pp_pair.size())) ||
+      (rc = upload(ctx, &P->d_tasks, all_tasks.data(), all_tasks.size()))) {
+    free_plan_buffers(P);
+    delete P;
+    return rc;
+  }
+  *out = P;
+  return MI_OK;
+}
+
+static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev) {
+  if (!P) return MI_EINVAL;
+  mi_ctx* ctx = P->ctx;
+  if (!out) return set_err(ctx, MI_EINVAL, "output pointer is NULL");
+  if (out_len != P->n_unique)
+    return set_err(ctx, MI_ESIZE, "output length %lld, basis needs %lld", (long long)out_len,
+                   (long long)P->n_unique);
+  hipStream_t s = (hipStream_t)stream;
+  if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], s));
+  HIP_TRY(ctx, mi::launch_pair_prims(P->npp, P->d_pp_pair, P->d_topo, P->d_shell_xyz, P->d_shell_prim_off,
+                                     P->d_prim_exp, P->d_prim_cn, P->d_pp, P->d_geom, s));
+  if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], s));
+  EriArgs a{};
+  a.topo = P->d_topo;
+  a.geom = P->d_geom;
+  a.pp = P->d_pp;
+  a.bf_scale = P->d_bf_scale;
+  a.boys = ctx->d_boys;
+  a.out = out;
+  a.M = (int64_t)P->nbf * (P->nbf + 1) / 2;
+  a.order = P->order;
+  for (size_t i = 0; i < P->launches.size(); ++i) {
+    const Launch& L = P->launches[i];
+    a.tasks = P->d_tasks + L.task_off;
+    a.ntasks = L.ntasks;
+    if (L.reg)
+      HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, s));
+    else
+      HIP_TRY(ctx, mi::launch_class_wave(L.cls, a, L.nblocks, ctx->wtables, s));
+    if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
+  }
+  return MI_OK;
+}
+
+int mi_plan_execute(mi_plan* P, double* out, int64_t out_len, void* stream) {
+  return plan_run(P, out, out_len, stream, nullptr);
+}
+
+int mi_plan_nlaunch(const mi_plan* P) { return P ? (int)P->launches.size() : -1; }
+
+int mi_plan_launch_info(const mi_plan* P, int32_t i, int32_t* cls4, int64_t* counts3) {
+  if (!P || i < 0 || i >= (int)P->launches.size() || !cls4 || !counts3) return MI_EINVAL;
+  const Launch& L = P->launches[i];
+  const WClassDesc& D = mi::wclass(L.cls);
+  cls4[0] = D.la;
+  cls4[1] = D.lb;
+  cls4[2] = D.lc;
+  cls4[3] = D.ld;
+  counts3[0] = L.nq_shell;
+  counts3[1] = L.nq_prim;
+  counts3[2] = L.nblocks;
+  return L.reg ? 1 : 0;
+}
+
+int mi_plan_profile(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms) {
+  if (!P || !ms) return MI_EINVAL;
+  mi_ctx* ctx = P->ctx;
+  const size_t nev = P->launches.size() + 2;
+  std::vector<hipEvent_t> ev(nev, nullptr);
+  for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
+  int rc = plan_run(P, out, out_len, stream, ev.data());
+  if (rc == MI_OK) {
+    hipError_t e = hipEventSynchronize(ev[nev - 1]);
+    if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "event sync failed: %s", hipGetErrorString(e));
+    for (size_t i = 0; rc == MI_OK && i + 1 < nev; ++i) {
+      e = hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+      if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "elapsed time failed: %s", hipGetErrorString(e));
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  return rc;
+}
+
+void mi_plan_destroy(mi_plan* P) {
+  if (!P) return;
+  hipSetDevice(P->ctx->device);
+  free_plan_buffers(P);
+  delete P;
+}
+
+int mi_plan_stats(const mi_plan* P, int64_t* st, int32_t n) {
+  if (!P || !st) return MI_EINVAL;
+  const int64_t v[7] = {P->nbf, P->npair, P->npp, P->nq_shell, P->nq_prim, P->n_unique,
+                        (int64_t)P->launches.size() + 1};
+  for (int i = 0; i < n && i < 7; ++i) st[i] = v[i];
+  return MI_OK;
+}
+
+int mi_eri_unique(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, double* out, int64_t out_len) {
+  if (!ctx) return MI_EINVAL;
+  if (!out) return set_err(ctx, MI_EINVAL, "output pointer is NULL");
+  int rc = validate(ctx, b);
+  if (rc) return rc;
+  if (out_len != mi_n_unique(b->nbf))
+    return set_err(ctx, MI_ESIZE, "output length %lld, basis needs %lld", (long long)out_len,
+                   (long long)mi_n_unique(b->nbf));
+  mi_plan* P = nullptr;
+  if ((rc = mi_plan_create(ctx, b, opts, &P))) return rc;
+  double* d_out = nullptr;
+  hipStream_t s = opts ? (hipStream_t)opts->stream : nullptr;
+  hipError_t e = hipMalloc(&d_out, sizeof(double) * (size_t)out_len);
+  if (e != hipSuccess) {
+    mi_plan_destroy(P);
+    return set_err(ctx, MI_ENOMEM, "hipMalloc of %lld doubles failed: %s", (long long)out_len, hipGetErrorString(e));
+  }
+  rc = mi_plan_execute(P, d_out, out_len, s);
+  if (rc == MI_OK) {
+    e = hipMemcpyAsync(out, d_out, sizeof(double) * (size_t)out_len, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "copy back failed: %s", hipGetErrorString(e));
+  }
+  hipFree(d_out);
+  mi_plan_destroy(P);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,25 @@
+// mi_kernels.h -- host-side entry points into mi_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mi_device.h"
+
+namespace mi {
+
+int nclass();
+int lmax_shell();
+const WClassDesc& wclass(int c);
+bool has_register_kernel(int c);
+
+hipError_t upload_wtables(void** handle);
+void free_wtables(void* handle);
+
+hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const PairTopo* topo, const double* shell_xyz,
+                             const int32_t* shell_prim_off, const double* prim_exp, const double* prim_cn,
+                             PrimPair* pp, PairGeom* geom, hipStream_t stream);
+hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStream_t stream);
+hipError_t launch_class_wave(int c, const EriArgs& args, int nblocks, void* wtables, hipStream_t stream);
+size_t wclass_lds_bytes(int c);
+hipError_t set_wave_kernel_lds_limit(size_t bytes);
+
+}  // namespace mi

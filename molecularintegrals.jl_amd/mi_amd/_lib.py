@@ -1,0 +1,110 @@
+"""ctypes binding of libmi_hgp.so (declarations: include/mi_hgp.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("MI_HGP_LIB", os.path.join(PKG_ROOT, "libmi_hgp.so"))
+
+MI_OK, MI_EINVAL, MI_EUNSUPPORTED, MI_EDEVICE, MI_ENOMEM, MI_ESIZE = range(6)
+MI_ORDER_IINDEX, MI_ORDER_CHRR = 0, 1
+MI_PREC_FP64, MI_PREC_MIXED = 0, 1
+
+# Every symbol include/mi_hgp.h declares (checked by the CPU test suite).
+EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique", "mi_eri_unique",
+            "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
+            "mi_plan_launch_info", "mi_plan_profile"]
+
+
+class MiBasis(ctypes.Structure):
+    _fields_ = [("nshell", ctypes.c_int32),
+                ("shell_xyz", ctypes.POINTER(ctypes.c_double)),
+                ("shell_l", ctypes.POINTER(ctypes.c_int32)),
+                ("shell_prim_off", ctypes.POINTER(ctypes.c_int32)),
+                ("prim_exp", ctypes.POINTER(ctypes.c_double)),
+                ("prim_coef", ctypes.POINTER(ctypes.c_double)),
+                ("prim_norm", ctypes.POINTER(ctypes.c_double)),
+                ("nbf", ctypes.c_int32),
+                ("bf_scale", ctypes.POINTER(ctypes.c_double))]
+
+
+class MiOpts(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int32),
+                ("precision", ctypes.c_int32),
+                ("tau", ctypes.c_double),
+                ("schwarz_cut", ctypes.c_double),
+                ("stream", ctypes.c_void_p)]
+
+
+class MiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libmi_hgp error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libmi_hgp.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libmi_hgp.so not found at {LIB_PATH}; build it with "
+                           f"python molecularintegrals.jl_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.mi_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.mi_create.restype = ctypes.c_int
+    L.mi_destroy.argtypes = [vp]
+    L.mi_destroy.restype = None
+    L.mi_last_error.argtypes = [vp]
+    L.mi_last_error.restype = ctypes.c_char_p
+    L.mi_lmax.argtypes = []
+    L.mi_lmax.restype = ctypes.c_int32
+    L.mi_n_unique.argtypes = [ctypes.c_int32]
+    L.mi_n_unique.restype = ctypes.c_int64
+    L.mi_eri_unique.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts),
+                                ctypes.POINTER(ctypes.c_double), ctypes.c_int64]
+    L.mi_eri_unique.restype = ctypes.c_int
+    L.mi_plan_create.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts), ctypes.POINTER(vp)]
+    L.mi_plan_create.restype = ctypes.c_int
+    L.mi_plan_execute.argtypes = [vp, vp, ctypes.c_int64, vp]
+    L.mi_plan_execute.restype = ctypes.c_int
+    L.mi_plan_destroy.argtypes = [vp]
+    L.mi_plan_destroy.restype = None
+    L.mi_plan_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
+    L.mi_plan_stats.restype = ctypes.c_int
+    L.mi_plan_nlaunch.argtypes = [vp]
+    L.mi_plan_nlaunch.restype = ctypes.c_int
+    L.mi_plan_launch_info.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int64)]
+    L.mi_plan_launch_info.restype = ctypes.c_int
+    L.mi_plan_profile.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.POINTER(ctypes.c_float)]
+    L.mi_plan_profile.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+class BasisArgs:
+    """Keeps the numpy arrays alive while the C struct points into them."""
+
+    def __init__(self, fb):
+        arrs = fb.arrays()
+        self._keep = arrs
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        self.struct = MiBasis(fb.nshell, dp(arrs[0]), ip(arrs[1]), ip(arrs[2]), dp(arrs[3]), dp(arrs[4]),
+                              dp(arrs[5]), fb.nbf, dp(arrs[6]))
+
+
+def opts(order=MI_ORDER_CHRR, precision=MI_PREC_FP64, tau=0.0, schwarz_cut=0.0, stream=None):
+    return MiOpts(order, precision, tau, schwarz_cut, stream)

@@ -1,0 +1,146 @@
+"""Drop-in replacements for the reference's ERI drivers, running on the GPU.
+
+* ``all_twoe_ints_chrr(bfs)`` -- rpmuller/MolecularIntegrals.jl src/HGP.jl:22-34.
+  Same argument (a ``Basis``), same return (1-d float64 array of length
+  n(n+1)(n^2+n+2)/8), same slot order (enumeration of ``iiterator``,
+  src/Basis.jl:203).  Values are the normalised contracted integrals
+  ``coulomb(bfs[i],bfs[j],bfs[k],bfs[l])`` (src/ERI.jl:102).
+* ``all_twoe_ints(bfs)`` -- src/ERI.jl:104-112, slot = ``iindex(i,j,k,l)``.
+
+Both call the C ABI (include/mi_hgp.h) through ``Context``; errors surface as
+``MiError`` (the reference raises Julia exceptions, e.g. a BoundsError for
+L beyond its tables).
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _lib
+from .basis import Basis, FlatBasis, flatten
+
+
+class Context:
+    """One device context (``mi_create``).  Not shared across threads."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.L.mi_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.MiError(rc, f"mi_create(device={device}) failed (no usable GPU?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            raise _lib.MiError(rc, self.L.mi_last_error(self.h).decode())
+
+    def eri_unique(self, fb: FlatBasis, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
+                   schwarz_cut=0.0) -> np.ndarray:
+        args = _lib.BasisArgs(fb)
+        n = int(self.L.mi_n_unique(fb.nbf))
+        out = np.empty(n, dtype=np.float64)
+        o = _lib.opts(order, precision, tau, schwarz_cut)
+        self.check(self.L.mi_eri_unique(self.h, ctypes.byref(args.struct), ctypes.byref(o),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        return out
+
+    def plan(self, fb: FlatBasis, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
+             schwarz_cut=0.0) -> "Plan":
+        return Plan(self, fb, order, precision, tau, schwarz_cut)
+
+
+class Plan:
+    """``mi_plan_create``: basis topology resident on the device; ``execute``
+    writes all unique ERIs into a caller-owned device buffer (e.g. a torch
+    tensor's ``data_ptr()``) on a given HIP stream."""
+
+    def __init__(self, ctx: Context, fb: FlatBasis, order, precision, tau, schwarz_cut):
+        self.ctx = ctx
+        self.fb = fb
+        self._args = _lib.BasisArgs(fb)
+        o = _lib.opts(order, precision, tau, schwarz_cut)
+        h = ctypes.c_void_p()
+        ctx.check(ctx.L.mi_plan_create(ctx.h, ctypes.byref(self._args.struct), ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self.n_unique = int(ctx.L.mi_n_unique(fb.nbf))
+
+    def execute(self, out_ptr: int, stream: int = 0):
+        self.ctx.check(self.ctx.L.mi_plan_execute(self.h, ctypes.c_void_p(out_ptr), self.n_unique,
+                                                  ctypes.c_void_p(stream)))
+
+    def stats(self) -> dict:
+        st = (ctypes.c_int64 * 7)()
+        self.ctx.check(self.ctx.L.mi_plan_stats(self.h, st, 7))
+        keys = ["nbf", "shell_pairs", "primitive_pairs", "shell_quartets", "primitive_quartets", "unique_eris",
+                "launches"]
+        return dict(zip(keys, [int(x) for x in st]))
+
+    def launches(self) -> list:
+        """Per class launch: ((la,lb,lc,ld), shell quartets, primitive quartets,
+        blocks, register-kernel flag)."""
+        out = []
+        n = self.ctx.L.mi_plan_nlaunch(self.h)
+        for i in range(n):
+            c4 = (ctypes.c_int32 * 4)()
+            n3 = (ctypes.c_int64 * 3)()
+            r = self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3)
+            out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), bool(r == 1)))
+        return out
+
+    def profile(self, out_ptr: int, stream: int = 0) -> list:
+        """Per-launch durations (ms) of one execute: [pair kernel, class 0, ...]."""
+        n = self.ctx.L.mi_plan_nlaunch(self.h) + 1
+        ms = (ctypes.c_float * n)()
+        self.ctx.check(self.ctx.L.mi_plan_profile(self.h, ctypes.c_void_p(out_ptr), self.n_unique,
+                                                  ctypes.c_void_p(stream), ms))
+        return [float(x) for x in ms]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.mi_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_tls = threading.local()
+
+
+def default_context() -> Context:
+    ctx = getattr(_tls, "ctx", None)
+    if ctx is None:
+        ctx = Context(0)
+        _tls.ctx = ctx
+    return ctx
+
+
+def _flat(bfs) -> FlatBasis:
+    return bfs if isinstance(bfs, FlatBasis) else flatten(bfs)
+
+
+def all_twoe_ints_chrr(bfs) -> np.ndarray:
+    """GPU ``all_twoe_ints_chrr`` (src/HGP.jl:22-34), 0-based slots."""
+    return default_context().eri_unique(_flat(bfs), order=_lib.MI_ORDER_CHRR)
+
+
+def all_twoe_ints(bfs) -> np.ndarray:
+    """GPU ``all_twoe_ints`` (src/ERI.jl:104-112), ``iindex`` slots (0-based)."""
+    return default_context().eri_unique(_flat(bfs), order=_lib.MI_ORDER_IINDEX)

@@ -1,0 +1,285 @@
+"""Pins the CPU oracle (oracle/hgp_oracle.c) before it is trusted as the checker.
+
+What pins each oracle function (a plausible mistake must fail one of these):
+
+* AO tables (Basis.jl:264-385)   -- test/runtests.jl:32-37 KATs (nao, m2ao) and
+  the literal shell_indices rows (incl. f: xyy, xzz, xyz).
+* Boys Fm / farray (Boys.jl)     -- KATs test/runtests.jl:70-72; mpmath closed
+  form F_m(T) = 1F1(m+1/2; m+3/2; -T)/(2m+1) at 1e-14 relative over T in
+  [0, 60]; F_0(T) = sqrt(pi/T) erf(sqrt T)/2; F_m(0) = 1/(2m+1); the
+  downward-recursion identity; the T>=30 branch equals the reference's
+  asymptotic formula exactly (e^-T dropped, as Boys.jl:21-26).  The last ulp of
+  SpecialFunctions.gamma_inc is NOT reachable here: parity unpinned at that level.
+* vrr (HGP.jl:135-242)           -- all 19 VRR KATs (test/runtests.jl:159-187).
+* hrr (HGP.jl:392-424)           -- all 19 HRR KATs (test/runtests.jl:198-234).
+* prefactor D1 / normalisation D2 / gather D3 -- closed-form s-primitive value
+  2 pi^{5/2}/(zeta eta sqrt(zeta+eta)) exp(..) F_0 at unequal exponents (D1
+  fails it by the factor zeta eta/(zeta+eta)); H2/STO-3G contracted KATs
+  (runtests.jl:48-52; D2 fails them); independent THO restatement (ERI.jl:15-94)
+  agreeing to 1e-13 on random s/p/d bases with general exponents (D3 and any
+  dropped VRR/HRR term fail it).
+* iindex / chrr slots            -- runtests.jl:22-24 KATs and brute-force
+  enumeration (test_indexing.py).
+* make2JmK index algebra (ERI.jl:114-128) -- brute force over the full 4-index
+  tensor expanded from the unique array (test_jk_from_unique_brute_force).
+"""
+import math
+
+import mpmath
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def m2ao0(ijk):
+    from mi_amd import m2ao
+    return m2ao(ijk) - 1
+
+
+# ------------------------------------------------------------------ Boys
+@pytest.mark.parametrize("m,T,val", [(0, 0, 1.0), (0, 1, 0.7468241328124271), (1, 0, 0.33333333333333)])
+def test_boys_reference_kats(m, T, val):
+    assert O.Fm(m, T) == pytest.approx(val, rel=1.5e-8)
+
+
+def boys_mp(m, T):
+    mpmath.mp.dps = 40
+    return mpmath.hyp1f1(m + 0.5, m + 1.5, -T) / (2 * m + 1)
+
+
+@pytest.mark.parametrize("m", [0, 1, 4, 8, 12])
+def test_boys_against_mpmath(m):
+    for T in [0.0, 1e-20, 1e-6, 0.3, 1.0, 2.5, 7.0, 12.0, 19.9, 29.99]:
+        ref = float(boys_mp(m, T))
+        assert O.Fm(m, T) == pytest.approx(ref, rel=1e-14, abs=0)
+
+
+def test_boys_closed_forms_and_recursion():
+    for T in [0.01, 0.5, 3.0, 11.0, 25.0]:
+        f0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        assert O.Fm(0, T) == pytest.approx(f0, rel=1e-14)
+        for m in range(6):
+            lhs = (2 * m + 1) * O.Fm(m, T)
+            rhs = 2 * T * O.Fm(m + 1, T) + math.exp(-T)
+            assert lhs == pytest.approx(rhs, rel=1e-13)
+    for m in range(10):
+        assert O.Fm(m, 0.0) == 1.0 / (2 * m + 1)
+
+
+def test_farray_branches():
+    # T < 30: top order from Fm, downward recursion with e^-T
+    F = O.farray(9, 12.5)
+    for m in range(9):
+        assert F[m] == pytest.approx(float(boys_mp(m, 12.5)), rel=1e-13)
+    # T >= 30: the reference's asymptotic branch exactly (no e^-T term)
+    T = 41.0
+    F = O.farray(6, T)
+    mt = 5
+    top = math.sqrt(math.pi / 2) * math.prod(range(2 * mt - 1, 0, -2)) / (2 * T) ** (mt + 0.5)
+    assert F[mt] == pytest.approx(top, rel=1e-15)
+    for m in range(mt - 1, -1, -1):
+        assert F[m] == pytest.approx(T * F[m + 1] / (m + 0.5), rel=1e-15)
+    assert F[0] == pytest.approx(float(boys_mp(0, T)), rel=1e-14)
+
+
+# ------------------------------------------------------------------ AO tables
+def test_ao_table_kats():
+    assert [O.nao(l) for l in range(3)] == [1, 4, 10]
+    assert m2ao0((0, 0, 0)) == 0 and m2ao0((0, 0, 1)) == 3
+    lib = O.lib()
+    import ctypes
+    m = (ctypes.c_int32 * 3)()
+    f_order = [(3, 0, 0), (2, 1, 0), (2, 0, 1), (1, 2, 0), (1, 0, 2), (1, 1, 1), (0, 3, 0), (0, 2, 1), (0, 1, 2),
+               (0, 0, 3)]
+    for k, ijk in enumerate(f_order):
+        lib.oracle_ao2m(10 + k, m)
+        assert tuple(m) == ijk
+    # shift_direction is the FIRST argmax (Julia argmax)
+    assert lib.oracle_shift_direction(m2ao0((1, 1, 0))) == 0
+    assert lib.oracle_shift_direction(m2ao0((0, 1, 1))) == 1
+    assert lib.oracle_shift_direction(m2ao0((1, 2, 1))) == 1
+
+
+# ------------------------------------------------------------------ VRR / HRR KATs
+VRR_KATS = [
+    ((0., 0., 0.), (0, 0, 0), (0, 0, 0), 4.37335456733),
+    ((0., 0., 0.), (1, 0, 0), (1, 0, 0), 0.182223107579),
+    ((0., 0., 0.), (0, 1, 0), (0, 1, 0), 0.182223107579),
+    ((0., 0., 0.), (0, 0, 1), (0, 0, 1), 0.182223107579),
+    ((0., 0., 0.), (2, 0, 0), (2, 0, 0), 0.223223306785),
+    ((0., 0., 0.), (0, 2, 0), (0, 2, 0), 0.223223306785),
+    ((0., 0., 0.), (0, 0, 2), (0, 0, 2), 0.223223306785),
+    ((1., 2., 3.), (1, 0, 0), (1, 0, 0), -5.63387712455e-06),
+    ((1., 2., 3.), (0, 1, 0), (0, 1, 0), -0.000116463120359),
+    ((1., 2., 3.), (0, 0, 1), (0, 0, 1), -0.000301178525749),
+    ((1., 2., 3.), (2, 0, 0), (2, 0, 0), 0.00022503308545040895),
+    ((1., 2., 3.), (0, 2, 0), (0, 2, 0), 0.0006102470883881907),
+    ((1., 2., 3.), (0, 0, 2), (0, 0, 2), 0.0013427831014563411),
+    ((0., 0., 0.), (1, 1, 0), (1, 1, 0), 0.0136667330685),
+    ((0., 0., 0.), (0, 1, 1), (0, 1, 1), 0.0136667330685),
+    ((0., 0., 0.), (1, 0, 1), (1, 0, 1), 0.0136667330685),
+    ((3., 2., 1.), (1, 1, 0), (1, 1, 0), 5.976771621486971e-5),
+    ((3., 2., 1.), (0, 1, 1), (0, 1, 1), 1.5742904443905067e-6),
+    ((3., 2., 1.), (1, 0, 1), (1, 0, 1), 4.00292848649699e-6),
+]  # test/runtests.jl:159-184
+
+HRR_KATS = [
+    ((0., 0., 0.), (0, 0, 0), (0, 0, 0), 0.0136667330685),
+    ((0., 0., 0.), (1, 0, 0), (1, 0, 0), 0.00821630976139),
+    ((0., 0., 0.), (0, 1, 0), (0, 1, 0), 0.00122024402397),
+    ((0., 0., 0.), (0, 0, 1), (0, 0, 1), 0.00821630976139),
+    ((0., 0., 0.), (2, 0, 0), (2, 0, 0), 0.0039759617781),
+    ((0., 0., 0.), (0, 2, 0), (0, 2, 0), 0.000599953311785),
+    ((0., 0., 0.), (0, 0, 2), (0, 0, 2), 0.0039759617781),
+    ((1., 2., 3.), (1, 0, 0), (1, 0, 0), -1.1851316496333975e-6),
+    ((1., 2., 3.), (0, 1, 0), (0, 1, 0), -4.669991667384835e-6),
+    ((1., 2., 3.), (0, 0, 1), (0, 0, 1), -3.474373852654044e-5),
+    ((1., 2., 3.), (2, 0, 0), (2, 0, 0), 2.81002247462e-6),
+    ((1., 2., 3.), (0, 2, 0), (0, 2, 0), 7.09856891538e-6),
+    ((1., 2., 3.), (0, 0, 2), (0, 0, 2), 3.62153023224e-5),
+    ((0., 0., 0.), (1, 1, 0), (1, 1, 0), 0.000599953311785),
+    ((0., 0., 0.), (0, 1, 1), (0, 1, 1), 0.000599953311785),
+    ((0., 0., 0.), (1, 0, 1), (1, 0, 1), 0.0116431617287),
+    ((3., 2., 1.), (1, 1, 0), (1, 1, 0), 7.37307761485e-6),
+    ((3., 2., 1.), (0, 1, 1), (0, 1, 1), 2.5333243119843164e-7),
+    ((3., 2., 1.), (1, 0, 1), (1, 0, 1), 2.452115184675799e-6),
+]  # test/runtests.jl:198-222, b = d = (1,0,1)
+
+
+@pytest.mark.parametrize("literal", [0, O.LIT_D1])
+@pytest.mark.parametrize("A,a,c,val", VRR_KATS)
+def test_vrr_kats(A, a, c, val, literal):
+    # all exponents 1.0 -> the D1 prefactor coincides with the correct one
+    v = O.vrr(sum(a), sum(c), 1.0, 1.0, 1.0, 1.0, A, (0, 0, 0), (0, 0, 0), (0, 0, 0), literal)
+    assert v[0, m2ao0(a), m2ao0(c)] == pytest.approx(val, rel=1.5e-8)
+
+
+@pytest.mark.parametrize("A,a,c,val", HRR_KATS)
+def test_hrr_kats(A, a, c, val):
+    b = d = (1, 0, 1)
+    h = O.hrr(sum(a), 2, sum(c), 2, 1.0, 1.0, 1.0, 1.0, A, (0, 0, 0), (0, 0, 0), (0, 0, 0))
+    assert h[m2ao0(a), m2ao0(b), m2ao0(c), m2ao0(d)] == pytest.approx(val, rel=1.5e-8)
+
+
+# ------------------------------------------------------------------ prefactor (D1)
+def test_prefactor_closed_form_unequal_exponents():
+    """[ss|ss]^(0) for four primitives on one centre is 2 pi^{5/2}/(zeta eta
+    sqrt(zeta+eta)) (T = 0, F_0 = 1).  The reference's vrr! (D1) would give
+    2 pi^{5/2} (zeta+eta)^{-3/2}."""
+    a, b, c, d = 0.5, 0.5, 3.0, 3.0
+    z, e = a + b, c + d
+    good = 2 * math.pi ** 2.5 / (z * e * math.sqrt(z + e))
+    lit = 2 * math.pi ** 2.5 / (z + e) ** 1.5
+    O0 = (0.0, 0.0, 0.0)
+    assert O.vrr(0, 0, a, b, c, d, O0, O0, O0, O0)[0, 0, 0] == pytest.approx(good, rel=1e-14)
+    assert O.vrr(0, 0, a, b, c, d, O0, O0, O0, O0, O.LIT_D1)[0, 0, 0] == pytest.approx(lit, rel=1e-14)
+    assert O.coulomb_prim(a, O0, (0, 0, 0), b, O0, (0, 0, 0), c, O0, (0, 0, 0), d, O0, (0, 0, 0)) == \
+        pytest.approx(good, rel=1e-14)
+
+
+def test_eri_tests_from_pyquante():
+    """test/runtests.jl:110-127: primitive and distance-scan KATs via THO and HGP."""
+    from mi_amd import flat_from_shells
+    O0 = (0, 0, 0)
+    assert O.coulomb_prim(1, O0, O0, 1, O0, O0, 1, O0, O0, 1, O0, O0) == pytest.approx(4.37335458, rel=1e-8)
+    for r, val in [(0.0, 1.1283791670951362), (1.0, 0.8427007900292186), (2.0, 0.49766113257563993),
+                   (3.0, 0.33332596983445223), (4.0, 0.2499999961456855), (5.0, 1 / 5), (6.0, 1 / 6),
+                   (9.0, 1 / 9)]:
+        fb = flat_from_shells([((0, 0, 0), 0, [1.0], [1.0]), ((0, 0, r), 0, [1.0], [1.0])])
+        e = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
+        # (11|22) is iindex slot 4 -> 0-based 3
+        assert e[3] == pytest.approx(val, rel=1e-8)
+    # (ss|px px) = 0.9403159725793302, (ss|s px) == 0 exactly (runtests.jl:112-113)
+    fb = flat_from_shells([((0, 0, 0), 0, [1.0], [1.0]), ((0, 0, 0), 1, [1.0], [1.0])])
+    e = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
+    from mi_amd import iindex
+    assert e[iindex(1, 1, 2, 2) - 1] == pytest.approx(0.9403159725793302, rel=1e-12)
+    assert e[iindex(1, 1, 1, 2) - 1] == 0.0
+
+
+# ------------------------------------------------------------------ contracted (D2)
+def test_h2_sto3g_contracted_kats():
+    from mi_amd import build_basis, flatten, iindex
+    from mi_amd.molecules import h2
+    fb = flatten(build_basis(h2))
+    e = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
+    kat = {(1, 1, 1, 1): 0.7746059439196398, (2, 2, 2, 2): 0.7746059439196398, (1, 1, 2, 2): 0.5727937653511646,
+           (1, 1, 1, 2): 0.4488373301593464, (1, 2, 1, 2): 0.3025451156654606}
+    for q, v in kat.items():
+        # pyquante2's values carry ~10 digits; the reference checks them at rtol 1.5e-8
+        assert e[iindex(*q) - 1] == pytest.approx(v, rel=1e-9)
+    lit = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX, literal=O.LIT_D1 | O.LIT_D2 | O.LIT_D3)
+    assert abs(lit[0] - kat[(1, 1, 1, 1)]) > 1.0  # the shipped driver is off
+
+
+# ------------------------------------------------------------------ THO cross-check (D3, VRR/HRR terms)
+def random_spec(rng, nshell, lmax=2, kmax=2):
+    spec = []
+    cents = rng.uniform(-1.5, 1.5, size=(3, 3))
+    for _ in range(nshell):
+        L = int(rng.integers(0, lmax + 1))
+        k = int(rng.integers(1, kmax + 1))
+        ex = np.exp(rng.uniform(np.log(0.2), np.log(8.0), size=k))
+        spec.append((cents[int(rng.integers(0, 3))], L, ex.tolist(), rng.uniform(0.2, 1, size=k).tolist()))
+    return spec
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_hgp_equals_tho_on_random_bases(seed):
+    from mi_amd import flat_from_shells
+    rng = np.random.Generator(np.random.PCG64(seed))
+    fb = flat_from_shells(random_spec(rng, 4))
+    for order in (O.ORDER_IINDEX, O.ORDER_CHRR):
+        h = O.eri_unique_hgp(fb, order=order)
+        t = O.eri_unique_tho(fb, order=order)
+        assert np.max(np.abs(h - t)) <= 1e-13 * max(1.0, np.max(np.abs(t)))
+
+
+def test_hgp_equals_tho_ethane_minimal():
+    from mi_amd import configs
+    fb = configs.ethane_minimal(0).flat()
+    h = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
+    t = O.eri_unique_tho(fb, order=O.ORDER_IINDEX)
+    assert np.max(np.abs(h - t)) < 1e-12
+
+
+def test_literal_d3_changes_p_shells_only():
+    from mi_amd import flat_from_shells
+    s_only = flat_from_shells([((0, 0, 0), 0, [1.0, 0.3], [0.5, 0.5]), ((0, 0, 1.2), 0, [0.7], [1.0])])
+    assert np.array_equal(O.eri_unique_hgp(s_only, literal=O.LIT_D3), O.eri_unique_hgp(s_only))
+    with_p = flat_from_shells([((0, 0, 0), 1, [1.0], [1.0]), ((0, 0, 1.2), 0, [0.7], [1.0])])
+    assert not np.allclose(O.eri_unique_hgp(with_p, literal=O.LIT_D3), O.eri_unique_hgp(with_p))
+
+
+# ------------------------------------------------------------------ 8-fold symmetry invariants
+def test_full_tensor_symmetry_from_unique():
+    """Expanding the unique array to n^4 reproduces all 8 symmetries and
+    equals THO element by element (ethane minimal, 16 functions)."""
+    from mi_amd import configs
+    from mi_amd.indexing import slots
+    fb = configs.ethane_minimal(1).flat()
+    n = fb.nbf
+    e = O.eri_unique_hgp(fb, order=O.ORDER_CHRR)
+    i, j, k, l = np.meshgrid(*[np.arange(1, n + 1)] * 4, indexing="ij")
+    full = e[slots(n, i.ravel(), j.ravel(), k.ravel(), l.ravel(), 1)].reshape(n, n, n, n)
+    assert np.array_equal(full, full.transpose(1, 0, 2, 3))
+    assert np.array_equal(full, full.transpose(2, 3, 0, 1))
+    t = O.eri_unique_tho(fb, order=O.ORDER_CHRR)
+    assert np.max(np.abs(e - t)) < 1e-12
+
+
+def test_jk_from_unique_brute_force():
+    from mi_amd import configs
+    from mi_amd.indexing import slots
+    fb = configs.ethane_minimal(2).flat()
+    n = fb.nbf
+    e = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
+    rng = np.random.Generator(np.random.PCG64(3))
+    D = rng.normal(size=(n, n))
+    D = D + D.T
+    J, K = O.jk_from_unique(n, e, D)
+    i, j, k, l = np.meshgrid(*[np.arange(1, n + 1)] * 4, indexing="ij")
+    full = e[slots(n, i.ravel(), j.ravel(), k.ravel(), l.ravel(), 0)].reshape(n, n, n, n)
+    assert np.allclose(J, np.einsum("ijkl,kl->ij", full, D), rtol=1e-13, atol=1e-12)
+    assert np.allclose(K, np.einsum("ikjl,kl->ij", full, D), rtol=1e-13, atol=1e-12)
