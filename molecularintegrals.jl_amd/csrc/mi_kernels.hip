@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
   const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
   const double rab2 = ABx * ABx + ABy * ABy + ABz * ABz;
   // 2 pi^{5/2} / (zeta eta sqrt(zeta+eta)) exp(...) split as K_ab K_cd / sqrt(zeta+eta)
-  const double kfac = 2.3597548520214004;  // sqrt(2) * pi^{5/4}
+  const double kfac = 5.9149671727956129;  // sqrt(2) * pi^{5/4}
   PrimPair o;
   o.Px = Px; o.Py = Py; o.Pz = Pz; o.zeta = zeta;
   o.PAx = Px - Ax; o.PAy = Py - Ay; o.PAz = Pz - Az;

@@ -6,7 +6,7 @@
  *
  * Citations are to files of rpmuller/MolecularIntegrals.jl.
  * Parity status of each part (how it is pinned) is listed in the header
- * comment of tests/test_oracle.py and tests/test_oracle_tho.py.
+ * comment of tests/test_oracle.py.
  */
 #include "hgp_oracle.h"
 
@@ -163,189 +163,22 @@ static double dist2(const double *a, const double *b) {
 /*   gamma_lower(a,x) = x^a e^{-x} sum_{n>=0} x^n / (a (a+1) ... (a+n)),      */
 /* so Fm(T) = e^{-T}/2 * sum_n T^n / prod_{k=0..n} (m+1/2+k).  All terms are   */
 /* positive (no cancellation); the sum runs in long double.                  */
-double oracle_Fm(int m, double T) {
-    const double SMALL = 1e-18;
-    if (T < SMALL) return 1.0 / (2 * m + 1);
+static long double boys_top_ld(int m, long double x) {
+    const long double SMALL = 1e-18L;
+    if (x < SMALL) return 1.0L / (2 * m + 1);
     long double a = (long double)m + 0.5L;
     long double term = 1.0L / a;
     long double sum = term;
-    long double x = (long double)T;
     for (int n = 1; n < 10000; ++n) {
         term *= x / (a + n);
         sum += term;
         if (term < sum * 1e-21L) break;
     }
-    return (double)(0.5L * expl(-x) * sum);
+    return 0.5L * expl(-x) * sum;
 }
 
-/* farray(mmax,T): T < Tmax=30 -> top m from Fm then downward recursion      */
-/* F_m = (2T F_{m+1} + e^{-T})/(2m+1) (Boys.jl:9-17); T >= 30 -> asymptotic  */
-/* Fm_asymp (Boys.jl:36) then F_m = T F_{m+1}/(m+1/2), no e^{-T} term        */
-/* (Boys.jl:21-26).  Fms[k] holds F_k (0-based).                              */
-void oracle_farray(int mmax, double T, double *Fms) {
-    const double Tmax = 30.0;
-    if (T < Tmax) {
-        Fms[mmax - 1] = oracle_Fm(mmax - 1, T);
-        double emt = exp(-T);
-        for (int m = mmax - 2; m >= 0; --m) Fms[m] = (2 * T * Fms[m + 1] + emt) / (2 * m + 1);
-        return;
-    }
-    int mt = mmax - 1;
-    Fms[mt] = sqrt(M_PI / 2) * factorial2(2 * mt - 1) / pow(2 * T, mt + 0.5);
-    for (int m = mt - 1; m >= 0; --m) Fms[m] = T * Fms[m + 1] / (m + 0.5);
-}
+double oracle_Fm(int m, double T) { return (double)boys_top_ld(m, (long double)T); }
 
-/* ------------------------------------------------------------------------ */
-/* vrr! -- src/HGP.jl:135-242 (HGP eq. 6, b = d = 0).                        */
-/* ------------------------------------------------------------------------ */
-#define VIDX(m, a, c) (((size_t)(m) * na + (a)) * nc + (c))
-
-void oracle_vrr(double *vrrs, int amax, int cmax, double aexpn, double bexpn,
-                double cexpn, double dexpn, const double *A, const double *B,
-                const double *C, const double *D, int literal) {
-    init_tables();
-    const int mmax = amax + cmax + 1;
-    const int na = oracle_nao(amax), nc = oracle_nao(cmax);
-    memset(vrrs, 0, sizeof(double) * (size_t)mmax * na * nc);
-
-    double zeta = aexpn + bexpn, eta = cexpn + dexpn, ze = zeta + eta;
-    double ooz = 1 / zeta, ooe = 1 / eta, ooze = 1 / ze;
-    double oortze = sqrt(ooze);
-    double P[3], Q[3], W[3], PA[3], WP[3], QC[3], WQ[3];
-    for (int k = 0; k < 3; ++k) {
-        P[k] = (aexpn * A[k] + bexpn * B[k]) * ooz;
-        Q[k] = (cexpn * C[k] + dexpn * D[k]) * ooe;
-    }
-    for (int k = 0; k < 3; ++k) {
-        W[k] = (zeta * P[k] + eta * Q[k]) * ooze;
-        PA[k] = P[k] - A[k];
-        WP[k] = W[k] - P[k];
-        QC[k] = Q[k] - C[k];
-        WQ[k] = W[k] - Q[k];
-    }
-    double rab2 = dist2(A, B), rcd2 = dist2(C, D), rpq2 = dist2(P, Q);
-    double T = zeta * eta * rpq2 * ooze;
-    double expo = exp(-aexpn * bexpn * rab2 * ooz - cexpn * dexpn * rcd2 * ooe);
-    double pref;
-    if (literal & ORACLE_LIT_D1)
-        pref = 2 * M_PI * M_PI * sqrt(M_PI) * ooze * oortze * expo; /* HGP.jl:150-151 */
-    else
-        pref = 2 * M_PI * M_PI * sqrt(M_PI) * ooz * ooe * oortze * expo; /* ERI.jl:45 */
-
-    double Fms[64];
-    oracle_farray(mmax, T, Fms); /* HGP.jl:168 */
-    for (int m = 0; m < mmax; ++m) vrrs[VIDX(m, 0, 0)] = pref * Fms[m];
-    if (mmax == 1) return;
-
-    /* a-build, HGP.jl:180-197 */
-    for (int aplus = 1; aplus < na; ++aplus) {
-        int ashell = g_shell_number[aplus];
-        int i = g_shift_direction[aplus];
-        int a = g_shift_index[aplus][i];
-        int lim = mmax - ashell;
-        int aminus = g_shift_index[a][i];
-        if (aminus >= 0) {
-            double a_i = 0.5 * ooz * g_ao2m[a][i];
-            for (int m = 0; m < lim; ++m)
-                vrrs[VIDX(m, aplus, 0)] = PA[i] * vrrs[VIDX(m, a, 0)] + WP[i] * vrrs[VIDX(m + 1, a, 0)] +
-                                          a_i * (vrrs[VIDX(m, aminus, 0)] - eta * ooze * vrrs[VIDX(m + 1, aminus, 0)]);
-        } else {
-            for (int m = 0; m < lim; ++m)
-                vrrs[VIDX(m, aplus, 0)] = PA[i] * vrrs[VIDX(m, a, 0)] + WP[i] * vrrs[VIDX(m + 1, a, 0)];
-        }
-    }
-
-    /* c-build, HGP.jl:204-240 */
-    for (int cplus = 1; cplus < nc; ++cplus) {
-        int cshell = g_shell_number[cplus];
-        int i = g_shift_direction[cplus];
-        int c = g_shift_index[cplus][i];
-        int cminus = g_shift_index[c][i];
-        for (int a = 0; a < na; ++a) {
-            int ashell = g_shell_number[a];
-            int lim = mmax - cshell - ashell;
-            int aminus = g_shift_index[a][i];
-            if (cminus >= 0) {
-                double c_i = 0.5 * ooe * g_ao2m[c][i];
-                if (aminus >= 0) {
-                    double a_i = 0.5 * ooze * g_ao2m[a][i];
-                    for (int m = 0; m < lim; ++m)
-                        vrrs[VIDX(m, a, cplus)] = QC[i] * vrrs[VIDX(m, a, c)] + WQ[i] * vrrs[VIDX(m + 1, a, c)] +
-                                                  a_i * vrrs[VIDX(m + 1, aminus, c)] +
-                                                  c_i * (vrrs[VIDX(m, a, cminus)] - zeta * ooze * vrrs[VIDX(m + 1, a, cminus)]);
-                } else {
-                    for (int m = 0; m < lim; ++m)
-                        vrrs[VIDX(m, a, cplus)] = QC[i] * vrrs[VIDX(m, a, c)] + WQ[i] * vrrs[VIDX(m + 1, a, c)] +
-                                                  c_i * (vrrs[VIDX(m, a, cminus)] - zeta * ooze * vrrs[VIDX(m + 1, a, cminus)]);
-                }
-            } else if (aminus >= 0) {
-                double a_i = 0.5 * ooze * g_ao2m[a][i];
-                for (int m = 0; m < lim; ++m)
-                    vrrs[VIDX(m, a, cplus)] = QC[i] * vrrs[VIDX(m, a, c)] + WQ[i] * vrrs[VIDX(m + 1, a, c)] +
-                                              a_i * vrrs[VIDX(m + 1, aminus, c)];
-            } else {
-                for (int m = 0; m < lim; ++m)
-                    vrrs[VIDX(m, a, cplus)] = QC[i] * vrrs[VIDX(m, a, c)] + WQ[i] * vrrs[VIDX(m + 1, a, c)];
-            }
-        }
-    }
-}
-#undef VIDX
-
-/* ------------------------------------------------------------------------ */
-/* HRR on a [nao(la+lb)][nao(lc+ld)] contracted (e0|f0) block:              */
-/* src/HGP.jl:87-114 (chrr) == :395-422 (hrr).                              */
-/* ------------------------------------------------------------------------ */
-#define HIDX(a, b, c, d) ((((size_t)(a) * nb + (b)) * nc + (c)) * nd + (d))
-static void hrr_from_vrr(double *hrrs, const double *e0f0, int la, int lb, int lc, int ld,
-                         const double *A, const double *B, const double *C, const double *D) {
-    init_tables();
-    const int na = oracle_nao(la + lb), nb = oracle_nao(lb), nc = oracle_nao(lc + ld), nd = oracle_nao(ld);
-    memset(hrrs, 0, sizeof(double) * (size_t)na * nb * nc * nd);
-    for (int c = 0; c < nc; ++c)
-        for (int a = 0; a < na; ++a) hrrs[HIDX(a, 0, c, 0)] = e0f0[(size_t)a * nc + c];
-    /* bra: (a, b+1_j | = (a+1_j, b| + (A_j - B_j)(a, b|  (HGP.jl:93-101) */
-    for (int c = 0; c < nc; ++c)
-        for (int bplus = 1; bplus < nb; ++bplus) {
-            int j = g_shift_direction[bplus];
-            int b = g_shift_index[bplus][j];
-            int bs = g_shell_number[b];
-            int alim = oracle_nao(la + lb - bs - 1);
-            for (int a = 0; a < alim; ++a) {
-                int aplus = g_shift_index_plus[a][j];
-                hrrs[HIDX(a, bplus, c, 0)] = hrrs[HIDX(aplus, b, c, 0)] + (A[j] - B[j]) * hrrs[HIDX(a, b, c, 0)];
-            }
-        }
-    /* ket: (ab| c, d+1_j) = (ab| c+1_j, d) + (C_j - D_j)(ab| c, d)  (HGP.jl:104-114) */
-    for (int dplus = 1; dplus < nd; ++dplus) {
-        int j = g_shift_direction[dplus];
-        int d = g_shift_index[dplus][j];
-        int ds = g_shell_number[d];
-        int clim = oracle_nao(lc + ld - ds - 1);
-        for (int c = 0; c < clim; ++c) {
-            int cplus = g_shift_index_plus[c][j];
-            for (int b = 0; b < nb; ++b)
-                for (int a = 0; a < oracle_nao(la); ++a)
-                    hrrs[HIDX(a, b, c, dplus)] = hrrs[HIDX(a, b, cplus, d)] + (C[j] - D[j]) * hrrs[HIDX(a, b, c, d)];
-        }
-    }
-}
-
-void oracle_hrr(double *hrrs, int la, int lb, int lc, int ld, double aexpn, double bexpn,
-                double cexpn, double dexpn, const double *A, const double *B, const double *C,
-                const double *D, int literal) {
-    /* hrr(): vrr then HRR on the [1,:,:] slice, no contraction (HGP.jl:392-424) */
-    int amax = la + lb, cmax = lc + ld, mmax = amax + cmax + 1;
-    int na = oracle_nao(amax), nc = oracle_nao(cmax);
-    double *v = (double *)malloc(sizeof(double) * (size_t)mmax * na * nc);
-    oracle_vrr(v, amax, cmax, aexpn, bexpn, cexpn, dexpn, A, B, C, D, literal);
-    hrr_from_vrr(hrrs, v, la, lb, lc, ld, A, B, C, D); /* m=0 slice is the first na*nc */
-    free(v);
-}
-
-/* ------------------------------------------------------------------------ */
-/* cvrr + chrr on shells -- src/HGP.jl:50-68 and :83-116.                     */
-/* ------------------------------------------------------------------------ */
 typedef struct {
     int nshell, nbf;
     const double *xyz;
@@ -365,27 +198,6 @@ static int check_basis(flat_basis *fb) {
     }
     fb->bfoff[fb->nshell] = n;
     return n == fb->nbf ? 0 : 3;
-}
-
-/* Contracted (e0|f0) over the primitive quartets, loop order d, c, b, a
- * (HGP.jl:57-58); coefficient = raw coef (literal D2) or coef * radial norm. */
-static void cvrr_shells(double *cv, double *work, const flat_basis *fb, int sa, int sb, int sc,
-                        int sd, int literal) {
-    int amax = fb->L[sa] + fb->L[sb], cmax = fb->L[sc] + fb->L[sd];
-    int na = oracle_nao(amax), nc = oracle_nao(cmax);
-    memset(cv, 0, sizeof(double) * (size_t)na * nc);
-    const double *A = fb->xyz + 3 * sa, *B = fb->xyz + 3 * sb, *C = fb->xyz + 3 * sc, *D = fb->xyz + 3 * sd;
-    int raw = (literal & ORACLE_LIT_D2) != 0;
-    for (int pd = fb->poff[sd]; pd < fb->poff[sd + 1]; ++pd)
-        for (int pc = fb->poff[sc]; pc < fb->poff[sc + 1]; ++pc)
-            for (int pb = fb->poff[sb]; pb < fb->poff[sb + 1]; ++pb)
-                for (int pa = fb->poff[sa]; pa < fb->poff[sa + 1]; ++pa) {
-                    oracle_vrr(work, amax, cmax, fb->pexp[pa], fb->pexp[pb], fb->pexp[pc], fb->pexp[pd], A, B, C,
-                               D, literal);
-                    double w = fb->pcoef[pa] * fb->pcoef[pb] * fb->pcoef[pc] * fb->pcoef[pd];
-                    if (!raw) w *= fb->pnorm[pa] * fb->pnorm[pb] * fb->pnorm[pc] * fb->pnorm[pd];
-                    for (size_t x = 0; x < (size_t)na * nc; ++x) cv[x] += w * work[x];
-                }
 }
 
 static int64_t tri1(int64_t i, int64_t j) { /* triangle(i,j), 1-based, Utils.jl:15 */
@@ -412,37 +224,51 @@ static int is_key(const flat_basis *fb, int I, int J, int K, int Lsh) {
     return tri1(i0, j0) <= tri1(k1, lmax);
 }
 
-static void process_key(const flat_basis *fb, int I, int J, int K, int Lsh, int order, int literal,
-                        double *cv, double *work, double *hr, double *out, double *checksum) {
-    int la = fb->L[I], lb = fb->L[J], lc = fb->L[K], ld = fb->L[Lsh];
-    cvrr_shells(cv, work, fb, I, J, K, Lsh, literal);
-    hrr_from_vrr(hr, cv, la, lb, lc, ld, fb->xyz + 3 * I, fb->xyz + 3 * J, fb->xyz + 3 * K, fb->xyz + 3 * Lsh);
-    const int nb = oracle_nao(lb), nc = oracle_nao(lc + ld), nd = oracle_nao(ld);
-    int64_t n = fb->nbf;
-    int raw = (literal & ORACLE_LIT_D2) != 0;
-    int d3 = (literal & ORACLE_LIT_D3) != 0;
-    for (int ma = 0; ma < ncart(la); ++ma)
-        for (int mb = 0; mb < ncart(lb); ++mb)
-            for (int mc = 0; mc < ncart(lc); ++mc)
-                for (int md = 0; md < ncart(ld); ++md) {
-                    int64_t i = fb->bfoff[I] + ma + 1, j = fb->bfoff[J] + mb + 1;
-                    int64_t k = fb->bfoff[K] + mc + 1, l = fb->bfoff[Lsh] + md + 1;
-                    if (i < j || k < l || tri1(i, j) > tri1(k, l)) continue;
-                    int ha, hb, hc, hd;
-                    if (d3) { /* HGP.jl:29-30 with mshell (1-based position in shell) */
-                        ha = ma; hb = mb; hc = mc; hd = md;
-                    } else {
-                        ha = oracle_nao(la - 1) + ma;
-                        hb = oracle_nao(lb - 1) + mb;
-                        hc = oracle_nao(lc - 1) + mc;
-                        hd = oracle_nao(ld - 1) + md;
-                    }
-                    double v = hr[(((size_t)ha * nb + hb) * nc + hc) * nd + hd];
-                    if (!raw)
-                        v *= fb->bfscale[i - 1] * fb->bfscale[j - 1] * fb->bfscale[k - 1] * fb->bfscale[l - 1];
-                    if (out) out[oracle_slot(n, i, j, k, l, order)] = v;
-                    if (checksum) *checksum += v;
-                }
+
+/* The recurrences, instantiated in FP64 and in x87 extended precision. */
+#define REAL double
+#define SFX(name) name##_d
+#define RSQRT_(x) sqrt(x)
+#define REXP_(x) exp(x)
+#define RPOW_(x, y) pow(x, y)
+#define RPI M_PI
+#include "hgp_core.inc"
+#undef REAL
+#undef SFX
+#undef RSQRT_
+#undef REXP_
+#undef RPOW_
+#undef RPI
+#define REAL long double
+#define SFX(name) name##_x
+#define RSQRT_(x) sqrtl(x)
+#define REXP_(x) expl(x)
+#define RPOW_(x, y) powl(x, y)
+#define RPI 3.141592653589793238462643383279502884L
+#include "hgp_core.inc"
+#undef REAL
+#undef SFX
+#undef RSQRT_
+#undef REXP_
+#undef RPOW_
+#undef RPI
+
+void oracle_farray(int mmax, double T, double *Fms) { farray_d(mmax, T, Fms); }
+
+void oracle_vrr(double *vrrs, int amax, int cmax, double aexpn, double bexpn, double cexpn, double dexpn,
+                const double *A, const double *B, const double *C, const double *D, int literal) {
+    vrr_d(vrrs, amax, cmax, aexpn, bexpn, cexpn, dexpn, A, B, C, D, literal);
+}
+
+void oracle_hrr(double *hrrs, int la, int lb, int lc, int ld, double aexpn, double bexpn, double cexpn,
+                double dexpn, const double *A, const double *B, const double *C, const double *D, int literal) {
+    /* hrr(): vrr then HRR on the [1,:,:] slice, no contraction (HGP.jl:392-424) */
+    int amax = la + lb, cmax = lc + ld, mmax = amax + cmax + 1;
+    int na = oracle_nao(amax), nc = oracle_nao(cmax);
+    double *v = (double *)malloc(sizeof(double) * (size_t)mmax * na * nc);
+    vrr_d(v, amax, cmax, aexpn, bexpn, cexpn, dexpn, A, B, C, D, literal);
+    hrr_from_vrr_d(hrrs, v, la, lb, lc, ld, A, B, C, D); /* m=0 slice is the first na*nc */
+    free(v);
 }
 
 static size_t work_sizes(int maxl, size_t *cvn, size_t *wn, size_t *hn) {
@@ -454,10 +280,11 @@ static size_t work_sizes(int maxl, size_t *cvn, size_t *wn, size_t *hn) {
     return *cvn + *wn + *hn;
 }
 
-int oracle_eri_unique_hgp(int nshell, const double *shell_xyz, const int32_t *shell_l,
-                          const int32_t *shell_prim_off, const double *prim_exp,
-                          const double *prim_coef, const double *prim_norm, int nbf,
-                          const double *bf_scale, int order, int literal, int nthreads, double *out) {
+static int eri_unique_hgp_impl(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                               const int32_t *shell_prim_off, const double *prim_exp,
+                               const double *prim_coef, const double *prim_norm, int nbf,
+                               const double *bf_scale, int order, int literal, int nthreads, double *out,
+                               int extended) {
     flat_basis fb = {nshell, nbf, shell_xyz, shell_l, shell_prim_off, prim_exp, prim_coef, prim_norm, bf_scale, NULL};
     fb.bfoff = (int *)malloc(sizeof(int) * (nshell + 1));
     int rc = check_basis(&fb);
@@ -483,16 +310,41 @@ int oracle_eri_unique_hgp(int nshell, const double *shell_xyz, const int32_t *sh
     {
         size_t cvn, wn, hn;
         work_sizes(maxl, &cvn, &wn, &hn);
-        double *buf = (double *)malloc(sizeof(double) * (cvn + wn + hn));
+        if (extended) {
+            long double *buf = (long double *)malloc(sizeof(long double) * (cvn + wn + hn));
 #pragma omp for schedule(dynamic, 16)
-        for (int64_t q = 0; q < nkeys; ++q)
-            process_key(&fb, keys[4 * q], keys[4 * q + 1], keys[4 * q + 2], keys[4 * q + 3], order, literal, buf,
-                        buf + cvn, buf + cvn + wn, out, NULL);
-        free(buf);
+            for (int64_t q = 0; q < nkeys; ++q)
+                process_key_x(&fb, keys[4 * q], keys[4 * q + 1], keys[4 * q + 2], keys[4 * q + 3], order, literal,
+                              buf, buf + cvn, buf + cvn + wn, out, NULL);
+            free(buf);
+        } else {
+            double *buf = (double *)malloc(sizeof(double) * (cvn + wn + hn));
+#pragma omp for schedule(dynamic, 16)
+            for (int64_t q = 0; q < nkeys; ++q)
+                process_key_d(&fb, keys[4 * q], keys[4 * q + 1], keys[4 * q + 2], keys[4 * q + 3], order, literal,
+                              buf, buf + cvn, buf + cvn + wn, out, NULL);
+            free(buf);
+        }
     }
     free(keys);
     free(fb.bfoff);
     return 0;
+}
+
+int oracle_eri_unique_hgp(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                          const int32_t *shell_prim_off, const double *prim_exp, const double *prim_coef,
+                          const double *prim_norm, int nbf, const double *bf_scale, int order, int literal,
+                          int nthreads, double *out) {
+    return eri_unique_hgp_impl(nshell, shell_xyz, shell_l, shell_prim_off, prim_exp, prim_coef, prim_norm, nbf,
+                               bf_scale, order, literal, nthreads, out, 0);
+}
+
+int oracle_eri_unique_hgp_ext(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                              const int32_t *shell_prim_off, const double *prim_exp, const double *prim_coef,
+                              const double *prim_norm, int nbf, const double *bf_scale, int order, int nthreads,
+                              double *out) {
+    return eri_unique_hgp_impl(nshell, shell_xyz, shell_l, shell_prim_off, prim_exp, prim_coef, prim_norm, nbf,
+                               bf_scale, order, 0, nthreads, out, 1);
 }
 
 int64_t oracle_eri_hgp_quartet_sample(int nshell, const double *shell_xyz, const int32_t *shell_l,
@@ -519,7 +371,7 @@ int64_t oracle_eri_hgp_quartet_sample(int nshell, const double *shell_xyz, const
         for (int64_t q = 0; q < nq; ++q) {
             const int32_t *k = quartets + 4 * q;
             double cs = 0.0;
-            process_key(&fb, k[0], k[1], k[2], k[3], ORACLE_ORDER_CHRR, 0, buf, buf + cvn, buf + cvn + wn, NULL, &cs);
+            process_key_d(&fb, k[0], k[1], k[2], k[3], ORACLE_ORDER_CHRR, 0, buf, buf + cvn, buf + cvn + wn, NULL, &cs);
             total += cs;
             count += (int64_t)ncart(shell_l[k[0]]) * ncart(shell_l[k[1]]) * ncart(shell_l[k[2]]) * ncart(shell_l[k[3]]);
         }
@@ -715,10 +567,10 @@ int oracle_schwarz_pairs(int nshell, const double *shell_xyz, const int32_t *she
     for (int I = 0; I < nshell; ++I)
         for (int J = 0; J <= I; ++J) {
             int la = shell_l[I], lb = shell_l[J];
-            cvrr_shells(buf, buf + cvn, &fb, I, J, I, J, 0);
+            cvrr_shells_d(buf, buf + cvn, &fb, I, J, I, J, 0);
             double *hr = buf + cvn + wn;
-            hrr_from_vrr(hr, buf, la, lb, la, lb, shell_xyz + 3 * I, shell_xyz + 3 * J, shell_xyz + 3 * I,
-                         shell_xyz + 3 * J);
+            hrr_from_vrr_d(hr, buf, la, lb, la, lb, shell_xyz + 3 * I, shell_xyz + 3 * J, shell_xyz + 3 * I,
+                           shell_xyz + 3 * J);
             int nb = oracle_nao(lb), nc = oracle_nao(la + lb), nd = oracle_nao(lb);
             double mx = 0.0;
             for (int ma = 0; ma < ncart(la); ++ma)

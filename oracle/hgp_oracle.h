@@ -87,6 +87,14 @@ int oracle_eri_unique_hgp(int nshell, const double *shell_xyz, const int32_t *sh
                           int nbf, const double *bf_scale, int order, int literal,
                           int nthreads, double *out);
 
+/* The same algorithm in x87 80-bit arithmetic (inputs and output FP64).  Used
+ * only to decide which side lost digits where the FP64 oracle and the GPU
+ * disagree (HRR cancellation with tight/diffuse shell pairs). */
+int oracle_eri_unique_hgp_ext(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                              const int32_t *shell_prim_off, const double *prim_exp,
+                              const double *prim_coef, const double *prim_norm, int nbf,
+                              const double *bf_scale, int order, int nthreads, double *out);
+
 /* Same contract as above but only for the listed shell quartets (sampled CPU
  * baseline); returns number of ERIs computed (written nowhere). */
 int64_t oracle_eri_hgp_quartet_sample(int nshell, const double *shell_xyz,

@@ -58,6 +58,7 @@ def lib():
         basis_args = [ctypes.c_int, _dp, _ip, _ip, _dp, _dp, _dp, ctypes.c_int, _dp]
         L.oracle_eri_unique_hgp.argtypes = basis_args + [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_eri_unique_tho.argtypes = basis_args + [ctypes.c_int, _dp]
+        L.oracle_eri_unique_hgp_ext.argtypes = basis_args + [ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_eri_hgp_quartet_sample.restype = ctypes.c_int64
         L.oracle_eri_hgp_quartet_sample.argtypes = basis_args + [ctypes.c_int64, _ip, ctypes.c_int, _dp]
         L.oracle_jk_from_unique.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -152,6 +153,17 @@ def eri_unique_hgp(fb, order=ORDER_CHRR, literal=0, nthreads=0):
     rc = lib().oracle_eri_unique_hgp(*args, int(order), int(literal), int(nthreads), out.ctypes.data_as(_dp))
     if rc:
         raise ValueError(f"oracle_eri_unique_hgp rejected the basis (code {rc})")
+    return out
+
+
+def eri_unique_hgp_ext(fb, order=ORDER_CHRR, nthreads=0):
+    """The same corrected HGP algorithm in x87 80-bit arithmetic: arbiter where
+    the FP64 oracle loses digits to HRR cancellation."""
+    keep, args = _basis_args(fb)
+    out = np.zeros(n_unique(fb.nbf))
+    rc = lib().oracle_eri_unique_hgp_ext(*args, int(order), int(nthreads), out.ctypes.data_as(_dp))
+    if rc:
+        raise ValueError(f"oracle_eri_unique_hgp_ext rejected the basis (code {rc})")
     return out
 
 

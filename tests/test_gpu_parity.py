@@ -16,12 +16,28 @@ ABS_TOL = 1e-12
 REL_TOL = 1e-10
 
 
-def assert_eri_close(got, ref, what=""):
-    assert got.shape == ref.shape, f"{what}: count {got.shape} vs {ref.shape}"
+def within(got, ref):
     d = np.abs(got - ref)
-    ok = (d <= ABS_TOL) | (d <= REL_TOL * np.abs(ref))
+    return (d <= ABS_TOL) | (d <= REL_TOL * np.abs(ref))
+
+
+def assert_eri_close(got, ref, what="", fb=None, order=None):
+    """Every slot within the bar of the FP64 oracle.  Where the FP64 oracle
+    itself loses digits (the reference's chrr keeps the lower-L shell on A and
+    cancels in the HRR; the GPU puts the higher-L shell on A), the same
+    algorithm in 80-bit arithmetic arbitrates: the slot passes only if the GPU
+    is within the bar of it and the FP64 oracle is not."""
+    assert got.shape == ref.shape, f"{what}: count {got.shape} vs {ref.shape}"
+    ok = within(got, ref)
+    if not ok.all() and fb is not None:
+        bad = np.nonzero(~ok)[0]
+        ext = O.eri_unique_hgp_ext(fb, order=order, nthreads=8)
+        arb = within(got[bad], ext[bad]) & ~within(ref[bad], ext[bad])
+        ok[bad[arb]] = True
+        print(f"{what}: {arb.sum()} slot(s) where the FP64 oracle deviates and the 80-bit oracle sides with the GPU")
     if not ok.all():
         bad = np.nonzero(~ok)[0]
+        d = np.abs(got - ref)
         i = bad[np.argmax(d[bad])]
         raise AssertionError(f"{what}: {bad.size} of {ref.size} integrals out of tolerance; worst slot {i}: "
                              f"gpu={got[i]!r} oracle={ref[i]!r} |d|={d[i]:.3e}")
@@ -48,7 +64,7 @@ def test_ethane_minimal_full_array(ctx, seed, order):
     got = ctx.eri_unique(fb, order=order)
     ref = O.eri_unique_hgp(fb, order=order)
     assert got.size == 9316
-    assert_eri_close(got, ref, f"ethane_minimal seed={seed} order={order}")
+    assert_eri_close(got, ref, f"ethane_minimal seed={seed} order={order}", fb, order)
 
 
 @pytest.mark.parametrize("order", [O.ORDER_CHRR, O.ORDER_IINDEX])
@@ -58,7 +74,7 @@ def test_ethane_split_valence_full_array(ctx, order):
     got = ctx.eri_unique(fb, order=order)
     ref = O.eri_unique_hgp(fb, order=order, nthreads=8)
     assert got.size == 1675365
-    assert_eri_close(got, ref, f"ethane_split_valence order={order}")
+    assert_eri_close(got, ref, f"ethane_split_valence order={order}", fb, order)
 
 
 def test_ethane_split_valence_golden(ctx):
@@ -68,7 +84,11 @@ def test_ethane_split_valence_golden(ctx):
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ethane_split_valence_seed0.npz"))
     fb = configs.ethane_split_valence(0).flat()
     got = ctx.eri_unique(fb, order=O.ORDER_CHRR)
-    assert_eri_close(got[g["slots"]], g["values"], "golden sample")
+    ok = within(got[g["slots"]], g["values"])
+    if not ok.all():  # golden values are FP64-oracle output: arbitrate as above
+        ext = O.eri_unique_hgp_ext(fb, order=O.ORDER_CHRR, nthreads=8)[g["slots"]]
+        ok |= within(got[g["slots"]], ext) & ~within(g["values"], ext)
+    assert ok.all()
     assert abs(got.sum() - float(g["sum"])) <= 1e-9 * abs(float(g["abs_sum"]))
 
 
@@ -93,7 +113,7 @@ def test_random_bases_all_classes(ctx, seed):
     for order in (O.ORDER_CHRR, O.ORDER_IINDEX):
         got = ctx.eri_unique(fb, order=order)
         ref = O.eri_unique_hgp(fb, order=order)
-        assert_eri_close(got, ref, f"random seed={seed} order={order}")
+        assert_eri_close(got, ref, f"random seed={seed} order={order}", fb, order)
 
 
 def test_deep_contraction_lane_groups(ctx):
@@ -108,7 +128,7 @@ def test_deep_contraction_lane_groups(ctx):
     fb = flat_from_shells(spec)
     got = ctx.eri_unique(fb)
     ref = O.eri_unique_hgp(fb)
-    assert_eri_close(got, ref, "deep contraction")
+    assert_eri_close(got, ref, "deep contraction", fb, O.ORDER_CHRR)
 
 
 def test_ethane_sto3g_library_basis(ctx):
@@ -117,7 +137,7 @@ def test_ethane_sto3g_library_basis(ctx):
     bfs = build_basis(ethane, "sto3g")
     got = all_twoe_ints_chrr(bfs)
     ref = O.eri_unique_hgp(flatten(bfs), order=O.ORDER_CHRR)
-    assert_eri_close(got, ref, "ethane/sto3g")
+    assert_eri_close(got, ref, "ethane/sto3g", flatten(bfs), O.ORDER_CHRR)
 
 
 def test_plain_is_bitwise_deterministic(ctx):
@@ -139,7 +159,7 @@ def test_plan_execute_into_device_buffer(ctx):
         plan.execute(out.data_ptr(), stream.cuda_stream)
     torch.cuda.synchronize()
     ref = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
-    assert_eri_close(out.cpu().numpy(), ref, "plan/execute")
+    assert_eri_close(out.cpu().numpy(), ref, "plan/execute", fb, O.ORDER_IINDEX)
     st = plan.stats()
     assert st["unique_eris"] == plan.n_unique and st["nbf"] == fb.nbf
 

@@ -283,3 +283,21 @@ def test_jk_from_unique_brute_force():
     full = e[slots(n, i.ravel(), j.ravel(), k.ravel(), l.ravel(), 0)].reshape(n, n, n, n)
     assert np.allclose(J, np.einsum("ijkl,kl->ij", full, D), rtol=1e-13, atol=1e-12)
     assert np.allclose(K, np.einsum("ikjl,kl->ij", full, D), rtol=1e-13, atol=1e-12)
+
+
+def test_extended_oracle_arbitrates_cancellation():
+    """(p d|p d) with a diffuse p (0.0592) on one centre and a tight d (1206)
+    2.1 bohr away: the reference's HRR order cancels ~7 digits in FP64.  The
+    exact value, 4.5557863656312922624e-05, was computed once with a 60-digit
+    mpmath restatement of the THO formula (src/ERI.jl:15-94) for this quartet of
+    the seed-0 split-valence workload (0-based chrr slot 838794)."""
+    from mi_amd import configs
+    fb = configs.ethane_split_valence(0).flat()
+    e64 = O.eri_unique_hgp(fb, order=O.ORDER_CHRR, nthreads=8)
+    ext = O.eri_unique_hgp_ext(fb, order=O.ORDER_CHRR, nthreads=8)
+    exact = 4.5557863656312922624e-05
+    assert abs(ext[838794] - exact) < 1e-14
+    assert abs(e64[838794] - exact) > 1e-12
+    d = np.abs(e64 - ext)
+    off = ~((d <= 1e-12) | (d <= 1e-10 * np.abs(ext)))
+    assert off.sum() <= 4
