@@ -57,6 +57,14 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 (same
+    # soname as /opt/rocm's).  Whichever loads first serves both, and torch
+    # cannot run on a newer runtime than it was built for, so let torch load
+    # its runtime first when it is installed.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libmi_hgp.so not found at {LIB_PATH}; build it with "
                            f"python molecularintegrals.jl_amd/build.py")
