@@ -88,6 +88,17 @@ const char *mi_last_error(const mi_ctx *ctx);
 int32_t mi_lmax(void);                   /* largest supported shell L      */
 int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
 
+/* Kernel selection (no counterpart in the reference).  rows: nrows x 5 int32
+ * (quartet class, bra contraction bucket, ket contraction bucket, variant
+ * [1 generated register kernel, 0 LDS wave kernel], lanes per quartet G).
+ * Replaces the whole table; unlisted entries use the built-in heuristic.
+ * mi_create loads hgp_tuning.txt beside the library (MI_HGP_TUNING overrides).
+ * Affects plans created afterwards. */
+int mi_ctx_set_tuning(mi_ctx *ctx, const int32_t *rows, int32_t nrows);
+/* Number of streams class launches are spread over (1 = caller's stream only;
+ * at most 4).  Affects plans created afterwards. */
+int mi_ctx_set_streams(mi_ctx *ctx, int32_t nstreams);
+
 /* One-shot drop-in: host basis in, host array out (caller-owned, out_len
  * must be mi_n_unique(nbf)).  Replaces all_twoe_ints_chrr (order CHRR,
  * src/HGP.jl:22) and all_twoe_ints (order IINDEX, src/ERI.jl:104). */

@@ -174,22 +174,34 @@ def emit_rkernel(c):
 
 
 def w_tables(c):
-    """Op tables for the wave-cooperative kernel.  Returns (desc, vitems,
-    accmap, hitems, outs)."""
+    """Op tables for the wave-cooperative kernel.
+
+    VRR: LDS holds [e|f]^(m) for the pruned (e,f) nodes, m contiguous, nodes in
+    level order; items enumerate the same order, so item k of the class writes
+    LDS slot vt0 + k (targets are implicit).  Item = uint2:
+      x = s1 | s2 << 16
+      y = s3 | dir << 16 | n1 << 18 | n2 << 21 | kind << 24 | has2 << 25 | has3 << 26
+    kind 0 (a-build): PA_i [s1] + WP_i [s1+1] + n1/(2 zeta) ([s2] - eta/s [s2+1])
+    kind 1 (c-build): QC_i [s1] + WQ_i [s1+1] + n1/(2 s) [s3+1]
+                      + n2/(2 eta) ([s2] - zeta/s [s2+1])
+    HRR: LDS slots 0..nacc-1 are the contracted (e0|f0) (accsrc[x] = VRR slot
+    of its m=0 value); item k writes slot nacc + k:
+      x = s1 | s2 << 16,  y = dir | ket << 2     ->  [s1] + X_dir [s2]
+    Out: hrr slot | ma << 16 | mb << 20 | mc << 24 | md << 28.
+    """
     la, lb, lc, ld = c
     g = VrrGraph(*c)
     h = HrrGraph(*c)
-    # VRR LDS layout: contiguous m per (e,f)
     base = {}
     off = 0
     for ef in g.nodes:
         base[ef] = off
         off += g.maxm[ef] + 1
     nvrr = off
-    # levels
+    assert g.nodes[0] == (0, 0)
+    vt0 = g.maxm[(0, 0)] + 1
     levels = sorted(set(g.level(*ef) for ef in g.nodes if ef != (0, 0)))
-    vitems = []
-    vlev = []
+    vitems, vlev = [], []
     for lev in levels:
         vlev.append(len(vitems))
         for ef in g.nodes:
@@ -198,13 +210,13 @@ def w_tables(c):
             e, f = ef
             rec = g.recurrence(e, f)
             for m in range(g.maxm[ef] + 1):
-                t = base[ef] + m
+                assert base[ef] + m == vt0 + len(vitems)
                 if rec[0] == "A":
                     _, i, e1, e2, n1 = rec
                     s1 = base[(e1, 0)] + m
                     has2 = e2 >= 0
                     s2 = base[(e2, 0)] + m if has2 else 0
-                    vitems.append((t | (s1 << 16), s2, i | (n1 << 2) | (1 << 14) | (int(has2) << 16), 0))
+                    vitems.append((s1 | (s2 << 16), (i << 16) | (n1 << 18) | (0 << 24) | (int(has2) << 25)))
                 else:
                     _, i, f1, f2, e1, ne, nf = rec
                     s1 = base[(e, f1)] + m
@@ -212,31 +224,29 @@ def w_tables(c):
                     s3 = base[(e1, f1)] + m if has3 else 0
                     has2 = f2 >= 0
                     s2 = base[(e, f2)] + m if has2 else 0
-                    vitems.append((t | (s1 << 16), s2 | (s3 << 16),
-                                   i | (ne << 2) | (nf << 8) | (2 << 14) | (int(has2) << 16) | (int(has3) << 17), 0))
+                    vitems.append((s1 | (s2 << 16), s3 | (i << 16) | (ne << 18) | (nf << 21) | (1 << 24)
+                                   | (int(has2) << 25) | (int(has3) << 26)))
     vlev.append(len(vitems))
-    # HRR layout
-    hbase = {}
-    hoff = 0
-    for n in h.nodes:
-        hbase[n] = hoff
-        hoff += 1
-    nhrr = hoff
-    accmap = []
-    for n in h.nodes:
-        if h.deps(n)[0] == "acc":
-            a, b, cc, d = n
-            accmap.append((base[(a, cc)], hbase[n]))
+    acc_nodes = [n for n in h.nodes if h.deps(n)[0] == "acc"]
     hlevels = sorted(set(h.level(n) for n in h.nodes if h.level(n) > 0))
-    hitems = []
-    hlev = []
+    hbase = {n: k for k, n in enumerate(acc_nodes)}
+    order = []
+    for lev in hlevels:
+        for n in h.nodes:
+            if h.level(n) == lev:
+                hbase[n] = len(acc_nodes) + len(order)
+                order.append(n)
+    nhrr = len(acc_nodes) + len(order)
+    accsrc = [base[(n[0], n[2])] for n in acc_nodes]
+    hitems, hlev = [], []
     for lev in hlevels:
         hlev.append(len(hitems))
-        for n in h.nodes:
+        for n in order:
             if h.level(n) != lev:
                 continue
             kind, s1, s2, j = h.deps(n)
-            hitems.append((hbase[n] | (hbase[s1] << 16), hbase[s2] | (j << 16) | ((1 if kind == "ket" else 0) << 18)))
+            assert hbase[n] == len(acc_nodes) + len(hitems)
+            hitems.append((hbase[s1] | (hbase[s2] << 16), j | ((1 if kind == "ket" else 0) << 2)))
     hlev.append(len(hitems))
     outs = []
     for ma, a in enumerate(shell_aos(la)):
@@ -244,9 +254,10 @@ def w_tables(c):
             for mc, cc in enumerate(shell_aos(lc)):
                 for md, d in enumerate(shell_aos(ld)):
                     outs.append(hbase[(a, b, cc, d)] | (ma << 16) | (mb << 20) | (mc << 24) | (md << 28))
-    desc = dict(la=la, lb=lb, lc=lc, ld=ld, mmax=g.mmax, nvrr=nvrr, nhrr=nhrr, base00=base[(0, 0)],
-                vlev=vlev, hlev=hlev, nacc=len(accmap), nout=len(outs))
-    return desc, vitems, accmap, hitems, outs
+    assert nvrr < 65536 and nhrr < 65536
+    desc = dict(la=la, lb=lb, lc=lc, ld=ld, mmax=g.mmax, nvrr=nvrr, nhrr=nhrr, vt0=vt0,
+                vlev=vlev, hlev=hlev, nacc=len(accsrc), nout=len(outs))
+    return desc, vitems, accsrc, hitems, outs
 
 
 # ---------------------------------------------------------------- driver
@@ -302,7 +313,7 @@ def main():
         vl = list(d["vlev"]) + [d["vlev"][-1]] * (maxlev - len(d["vlev"]))
         hl = list(d["hlev"]) + [d["hlev"][-1]] * (maxhlev - len(d["hlev"]))
         t.append(f"  {{{d['la']}, {d['lb']}, {d['lc']}, {d['ld']}, {d['mmax']}, {d['nvrr']}, {d['nhrr']}, "
-                 f"{d['base00']}, {len(d['vlev']) - 1}, {len(d['hlev']) - 1}, {d['nacc']}, {d['nout']}, "
+                 f"{d['vt0']}, {len(d['vlev']) - 1}, {len(d['hlev']) - 1}, {d['nacc']}, {d['nout']}, "
                  f"{d['voff']}, {d['aoff']}, {d['hoff']}, {d['ooff']}, "
                  f"{{{', '.join(map(str, vl))}}}, {{{', '.join(map(str, hl))}}}, "
                  f"{1 if c in rcls else 0}}},  // {cname(c)}")
@@ -325,8 +336,8 @@ def main():
             t.append("  " + ("0u" if width == 1 else "{" + ", ".join(["0u"] * width) + "}"))
         t.append("};")
 
-    arr("kWVItems", "uint4_t", V, 4)
-    arr("kWAccMap", "uint2_t", A, 2)
+    arr("kWVItems", "uint2_t", V, 2)
+    arr("kWAccMap", "uint32_t", A, 1)
     arr("kWHItems", "uint2_t", H, 2)
     arr("kWOut", "uint32_t", O, 1)
     t.append(f"static const int kWVItemsN = {len(V)};")

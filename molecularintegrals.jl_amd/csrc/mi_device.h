@@ -46,8 +46,10 @@ struct TaskDesc {
   int32_t G;           // lanes per quartet (power of two, <= 64)
   int32_t blockStart;  // first block of this task in its launch
   int32_t cls;         // quartet class id
-  int64_t nq;          // quartets in the task
-  int64_t pad;
+  int32_t variant;     // 1 register kernel, 0 LDS wave kernel
+  int32_t pad;
+  int64_t nq;          // shell quartets in the task
+  int64_t nprim;       // primitive quartets in the task
 };
 
 struct EriArgs {
@@ -81,7 +83,7 @@ struct ScatterCtx {
 // Wave-cooperative class descriptor (tables generated by gen_hgp.py).
 struct WClassDesc {
   int la, lb, lc, ld, mmax;
-  int nvrr, nhrr, base00;
+  int nvrr, nhrr, vt0;       // vt0: LDS slot written by VRR item 0
   int nvlev, nhlev, nacc, nout;
   int voff, aoff, hoff, ooff;
   int vlev[16];

@@ -10,8 +10,13 @@
 // kernel launch over a table of tasks.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -21,12 +26,25 @@
 #include "../../include/mi_hgp.h"
 #include "mi_kernels.h"
 
+#define MI_NBUCKET 5
+#define MI_MAX_STREAMS 4
+
+// Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
+// generated register kernel with G lanes per quartet, 0 = LDS wave kernel.
+struct TuneEntry {
+  int variant = -1;  // -1: not tuned, use the heuristic
+  int G = 1;
+};
+
 struct mi_ctx {
   int device = 0;
   double* d_boys = nullptr;
   void* wtables = nullptr;
   std::string err;
   size_t lds_limit = 0;
+  int nstreams = MI_MAX_STREAMS;              // class launches spread over this many streams
+  hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
+  std::vector<TuneEntry> tune;                // [class][bra bucket][ket bucket]
 };
 
 namespace {
@@ -102,6 +120,40 @@ int pow2floor(double x) {
   return g;
 }
 
+size_t tune_index(int cls, int bb, int kb) { return ((size_t)cls * MI_NBUCKET + bb) * MI_NBUCKET + kb; }
+
+// Tuning file: lines "class bra_bucket ket_bucket variant G" ('#' comments),
+// written by mi_amd/autotune.py.  Default location: hgp_tuning.txt next to
+// the library; MI_HGP_TUNING overrides, MI_HGP_TUNING=none disables.
+void load_tuning_file(mi_ctx* c) {
+  std::string path;
+  const char* env = std::getenv("MI_HGP_TUNING");
+  if (env && *env) {
+    if (std::string(env) == "none") return;
+    path = env;
+  } else {
+    Dl_info info;
+    if (dladdr((void*)&load_tuning_file, &info) && info.dli_fname) {
+      std::string so = info.dli_fname;
+      size_t k = so.rfind('/');
+      path = (k == std::string::npos ? std::string(".") : so.substr(0, k)) + "/hgp_tuning.txt";
+    }
+  }
+  std::ifstream f(path);
+  std::string line;
+  while (f && std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream is(line);
+    int cls, bb, kb, var, G;
+    if (!(is >> cls >> bb >> kb >> var >> G)) continue;
+    if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET) continue;
+    if (var == 1 && !mi::has_register_kernel(cls)) continue;
+    if (var == 0 && !mi::wclass_fits(cls)) continue;
+    if (G < 1 || G > 64 || (G & (G - 1))) continue;
+    c->tune[tune_index(cls, bb, kb)] = {var, G};
+  }
+}
+
 }  // namespace
 
 struct Launch {
@@ -109,6 +161,8 @@ struct Launch {
   bool reg;
   int task_off, ntasks, nblocks;
   int64_t nq_shell, nq_prim;
+  double cost;  // relative device time estimate (longest launches go first)
+  int lane;     // stream lane: 0 = caller's stream, 1.. = context aux streams
 };
 
 struct mi_plan {
@@ -127,7 +181,10 @@ struct mi_plan {
   PrimPair* d_pp = nullptr;
   int32_t* d_pp_pair = nullptr;
   TaskDesc* d_tasks = nullptr;
-  std::vector<Launch> launches;
+  std::vector<Launch> launches;  // in issue order (cost descending)
+  int nlanes = 1;
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[MI_MAX_STREAMS] = {};
 };
 
 namespace {
@@ -165,6 +222,9 @@ int upload(mi_ctx* ctx, T** dst, const T* src, size_t n) {
 }
 
 void free_plan_buffers(mi_plan* p) {
+  if (p->ev_fork) hipEventDestroy(p->ev_fork);
+  for (auto& e : p->ev_join)
+    if (e) hipEventDestroy(e);
   hipFree(p->d_shell_xyz);
   hipFree(p->d_shell_prim_off);
   hipFree(p->d_prim_exp);
@@ -213,6 +273,13 @@ int mi_create(int device, mi_ctx** out) {
     delete c;
     return MI_EDEVICE;
   }
+  for (int k = 0; k < MI_MAX_STREAMS - 1; ++k)
+    if (hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking) != hipSuccess) {
+      c->nstreams = k + 1;
+      break;
+    }
+  c->tune.assign((size_t)mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
+  load_tuning_file(c);
   size_t maxlds = 0;
   for (int k = 0; k < mi::nclass(); ++k) maxlds = std::max(maxlds, mi::wclass_lds_bytes(k));
   hipDeviceProp_t prop;
@@ -225,6 +292,8 @@ int mi_create(int device, mi_ctx** out) {
 void mi_destroy(mi_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
+  for (auto& st : ctx->aux)
+    if (st) hipStreamDestroy(st);
   hipFree(ctx->d_boys);
   mi::free_wtables(ctx->wtables);
   delete ctx;
@@ -316,22 +385,37 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
       t.cls = quartet_class(bra->pc, ket->pc);
       t.nq = t.same ? (int64_t)bra->n * (bra->n + 1) / 2 : (int64_t)bra->n * ket->n;
       const double kavg = ((double)bra->Ksum / bra->n) * ((double)ket->Ksum / ket->n);
-      int G = pow2floor(std::max(1.0, kavg / 4.0));
-      const WClassDesc& D = mi::wclass(t.cls);
-      if (D.nacc > 16) G = std::min(G, 4);
-      t.G = G;
+      const int bb = keys[bra->start].bucket, kb = keys[ket->start].bucket;
+      const TuneEntry& te = ctx->tune[tune_index(t.cls, bb, kb)];
+      int variant, G;
+      if (te.variant >= 0) {
+        variant = te.variant;
+        G = te.G;
+      } else {
+        // heuristic: register kernel when one exists, lanes ~ primitive quartets / 4
+        variant = mi::has_register_kernel(t.cls) ? 1 : 0;
+        G = pow2floor(std::max(1.0, kavg / 4.0));
+        if (mi::wclass(t.cls).nacc > 16) G = std::min(G, 4);
+      }
+      if (variant == 0 && !mi::wclass_fits(t.cls)) {
+        delete P;
+        return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
+      }
+      t.G = variant ? G : 1;
+      t.variant = variant;
       int64_t tprim;
       if (t.same) {
         // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
-        int64_t s = 0, s2 = 0;
+        int64_t sk = 0, sk2 = 0;
         for (int r = bra->start; r < bra->start + bra->n; ++r) {
-          s += keys[r].K;
-          s2 += (int64_t)keys[r].K * keys[r].K;
+          sk += keys[r].K;
+          sk2 += (int64_t)keys[r].K * keys[r].K;
         }
-        tprim = (s * s + s2) / 2;
+        tprim = (sk * sk + sk2) / 2;
       } else {
         tprim = (int64_t)bra->Ksum * ket->Ksum;
       }
+      t.nprim = tprim;
       nq_shell += t.nq;
       nq_prim += tprim;
       cls_shell[t.cls] += t.nq;
@@ -340,25 +424,57 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
     }
   P->nq_shell = nq_shell;
   P->nq_prim = nq_prim;
+  // One launch per (class, variant); within a launch, tasks with the most
+  // work per block come first so the longest blocks start earliest.
   std::vector<TaskDesc> all_tasks;
   for (int c = 0; c < mi::nclass(); ++c) {
-    if (per_class[c].empty()) continue;
-    Launch L;
-    L.cls = c;
-    L.reg = mi::has_register_kernel(c);
-    L.task_off = (int)all_tasks.size();
-    L.ntasks = (int)per_class[c].size();
-    int blocks = 0;
-    for (TaskDesc& t : per_class[c]) {
-      t.blockStart = blocks;
-      const int64_t nb = L.reg ? (t.nq * t.G + 255) / 256 : t.nq;
-      blocks += (int)nb;
-      all_tasks.push_back(t);
+    for (int var = 0; var < 2; ++var) {
+      std::vector<TaskDesc> ts;
+      for (const TaskDesc& t : per_class[c])
+        if (t.variant == var) ts.push_back(t);
+      if (ts.empty()) continue;
+      Launch L{};
+      L.cls = c;
+      L.reg = var == 1;
+      L.task_off = (int)all_tasks.size();
+      L.ntasks = (int)ts.size();
+      int blocks = 0;
+      int64_t qs = 0;
+      for (TaskDesc& t : ts) {
+        t.blockStart = blocks;
+        const int64_t nb = L.reg ? (t.nq * t.G + 255) / 256 : t.nq;
+        blocks += (int)nb;
+        qs += t.nq;
+        all_tasks.push_back(t);
+      }
+      L.nblocks = blocks;
+      L.nq_shell = qs;
+      L.nq_prim = 0;
+      for (const TaskDesc& t : ts) L.nq_prim += t.nprim;
+      L.cost = 0.0;
+      P->launches.push_back(L);
     }
-    L.nblocks = blocks;
-    L.nq_shell = cls_shell[c];
-    L.nq_prim = cls_prim[c];
-    P->launches.push_back(L);
+  }
+  // primitive-quartet counts per launch and a cost estimate (VRR+contraction
+  // work per primitive quartet, HRR+scatter per shell quartet, plus a latency
+  // floor per block wave)
+  for (Launch& L : P->launches) {
+    const WClassDesc& D = mi::wclass(L.cls);
+    L.cost = (double)L.nq_prim * (D.nvrr + D.nacc) + (double)L.nq_shell * (D.nhrr + D.nout) +
+             2.0e4 * (D.nvlev + D.nhlev + 1);
+  }
+  std::stable_sort(P->launches.begin(), P->launches.end(),
+                   [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
+  P->nlanes = std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
+  {
+    std::vector<double> load(P->nlanes, 0.0);
+    for (Launch& L : P->launches) {
+      int best = 0;
+      for (int k = 1; k < P->nlanes; ++k)
+        if (load[k] < load[best]) best = k;
+      L.lane = best;
+      load[best] += L.cost;
+    }
   }
   // Device buffers.
   std::vector<double> cn(b->shell_prim_off[ns]);
@@ -377,6 +493,16 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
     free_plan_buffers(P);
     delete P;
     return rc;
+  }
+  if (P->nlanes > 1) {
+    hipError_t e = hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming);
+    for (int k = 1; e == hipSuccess && k < P->nlanes; ++k)
+      e = hipEventCreateWithFlags(&P->ev_join[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      free_plan_buffers(P);
+      delete P;
+      return set_err(ctx, MI_EDEVICE, "hipEventCreate failed: %s", hipGetErrorString(e));
+    }
   }
   *out = P;
   return MI_OK;
@@ -403,21 +529,63 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
   a.out = out;
   a.M = (int64_t)P->nbf * (P->nbf + 1) / 2;
   a.order = P->order;
+  // Class launches are independent (disjoint output slots): fork them over the
+  // plan's stream lanes and join back into the caller's stream.  Profiling
+  // (ev != NULL) keeps them on one stream to time each launch.
+  const bool fork = !ev && P->nlanes > 1;
+  hipStream_t lanes[MI_MAX_STREAMS] = {s};
+  if (fork) {
+    HIP_TRY(ctx, hipEventRecord(P->ev_fork, s));
+    for (int k = 1; k < P->nlanes; ++k) {
+      lanes[k] = ctx->aux[k - 1];
+      HIP_TRY(ctx, hipStreamWaitEvent(lanes[k], P->ev_fork, 0));
+    }
+  }
   for (size_t i = 0; i < P->launches.size(); ++i) {
     const Launch& L = P->launches[i];
+    hipStream_t ls = fork ? lanes[L.lane] : s;
     a.tasks = P->d_tasks + L.task_off;
     a.ntasks = L.ntasks;
     if (L.reg)
-      HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, s));
+      HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
     else
-      HIP_TRY(ctx, mi::launch_class_wave(L.cls, a, L.nblocks, ctx->wtables, s));
+      HIP_TRY(ctx, mi::launch_class_wave(L.cls, a, L.nblocks, ctx->wtables, ls));
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
   }
+  if (fork)
+    for (int k = 1; k < P->nlanes; ++k) {
+      HIP_TRY(ctx, hipEventRecord(P->ev_join[k], lanes[k]));
+      HIP_TRY(ctx, hipStreamWaitEvent(s, P->ev_join[k], 0));
+    }
   return MI_OK;
 }
 
 int mi_plan_execute(mi_plan* P, double* out, int64_t out_len, void* stream) {
   return plan_run(P, out, out_len, stream, nullptr);
+}
+
+int mi_ctx_set_tuning(mi_ctx* ctx, const int32_t* rows, int32_t nrows) {
+  if (!ctx || (nrows > 0 && !rows)) return MI_EINVAL;
+  ctx->tune.assign((size_t)mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
+  for (int r = 0; r < nrows; ++r) {
+    const int32_t* q = rows + 5 * r;
+    const int cls = q[0], bb = q[1], kb = q[2], var = q[3], G = q[4];
+    if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET)
+      return set_err(ctx, MI_EINVAL, "tuning row %d: bad class/bucket", r);
+    if ((var != 0 && var != 1) || (var == 1 && !mi::has_register_kernel(cls)) || (var == 0 && !mi::wclass_fits(cls)))
+      return set_err(ctx, MI_EINVAL, "tuning row %d: variant %d unavailable for class %d", r, var, cls);
+    if (G < 1 || G > 64 || (G & (G - 1))) return set_err(ctx, MI_EINVAL, "tuning row %d: G=%d", r, G);
+    ctx->tune[tune_index(cls, bb, kb)] = {var, G};
+  }
+  return MI_OK;
+}
+
+int mi_ctx_set_streams(mi_ctx* ctx, int32_t nstreams) {
+  if (!ctx || nstreams < 1) return MI_EINVAL;
+  int avail = 1;
+  while (avail < MI_MAX_STREAMS && ctx->aux[avail - 1]) ++avail;
+  ctx->nstreams = std::min<int>(nstreams, avail);
+  return MI_OK;
 }
 
 int mi_plan_nlaunch(const mi_plan* P) { return P ? (int)P->launches.size() : -1; }

@@ -20,6 +20,7 @@ hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const PairTopo* to
 hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStream_t stream);
 hipError_t launch_class_wave(int c, const EriArgs& args, int nblocks, void* wtables, hipStream_t stream);
 size_t wclass_lds_bytes(int c);
+bool wclass_fits(int c);
 hipError_t set_wave_kernel_lds_limit(size_t bytes);
 
 }  // namespace mi

@@ -37,14 +37,20 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
   const double Ax = shell_xyz[3 * t.shA], Ay = shell_xyz[3 * t.shA + 1], Az = shell_xyz[3 * t.shA + 2];
   const double Bx = shell_xyz[3 * t.shB], By = shell_xyz[3 * t.shB + 1], Bz = shell_xyz[3 * t.shB + 2];
   const double zeta = a + b, ooz = 1.0 / zeta;
-  const double Px = (a * Ax + b * Bx) * ooz, Py = (a * Ay + b * By) * ooz, Pz = (a * Az + b * Bz) * ooz;
   const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
   const double rab2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  // PA = P - A = -(b/zeta) AB and P = A + PA: exactly A for one-centre pairs,
+  // so one-centre quartets keep PA = QC = W - P = 0 without rounding residue
+  // (the textbook (aA + bB)/zeta leaves ~1e-16|A| that tight-exponent
+  // prefactors amplify past 1e-12).
+  const double bz = b * ooz;
+  const double PAx = -bz * ABx, PAy = -bz * ABy, PAz = -bz * ABz;
+  const double Px = Ax + PAx, Py = Ay + PAy, Pz = Az + PAz;
   // 2 pi^{5/2} / (zeta eta sqrt(zeta+eta)) exp(...) split as K_ab K_cd / sqrt(zeta+eta)
   const double kfac = 5.9149671727956129;  // sqrt(2) * pi^{5/4}
   PrimPair o;
   o.Px = Px; o.Py = Py; o.Pz = Pz; o.zeta = zeta;
-  o.PAx = Px - Ax; o.PAy = Py - Ay; o.PAz = Pz - Az;
+  o.PAx = PAx; o.PAy = PAy; o.PAz = PAz;
   o.K = kfac * prim_cn[pa] * prim_cn[pb] * exp(-a * b * rab2 * ooz) * ooz;
   o.oo2z = 0.5 * ooz;
   o.pad0 = o.pad1 = o.pad2 = 0.0;
@@ -57,21 +63,37 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
 }
 
 // ---------------------------------------------------------------------------
-// Wave-cooperative kernel: one 64-lane block per shell quartet.
+// Wave-cooperative kernel: one 64-lane block per shell quartet.  Per primitive
+// quartet the lanes fill the VRR levels of the class (items of one level are
+// independent; a barrier separates levels), each lane keeps up to W_ACC_MAX
+// contracted (e0|f0) values in registers; after the primitive loop the HRR
+// levels run in LDS and the lanes scatter the block.  Item descriptors are
+// loaded four per lane before use so their latencies overlap.
 // ---------------------------------------------------------------------------
+#define W_ACC_MAX 16   // contracted values per lane: nacc <= 1024 (classes up to (dd|dd))
+#define W_SCL 16       // LDS stride of the per-shell scale rows
+
 struct WTablesDev {
-  const uint4_t* vitems;
-  const uint2_t* accmap;
+  const uint2_t* vitems;
+  const uint32_t* accsrc;
   const uint2_t* hitems;
   const uint32_t* outs;
 };
 
+__constant__ double kInvOdd[32] = {
+    1.0 / 1,  1.0 / 3,  1.0 / 5,  1.0 / 7,  1.0 / 9,  1.0 / 11, 1.0 / 13, 1.0 / 15, 1.0 / 17, 1.0 / 19, 1.0 / 21,
+    1.0 / 23, 1.0 / 25, 1.0 / 27, 1.0 / 29, 1.0 / 31, 1.0 / 33, 1.0 / 35, 1.0 / 37, 1.0 / 39, 1.0 / 41, 1.0 / 43,
+    1.0 / 45, 1.0 / 47, 1.0 / 49, 1.0 / 51, 1.0 / 53, 1.0 / 55, 1.0 / 57, 1.0 / 59, 1.0 / 61, 1.0 / 63};
+
 __device__ __forceinline__ double sel3(int i, double x, double y, double z) { return i == 0 ? x : (i == 1 ? y : z); }
+
+__device__ __forceinline__ int ncart_dev(int l) { return (l + 1) * (l + 2) / 2; }
 
 __global__ void __launch_bounds__(64) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W) {
   extern __shared__ double lds[];
   double* vrr = lds;
   double* hrr = lds + D.nvrr;
+  double* scl = hrr + D.nhrr;
   const int lane = threadIdx.x;
   const TaskDesc T = args.tasks[find_task(args)];
   const int64_t qi = (int64_t)(blockIdx.x - T.blockStart);
@@ -82,12 +104,24 @@ __global__ void __launch_bounds__(64) hgp_w_kernel(const EriArgs args, const WCl
   qc.g = 0;
   const PairTopo P = args.topo[qc.bra];
   const PairTopo Q = args.topo[qc.ket];
-  for (int x = lane; x < D.nhrr; x += 64) hrr[x] = 0.0;
+  {
+    const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
+    if (lane < na) scl[lane] = args.bf_scale[P.bfA + lane];
+    if (lane < nb) scl[W_SCL + lane] = args.bf_scale[P.bfB + lane];
+    if (lane < nc) scl[2 * W_SCL + lane] = args.bf_scale[Q.bfA + lane];
+    if (lane < nd) scl[3 * W_SCL + lane] = args.bf_scale[Q.bfB + lane];
+  }
+  int asrc[W_ACC_MAX];
+  double acc[W_ACC_MAX];
+#pragma unroll
+  for (int k = 0; k < W_ACC_MAX; ++k) {
+    const int x = k * 64 + lane;
+    asrc[k] = x < D.nacc ? (int)W.accsrc[D.aoff + x] : 0;
+    acc[k] = 0.0;
+  }
   const int mtop = D.mmax - 1;
   const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
-  const uint4_t* VI = W.vitems + D.voff;
-  const uint2_t* AM = W.accmap + D.aoff;
-  __syncthreads();
+  const uint2_t* VI = W.vitems + D.voff;
   for (int u = 0; u < P.npp; ++u) {
     const PrimPair bp = args.pp[P.ppoff + u];
     for (int v = 0; v < Q.npp; ++v) {
@@ -97,8 +131,8 @@ __global__ void __launch_bounds__(64) hgp_w_kernel(const EriArgs args, const WCl
       const double PQx = bp.Px - kp.Px, PQy = bp.Py - kp.Py, PQz = bp.Pz - kp.Pz;
       const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
       const double pref = bp.K * kp.K * sqrt(inv_s);
-      // Boys: every lane evaluates the top order and the recursion; lane m keeps F_m.
-      if (lane < D.mmax) {
+      // Boys: lane m (< vt0) evaluates the top order and recurses down to m.
+      if (lane < D.vt0) {
         double fm;
         if (T2 < BOYS_TMAX) {
           const int k = (int)(T2 * 16.0 + 0.5);
@@ -114,89 +148,106 @@ __global__ void __launch_bounds__(64) hgp_w_kernel(const EriArgs args, const WCl
           const double eT = r[7] * exp_small(d);
           const double t2 = T2 + T2;
           fm = f;
-          for (int m = mtop - 1; m >= lane; --m) fm = fma(t2, fm, eT) / (double)(2 * m + 1);
+          for (int m = mtop - 1; m >= lane; --m) fm = fma(t2, fm, eT) * kInvOdd[m];
         } else {
           const double rT = 1.0 / T2;
           fm = 0.88622692545275801365 * sqrt(rT);
           const double o2T = 0.5 * rT;
           for (int m = 1; m <= lane; ++m) fm *= (2 * m - 1) * o2T;
         }
-        vrr[D.base00 + lane] = pref * fm;
+        vrr[lane] = pref * fm;
       }
       __syncthreads();
       const double oo2z = bp.oo2z, oo2e = kp.oo2z, oo2s = 0.5 * inv_s;
       const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
+      const double WPx = -eta_s * PQx, WPy = -eta_s * PQy, WPz = -eta_s * PQz;
+      const double WQx = zeta_s * PQx, WQy = zeta_s * PQy, WQz = zeta_s * PQz;
       for (int lv = 0; lv < D.nvlev; ++lv) {
         const int i0 = D.vlev[lv], i1 = D.vlev[lv + 1];
-        for (int it = i0 + lane; it < i1; it += 64) {
-          const uint4_t o = VI[it];
-          const int t = o.x & 0xFFFF, s1 = o.x >> 16;
-          const int dir = o.z & 3;
-          const int kind = (o.z >> 14) & 3;
-          const bool has2 = (o.z >> 16) & 1;
-          const double PQd = sel3(dir, PQx, PQy, PQz);
-          double val;
-          if (kind == 1) {
-            const double PA = sel3(dir, bp.PAx, bp.PAy, bp.PAz);
-            const double WP = -eta * inv_s * PQd;
-            val = PA * vrr[s1] + WP * vrr[s1 + 1];
-            if (has2) {
-              const int s2 = o.y & 0xFFFF;
-              const double n1 = (double)((o.z >> 2) & 63);
-              val += (n1 * oo2z) * (vrr[s2] - eta_s * vrr[s2 + 1]);
-            }
-          } else {
-            const double QC = sel3(dir, kp.PAx, kp.PAy, kp.PAz);
-            const double WQ = zeta * inv_s * PQd;
-            val = QC * vrr[s1] + WQ * vrr[s1 + 1];
-            if ((o.z >> 17) & 1) {
-              const int s3 = o.y >> 16;
-              const double ne = (double)((o.z >> 2) & 63);
-              val += (ne * oo2s) * vrr[s3 + 1];
-            }
-            if (has2) {
-              const int s2 = o.y & 0xFFFF;
-              const double nf = (double)((o.z >> 8) & 63);
-              val += (nf * oo2e) * (vrr[s2] - zeta_s * vrr[s2 + 1]);
+        for (int b = i0 + lane; b < i1; b += 256) {
+          uint2_t o[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int it = b + 64 * k;
+            if (it < i1) o[k] = VI[it]; else o[k] = uint2_t{0u, 0u};
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int it = b + 64 * k;
+            if (it < i1) {
+              const int s1 = o[k].x & 0xFFFF, s2 = o[k].x >> 16, s3 = o[k].y & 0xFFFF;
+              const int dir = (o[k].y >> 16) & 3;
+              const double n1 = (double)((o[k].y >> 18) & 7), n2 = (double)((o[k].y >> 21) & 7);
+              const bool cb = (o[k].y >> 24) & 1;
+              // a-build: PA [s1] + WP [s1+1] + n1/(2 zeta)([s2] - eta/s [s2+1])
+              // c-build: QC [s1] + WQ [s1+1] + n1/(2s) [s3+1] + n2/(2 eta)([s2] - zeta/s [s2+1])
+              const double X1 = cb ? sel3(dir, kp.PAx, kp.PAy, kp.PAz) : sel3(dir, bp.PAx, bp.PAy, bp.PAz);
+              const double X2 = cb ? sel3(dir, WQx, WQy, WQz) : sel3(dir, WPx, WPy, WPz);
+              const double c3 = cb ? n1 * oo2s : 0.0;
+              const double c2 = cb ? n2 * oo2e : n1 * oo2z;
+              const double r2 = cb ? zeta_s : eta_s;
+              double val = X1 * vrr[s1] + X2 * vrr[s1 + 1];
+              val += c3 * vrr[s3 + 1];
+              val += c2 * (vrr[s2] - r2 * vrr[s2 + 1]);
+              vrr[D.vt0 + it] = val;
             }
           }
-          vrr[t] = val;
         }
         __syncthreads();
       }
-      for (int x = lane; x < D.nacc; x += 64) {
-        const uint2_t m = AM[x];
-        hrr[m.y] += vrr[m.x];
-      }
+#pragma unroll
+      for (int k = 0; k < W_ACC_MAX; ++k)
+        if (k * 64 + lane < D.nacc) acc[k] += vrr[asrc[k]];
       __syncthreads();
     }
   }
-  // HRR levels
+#pragma unroll
+  for (int k = 0; k < W_ACC_MAX; ++k)
+    if (k * 64 + lane < D.nacc) hrr[k * 64 + lane] = acc[k];
+  __syncthreads();
+  // HRR levels: item k writes slot nacc + k.
   const PairGeom PG = args.geom[qc.bra];
   const PairGeom QG = args.geom[qc.ket];
   const uint2_t* HI = W.hitems + D.hoff;
   for (int lv = 0; lv < D.nhlev; ++lv) {
     const int i0 = D.hlev[lv], i1 = D.hlev[lv + 1];
-    for (int it = i0 + lane; it < i1; it += 64) {
-      const uint2_t o = HI[it];
-      const int t = o.x & 0xFFFF, s1 = o.x >> 16, s2 = o.y & 0xFFFF;
-      const int j = (o.y >> 16) & 3;
-      const bool ket = (o.y >> 18) & 1;
-      const double X = ket ? sel3(j, QG.ABx, QG.ABy, QG.ABz) : sel3(j, PG.ABx, PG.ABy, PG.ABz);
-      hrr[t] = hrr[s1] + X * hrr[s2];
+    for (int b = i0 + lane; b < i1; b += 256) {
+      uint2_t o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int it = b + 64 * k;
+        if (it < i1) o[k] = HI[it]; else o[k] = uint2_t{0u, 0u};
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int it = b + 64 * k;
+        if (it < i1) {
+          const int s1 = o[k].x & 0xFFFF, s2 = o[k].x >> 16;
+          const int j = o[k].y & 3;
+          const bool ket = (o[k].y >> 2) & 1;
+          const double X = ket ? sel3(j, QG.ABx, QG.ABy, QG.ABz) : sel3(j, PG.ABx, PG.ABy, PG.ABz);
+          hrr[D.nacc + it] = hrr[s1] + X * hrr[s2];
+        }
+      }
     }
     __syncthreads();
   }
   const ScatterCtx sc = make_scatter(args, qc, P, Q);
   const uint32_t* OU = W.outs + D.ooff;
-  for (int x = lane; x < D.nout; x += 64) {
-    const uint32_t o = OU[x];
-    const int off = o & 0xFFFF;
-    const int ma = (o >> 16) & 15, mb = (o >> 20) & 15, mc = (o >> 24) & 15, md = (o >> 28) & 15;
-    const int fa = sc.fa0 + ma, fb = sc.fb0 + mb, fc = sc.fc0 + mc, fd = sc.fd0 + md;
-    const int64_t slot = unique_slot(sc, fa, fb, fc, fd);
-    if (slot >= 0)
-      sc.out[slot] = hrr[off] * (args.bf_scale[fa] * args.bf_scale[fb]) * (args.bf_scale[fc] * args.bf_scale[fd]);
+  for (int b = lane; b < D.nout; b += 256) {
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = (b + 64 * k < D.nout) ? OU[b + 64 * k] : 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (b + 64 * k < D.nout) {
+        const int off = o[k] & 0xFFFF;
+        const int ma = (o[k] >> 16) & 15, mb = (o[k] >> 20) & 15, mc = (o[k] >> 24) & 15, md = (o[k] >> 28) & 15;
+        const int64_t slot = unique_slot(sc, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md);
+        if (slot >= 0)
+          sc.out[slot] = hrr[off] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
+      }
+    }
   }
 }
 
@@ -211,8 +262,8 @@ const WClassDesc& wclass(int c) { return kWClassDesc[c]; }
 bool has_register_kernel(int c) { return kRegisterKernels[c] != nullptr; }
 
 struct WTablesHost {
-  uint4_t* v = nullptr;
-  uint2_t* a = nullptr;
+  uint2_t* v = nullptr;
+  uint32_t* a = nullptr;
   uint2_t* h = nullptr;
   uint32_t* o = nullptr;
 };
@@ -260,8 +311,10 @@ hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStr
 
 size_t wclass_lds_bytes(int c) {
   const WClassDesc& D = kWClassDesc[c];
-  return sizeof(double) * (size_t)(D.nvrr + D.nhrr);
+  return sizeof(double) * (size_t)(D.nvrr + D.nhrr + 4 * W_SCL);
 }
+
+bool wclass_fits(int c) { return kWClassDesc[c].nacc <= 64 * W_ACC_MAX; }
 
 hipError_t launch_class_wave(int c, const EriArgs& args, int nblocks, void* wtables, hipStream_t stream) {
   const WTablesHost* t = static_cast<const WTablesHost*>(wtables);

@@ -20,7 +20,7 @@ MI_PREC_FP64, MI_PREC_MIXED = 0, 1
 # Every symbol include/mi_hgp.h declares (checked by the CPU test suite).
 EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique", "mi_eri_unique",
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
-            "mi_plan_launch_info", "mi_plan_profile"]
+            "mi_plan_launch_info", "mi_plan_profile", "mi_ctx_set_tuning", "mi_ctx_set_streams"]
 
 
 class MiBasis(ctypes.Structure):
@@ -98,6 +98,10 @@ def load():
     L.mi_plan_launch_info.restype = ctypes.c_int
     L.mi_plan_profile.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.POINTER(ctypes.c_float)]
     L.mi_plan_profile.restype = ctypes.c_int
+    L.mi_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
+    L.mi_ctx_set_tuning.restype = ctypes.c_int
+    L.mi_ctx_set_streams.argtypes = [vp, ctypes.c_int32]
+    L.mi_ctx_set_streams.restype = ctypes.c_int
     _lib = L
     return L
 

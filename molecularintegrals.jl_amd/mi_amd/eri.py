@@ -58,6 +58,15 @@ class Context:
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
         return out
 
+    def set_tuning(self, rows):
+        """rows: iterable of (class, bra_bucket, ket_bucket, variant, G)."""
+        arr = np.ascontiguousarray(np.asarray(list(rows), dtype=np.int32).reshape(-1, 5))
+        self.check(self.L.mi_ctx_set_tuning(self.h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            arr.shape[0]))
+
+    def set_streams(self, n: int):
+        self.check(self.L.mi_ctx_set_streams(self.h, int(n)))
+
     def plan(self, fb: FlatBasis, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
              schwarz_cut=0.0) -> "Plan":
         return Plan(self, fb, order, precision, tau, schwarz_cut)

@@ -177,3 +177,46 @@ def test_errors_cross_the_abi_as_codes(ctx):
     out = torch.empty(plan.n_unique + 1, dtype=torch.float64, device="cuda")
     rc = ctx.L.mi_plan_execute(plan.h, __import__("ctypes").c_void_p(out.data_ptr()), plan.n_unique + 1, None)
     assert rc == MI_ESIZE
+
+
+@pytest.mark.parametrize("variant,G", [(0, 1), (1, 1), (1, 4), (1, 64)])
+def test_every_kernel_variant(variant, G):
+    """Force one kernel variant for every class (the autotuner's search space)
+    and check the full array; register kernels also at G lanes per quartet."""
+    from mi_amd import Context, flat_from_shells
+    from mi_amd.hgp_tables import quartet_classes
+    c = Context(0)
+    rows = []
+    for k in range(len(quartet_classes())):
+        for bb in range(5):
+            for kb in range(5):
+                rows.append((k, bb, kb, variant, G))
+    if variant == 1:  # classes without a register kernel keep the wave kernel
+        ok = []
+        for r in rows:
+            try:
+                c.set_tuning([r])
+                ok.append(r)
+            except Exception:
+                ok.append((r[0], r[1], r[2], 0, 1))
+        rows = ok
+    c.set_tuning(rows)
+    rng = np.random.Generator(np.random.PCG64(900 + G))
+    fb = flat_from_shells(random_spec(rng, 8, kmax=4))
+    for order in (O.ORDER_CHRR, O.ORDER_IINDEX):
+        got = c.eri_unique(fb, order=order)
+        ref = O.eri_unique_hgp(fb, order=order)
+        assert_eri_close(got, ref, f"variant={variant} G={G} order={order}", fb, order)
+    c.close()
+
+
+@pytest.mark.parametrize("nstreams", [1, 4])
+def test_stream_fork_join_matches(nstreams):
+    from mi_amd import Context, configs
+    c = Context(0)
+    c.set_streams(nstreams)
+    fb = configs.ethane_split_valence(2).flat()
+    a = c.eri_unique(fb)
+    ref = O.eri_unique_hgp(fb, nthreads=8)
+    assert_eri_close(a, ref, f"streams={nstreams}", fb, O.ORDER_CHRR)
+    c.close()
