@@ -90,7 +90,8 @@ int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
 
 /* Kernel selection (no counterpart in the reference).  rows: nrows x 5 int32
  * (quartet class, bra contraction bucket, ket contraction bucket, variant
- * [1 generated register kernel, 0 LDS wave kernel], lanes per quartet G).
+ * [1 generated register kernel, 0 LDS wave kernel], G = lanes per quartet for
+ * the register kernel or wavefronts per quartet (1, 2, 4) for the wave kernel).
  * Replaces the whole table; unlisted entries use the built-in heuristic.
  * mi_create loads hgp_tuning.txt beside the library (MI_HGP_TUNING overrides).
  * Affects plans created afterwards. */

@@ -291,15 +291,23 @@ def main():
     parts.append("};")
     with open(os.path.join(args.out, "hgp_rkernels.inc"), "w") as f:
         f.write("\n".join(parts) + "\n")
-    # W tables
-    descs, V, A, H, O = [], [], [], [], []
+    # W tables: one contiguous blob per class, 16-byte aligned, laid out as
+    # [VRR items (2 words) | HRR items (2 words) | out words | acc sources],
+    # so a block stages it with wide loads.
+    descs, BLOB = [], []
     for c in classes:
         d, vi, am, hi, ou = w_tables(c)
-        d["voff"], d["aoff"], d["hoff"], d["ooff"] = len(V), len(A), len(H), len(O)
-        V += vi
-        A += am
-        H += hi
-        O += ou
+        d["boff"] = len(BLOB) // 4
+        words = []
+        for x, y in vi:
+            words += [x, y]
+        for x, y in hi:
+            words += [x, y]
+        words += list(ou) + list(am)
+        while len(words) % 4:
+            words.append(0)
+        d["bwords"] = len(words)
+        BLOB += words
         descs.append(d)
     maxlev = max(len(d["vlev"]) for d in descs)
     maxhlev = max(len(d["hlev"]) for d in descs)
@@ -314,36 +322,15 @@ def main():
         hl = list(d["hlev"]) + [d["hlev"][-1]] * (maxhlev - len(d["hlev"]))
         t.append(f"  {{{d['la']}, {d['lb']}, {d['lc']}, {d['ld']}, {d['mmax']}, {d['nvrr']}, {d['nhrr']}, "
                  f"{d['vt0']}, {len(d['vlev']) - 1}, {len(d['hlev']) - 1}, {d['nacc']}, {d['nout']}, "
-                 f"{d['voff']}, {d['aoff']}, {d['hoff']}, {d['ooff']}, "
+                 f"{d['boff']}, {d['bwords']}, "
                  f"{{{', '.join(map(str, vl))}}}, {{{', '.join(map(str, hl))}}}, "
                  f"{1 if c in rcls else 0}}},  // {cname(c)}")
     t.append("};")
-
-    def arr(name, typ, rows, width):
-        t.append(f"static const {typ} {name}[] = {{")
-        line = []
-        for r in rows:
-            if width == 1:
-                line.append(f"{r}u")
-            else:
-                line.append("{" + ", ".join(f"{x}u" for x in r) + "}")
-            if len(line) == 8:
-                t.append("  " + ", ".join(line) + ",")
-                line = []
-        if line:
-            t.append("  " + ", ".join(line) + ",")
-        if not rows:
-            t.append("  " + ("0u" if width == 1 else "{" + ", ".join(["0u"] * width) + "}"))
-        t.append("};")
-
-    arr("kWVItems", "uint2_t", V, 2)
-    arr("kWAccMap", "uint32_t", A, 1)
-    arr("kWHItems", "uint2_t", H, 2)
-    arr("kWOut", "uint32_t", O, 1)
-    t.append(f"static const int kWVItemsN = {len(V)};")
-    t.append(f"static const int kWAccMapN = {len(A)};")
-    t.append(f"static const int kWHItemsN = {len(H)};")
-    t.append(f"static const int kWOutN = {len(O)};")
+    t.append("static const uint32_t kWBlob[] __attribute__((aligned(16))) = {")
+    for k in range(0, len(BLOB), 12):
+        t.append("  " + ", ".join(f"{x}u" for x in BLOB[k:k + 12]) + ",")
+    t.append("};")
+    t.append(f"static const int kWBlobWords = {len(BLOB)};")
     with open(os.path.join(args.out, "hgp_wtables.inc"), "w") as f:
         f.write("\n".join(t) + "\n")
     print(f"gen_hgp: {len(classes)} classes, {len(rcls)} register kernels: "

@@ -85,7 +85,7 @@ struct WClassDesc {
   int la, lb, lc, ld, mmax;
   int nvrr, nhrr, vt0;       // vt0: LDS slot written by VRR item 0
   int nvlev, nhlev, nacc, nout;
-  int voff, aoff, hoff, ooff;
+  int boff, bwords;          // op-table blob: 16-byte offset and 32-bit word count
   int vlev[16];
   int hlev[8];
   int is_register;
@@ -143,11 +143,28 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
   }
 }
 
+// Copy n 16-byte chunks global -> LDS with all threads of the block, eight
+// independent loads in flight per thread before the stores.
+__device__ __forceinline__ void stage_chunks(uint4* __restrict__ dst, const uint4* __restrict__ src, int n) {
+  for (int base = threadIdx.x; base < n; base += 8 * blockDim.x) {
+    uint4 r[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = base + k * blockDim.x;
+      if (i < n) r[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = base + k * blockDim.x;
+      if (i < n) dst[i] = r[k];
+    }
+  }
+}
+
 // Copy the Boys slice of one top order into LDS (all threads of the block).
 __device__ __forceinline__ void stage_boys_slice(double* lds, const double* __restrict__ boys, int mtop) {
-  const double2* src = reinterpret_cast<const double2*>(boys + (size_t)mtop * BOYS_NROW * 8);
-  double2* dst = reinterpret_cast<double2*>(lds);
-  for (int i = threadIdx.x; i < BOYS_NROW * 4; i += blockDim.x) dst[i] = src[i];
+  stage_chunks(reinterpret_cast<uint4*>(lds), reinterpret_cast<const uint4*>(boys + (size_t)mtop * BOYS_NROW * 8),
+               BOYS_NROW * 4);
   __syncthreads();
 }
 

@@ -42,6 +42,7 @@ struct mi_ctx {
   void* wtables = nullptr;
   std::string err;
   size_t lds_limit = 0;
+  int ncu = 256;
   int nstreams = MI_MAX_STREAMS;              // class launches spread over this many streams
   hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
   std::vector<TuneEntry> tune;                // [class][bra bucket][ket bucket]
@@ -148,8 +149,8 @@ void load_tuning_file(mi_ctx* c) {
     if (!(is >> cls >> bb >> kb >> var >> G)) continue;
     if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET) continue;
     if (var == 1 && !mi::has_register_kernel(cls)) continue;
-    if (var == 0 && !mi::wclass_fits(cls)) continue;
     if (G < 1 || G > 64 || (G & (G - 1))) continue;
+    if (var == 0 && !mi::wclass_fits_nw(cls, G, c->lds_limit)) continue;
     c->tune[tune_index(cls, bb, kb)] = {var, G};
   }
 }
@@ -159,6 +160,7 @@ void load_tuning_file(mi_ctx* c) {
 struct Launch {
   int cls;
   bool reg;
+  int nw;  // wave kernel: wavefronts per block
   int task_off, ntasks, nblocks;
   int64_t nq_shell, nq_prim;
   double cost;  // relative device time estimate (longest launches go first)
@@ -278,13 +280,16 @@ int mi_create(int device, mi_ctx** out) {
       c->nstreams = k + 1;
       break;
     }
+  hipDeviceProp_t prop;
+  c->lds_limit = 65536;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->ncu = prop.multiProcessorCount;
+    c->lds_limit = std::max<size_t>(prop.sharedMemPerBlock, prop.maxSharedMemoryPerMultiProcessor);
+  }
+  c->lds_limit = std::min<size_t>(c->lds_limit, 160 * 1024);  // CDNA4: 160 KiB LDS per CU
+  if (c->lds_limit > 65536 && mi::set_wave_kernel_lds_limit(c->lds_limit) != hipSuccess) c->lds_limit = 65536;
   c->tune.assign((size_t)mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
   load_tuning_file(c);
-  size_t maxlds = 0;
-  for (int k = 0; k < mi::nclass(); ++k) maxlds = std::max(maxlds, mi::wclass_lds_bytes(k));
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->lds_limit = prop.sharedMemPerBlock;
-  if (maxlds > 65536) mi::set_wave_kernel_lds_limit(maxlds);
   *out = c;
   return MI_OK;
 }
@@ -392,16 +397,22 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
         variant = te.variant;
         G = te.G;
       } else {
-        // heuristic: register kernel when one exists, lanes ~ primitive quartets / 4
+        // heuristic: register kernel when one exists, lanes ~ primitive quartets / 4;
+        // wave kernel with as many wavefronts (<= 4) as the LDS budget allows
         variant = mi::has_register_kernel(t.cls) ? 1 : 0;
-        G = pow2floor(std::max(1.0, kavg / 4.0));
-        if (mi::wclass(t.cls).nacc > 16) G = std::min(G, 4);
+        if (variant) {
+          G = pow2floor(std::max(1.0, kavg / 4.0));
+          if (mi::wclass(t.cls).nacc > 16) G = std::min(G, 4);
+        } else {
+          G = 4;
+          while (G > 1 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) G >>= 1;
+        }
       }
-      if (variant == 0 && !mi::wclass_fits(t.cls)) {
+      if (variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) {
         delete P;
         return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
       }
-      t.G = variant ? G : 1;
+      t.G = G;
       t.variant = variant;
       int64_t tprim;
       if (t.same) {
@@ -428,26 +439,33 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
   // work per block come first so the longest blocks start earliest.
   std::vector<TaskDesc> all_tasks;
   for (int c = 0; c < mi::nclass(); ++c) {
-    for (int var = 0; var < 2; ++var) {
+    for (int vk = 0; vk < 4; ++vk) {  // 0: register kernel; 1,2,3: wave kernel with 1,2,4 wavefronts
+      const int var = vk == 0 ? 1 : 0, nw = vk == 0 ? 0 : (1 << (vk - 1));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
-        if (t.variant == var) ts.push_back(t);
+        if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
       if (ts.empty()) continue;
       Launch L{};
       L.cls = c;
       L.reg = var == 1;
+      L.nw = nw;
       L.task_off = (int)all_tasks.size();
       L.ntasks = (int)ts.size();
-      int blocks = 0;
+      int64_t blocks = 0;
       int64_t qs = 0;
       for (TaskDesc& t : ts) {
-        t.blockStart = blocks;
-        const int64_t nb = L.reg ? (t.nq * t.G + 255) / 256 : t.nq;
-        blocks += (int)nb;
+        // register kernel: first block of the task; wave kernel: first quartet
+        t.blockStart = (int32_t)(L.reg ? blocks : qs);
+        blocks += L.reg ? (t.nq * t.G + 255) / 256 : 0;
         qs += t.nq;
         all_tasks.push_back(t);
       }
-      L.nblocks = blocks;
+      if (!L.reg) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
+      if (blocks > INT32_MAX || qs > INT32_MAX) {
+        delete P;
+        return set_err(ctx, MI_ENOMEM, "class %d: %lld quartets exceed one launch", c, (long long)qs);
+      }
+      L.nblocks = (int)blocks;
       L.nq_shell = qs;
       L.nq_prim = 0;
       for (const TaskDesc& t : ts) L.nq_prim += t.nprim;
@@ -549,7 +567,7 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
     if (L.reg)
       HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
     else
-      HIP_TRY(ctx, mi::launch_class_wave(L.cls, a, L.nblocks, ctx->wtables, ls));
+      HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
   }
   if (fork)
@@ -572,9 +590,10 @@ int mi_ctx_set_tuning(mi_ctx* ctx, const int32_t* rows, int32_t nrows) {
     const int cls = q[0], bb = q[1], kb = q[2], var = q[3], G = q[4];
     if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET)
       return set_err(ctx, MI_EINVAL, "tuning row %d: bad class/bucket", r);
-    if ((var != 0 && var != 1) || (var == 1 && !mi::has_register_kernel(cls)) || (var == 0 && !mi::wclass_fits(cls)))
-      return set_err(ctx, MI_EINVAL, "tuning row %d: variant %d unavailable for class %d", r, var, cls);
     if (G < 1 || G > 64 || (G & (G - 1))) return set_err(ctx, MI_EINVAL, "tuning row %d: G=%d", r, G);
+    if ((var != 0 && var != 1) || (var == 1 && !mi::has_register_kernel(cls)) ||
+        (var == 0 && !mi::wclass_fits_nw(cls, G, ctx->lds_limit)))
+      return set_err(ctx, MI_EINVAL, "tuning row %d: variant %d (G=%d) unavailable for class %d", r, var, G, cls);
     ctx->tune[tune_index(cls, bb, kb)] = {var, G};
   }
   return MI_OK;

@@ -74,10 +74,7 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
 #define W_SCL 16       // LDS stride of the per-shell scale rows
 
 struct WTablesDev {
-  const uint2_t* vitems;
-  const uint32_t* accsrc;
-  const uint2_t* hitems;
-  const uint32_t* outs;
+  const uint32_t* blob;  // kWBlob on the device
 };
 
 __constant__ double kInvOdd[32] = {
@@ -89,42 +86,65 @@ __device__ __forceinline__ double sel3(int i, double x, double y, double z) { re
 
 __device__ __forceinline__ int ncart_dev(int l) { return (l + 1) * (l + 2) / 2; }
 
-__global__ void __launch_bounds__(64) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W) {
+// LDS layout of a wave-kernel block: the class's op tables (staged once per
+// block, then read at LDS latency), followed by the per-quartet work areas.
+__host__ __device__ __forceinline__ int w_desc_doubles(const WClassDesc& D) { return D.bwords / 2; }
+
+// Persistent: each block stages the tables, then walks quartets
+// blockIdx.x, blockIdx.x + gridDim.x, ... of the launch (tasks hold the
+// first launch-quartet index of each task in blockStart).  A block has nw
+// wavefronts: wave w evaluates primitive quartets w, w+nw, ... in its own
+// VRR area (levels separated by wave-local syncs) and keeps its share of the
+// contracted (e0|f0) in registers; the shares are summed in wave order (a
+// fixed order: results do not depend on scheduling), then all waves run the
+// HRR levels and the scatter.
+// LDS: [op tables][hrr: nhrr][scales: 4*W_SCL][vrr: nw * nvrr]
+__global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W,
+                                                    int64_t nq_launch) {
   extern __shared__ double lds[];
-  double* vrr = lds;
-  double* hrr = lds + D.nvrr;
+  const int nv = D.vlev[D.nvlev], nh = D.hlev[D.nhlev];
+  uint2_t* sV = reinterpret_cast<uint2_t*>(lds);
+  uint2_t* sH = sV + nv;
+  uint32_t* sO = reinterpret_cast<uint32_t*>(sH + nh);
+  uint32_t* sA = sO + D.nout;
+  double* hrr = lds + w_desc_doubles(D);
   double* scl = hrr + D.nhrr;
-  const int lane = threadIdx.x;
-  const TaskDesc T = args.tasks[find_task(args)];
-  const int64_t qi = (int64_t)(blockIdx.x - T.blockStart);
-  if (qi >= T.nq) return;  // uniform per block
-  QuartetCtx qc;
-  decode_index(T, qi, qc);
-  qc.G = 1;
-  qc.g = 0;
-  const PairTopo P = args.topo[qc.bra];
-  const PairTopo Q = args.topo[qc.ket];
-  {
-    const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
-    if (lane < na) scl[lane] = args.bf_scale[P.bfA + lane];
-    if (lane < nb) scl[W_SCL + lane] = args.bf_scale[P.bfB + lane];
-    if (lane < nc) scl[2 * W_SCL + lane] = args.bf_scale[Q.bfA + lane];
-    if (lane < nd) scl[3 * W_SCL + lane] = args.bf_scale[Q.bfB + lane];
-  }
+  double* vrr_all = scl + 4 * W_SCL;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int nw = nthr >> 6, wv = tid >> 6, lane = tid & 63;
+  double* vrr = vrr_all + wv * D.nvrr;
+  stage_chunks(reinterpret_cast<uint4*>(lds), reinterpret_cast<const uint4*>(W.blob) + D.boff, D.bwords / 4);
+  __syncthreads();
+  const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
+  const int mtop = D.mmax - 1;
+  const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
   int asrc[W_ACC_MAX];
-  double acc[W_ACC_MAX];
 #pragma unroll
   for (int k = 0; k < W_ACC_MAX; ++k) {
     const int x = k * 64 + lane;
-    asrc[k] = x < D.nacc ? (int)W.accsrc[D.aoff + x] : 0;
-    acc[k] = 0.0;
+    asrc[k] = x < D.nacc ? (int)sA[x] : 0;
   }
-  const int mtop = D.mmax - 1;
-  const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
-  const uint2_t* VI = W.vitems + D.voff;
-  for (int u = 0; u < P.npp; ++u) {
-    const PrimPair bp = args.pp[P.ppoff + u];
-    for (int v = 0; v < Q.npp; ++v) {
+  int ti = 0;
+  for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
+    while (ti + 1 < args.ntasks && args.tasks[ti + 1].blockStart <= q) ++ti;
+    const TaskDesc T = args.tasks[ti];
+    QuartetCtx qc;
+    decode_index(T, q - T.blockStart, qc);
+    qc.G = 1;
+    qc.g = 0;
+    const PairTopo P = args.topo[qc.bra];
+    const PairTopo Q = args.topo[qc.ket];
+    if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
+    if (tid < nb) scl[W_SCL + tid] = args.bf_scale[P.bfB + tid];
+    if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
+    if (tid < nd) scl[3 * W_SCL + tid] = args.bf_scale[Q.bfB + tid];
+    double acc[W_ACC_MAX];
+#pragma unroll
+    for (int k = 0; k < W_ACC_MAX; ++k) acc[k] = 0.0;
+    const int K = P.npp * Q.npp;
+    for (int t = wv; t < K; t += nw) {
+      const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
+      const PrimPair bp = args.pp[P.ppoff + u];
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
       const double inv_s = 1.0 / (zeta + eta);
@@ -157,97 +177,75 @@ __global__ void __launch_bounds__(64) hgp_w_kernel(const EriArgs args, const WCl
         }
         vrr[lane] = pref * fm;
       }
-      __syncthreads();
+      __syncwarp();
       const double oo2z = bp.oo2z, oo2e = kp.oo2z, oo2s = 0.5 * inv_s;
       const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
       const double WPx = -eta_s * PQx, WPy = -eta_s * PQy, WPz = -eta_s * PQz;
       const double WQx = zeta_s * PQx, WQy = zeta_s * PQy, WQz = zeta_s * PQz;
       for (int lv = 0; lv < D.nvlev; ++lv) {
-        const int i0 = D.vlev[lv], i1 = D.vlev[lv + 1];
-        for (int b = i0 + lane; b < i1; b += 256) {
-          uint2_t o[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int it = b + 64 * k;
-            if (it < i1) o[k] = VI[it]; else o[k] = uint2_t{0u, 0u};
-          }
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int it = b + 64 * k;
-            if (it < i1) {
-              const int s1 = o[k].x & 0xFFFF, s2 = o[k].x >> 16, s3 = o[k].y & 0xFFFF;
-              const int dir = (o[k].y >> 16) & 3;
-              const double n1 = (double)((o[k].y >> 18) & 7), n2 = (double)((o[k].y >> 21) & 7);
-              const bool cb = (o[k].y >> 24) & 1;
-              // a-build: PA [s1] + WP [s1+1] + n1/(2 zeta)([s2] - eta/s [s2+1])
-              // c-build: QC [s1] + WQ [s1+1] + n1/(2s) [s3+1] + n2/(2 eta)([s2] - zeta/s [s2+1])
-              const double X1 = cb ? sel3(dir, kp.PAx, kp.PAy, kp.PAz) : sel3(dir, bp.PAx, bp.PAy, bp.PAz);
-              const double X2 = cb ? sel3(dir, WQx, WQy, WQz) : sel3(dir, WPx, WPy, WPz);
-              const double c3 = cb ? n1 * oo2s : 0.0;
-              const double c2 = cb ? n2 * oo2e : n1 * oo2z;
-              const double r2 = cb ? zeta_s : eta_s;
-              double val = X1 * vrr[s1] + X2 * vrr[s1 + 1];
-              val += c3 * vrr[s3 + 1];
-              val += c2 * (vrr[s2] - r2 * vrr[s2 + 1]);
-              vrr[D.vt0 + it] = val;
-            }
-          }
+        const int i1 = D.vlev[lv + 1];
+        for (int it = D.vlev[lv] + lane; it < i1; it += 64) {
+          const uint2_t o = sV[it];
+          const int s1 = o.x & 0xFFFF, s2 = o.x >> 16, s3 = o.y & 0xFFFF;
+          const int dir = (o.y >> 16) & 3;
+          const double n1 = (double)((o.y >> 18) & 7), n2 = (double)((o.y >> 21) & 7);
+          const bool cb = (o.y >> 24) & 1;
+          // a-build: PA [s1] + WP [s1+1] + n1/(2 zeta)([s2] - eta/s [s2+1])
+          // c-build: QC [s1] + WQ [s1+1] + n1/(2s) [s3+1] + n2/(2 eta)([s2] - zeta/s [s2+1])
+          const double X1 = cb ? sel3(dir, kp.PAx, kp.PAy, kp.PAz) : sel3(dir, bp.PAx, bp.PAy, bp.PAz);
+          const double X2 = cb ? sel3(dir, WQx, WQy, WQz) : sel3(dir, WPx, WPy, WPz);
+          const double c3 = cb ? n1 * oo2s : 0.0;
+          const double c2 = cb ? n2 * oo2e : n1 * oo2z;
+          const double r2 = cb ? zeta_s : eta_s;
+          double val = X1 * vrr[s1] + X2 * vrr[s1 + 1];
+          val += c3 * vrr[s3 + 1];
+          val += c2 * (vrr[s2] - r2 * vrr[s2 + 1]);
+          vrr[D.vt0 + it] = val;
         }
-        __syncthreads();
+        __syncwarp();
       }
 #pragma unroll
       for (int k = 0; k < W_ACC_MAX; ++k)
         if (k * 64 + lane < D.nacc) acc[k] += vrr[asrc[k]];
-      __syncthreads();
+      __syncwarp();
     }
-  }
+    // per-wave shares -> own VRR area, then summed in wave order into hrr[0..nacc)
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < W_ACC_MAX; ++k)
-    if (k * 64 + lane < D.nacc) hrr[k * 64 + lane] = acc[k];
-  __syncthreads();
-  // HRR levels: item k writes slot nacc + k.
-  const PairGeom PG = args.geom[qc.bra];
-  const PairGeom QG = args.geom[qc.ket];
-  const uint2_t* HI = W.hitems + D.hoff;
-  for (int lv = 0; lv < D.nhlev; ++lv) {
-    const int i0 = D.hlev[lv], i1 = D.hlev[lv + 1];
-    for (int b = i0 + lane; b < i1; b += 256) {
-      uint2_t o[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int it = b + 64 * k;
-        if (it < i1) o[k] = HI[it]; else o[k] = uint2_t{0u, 0u};
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int it = b + 64 * k;
-        if (it < i1) {
-          const int s1 = o[k].x & 0xFFFF, s2 = o[k].x >> 16;
-          const int j = o[k].y & 3;
-          const bool ket = (o[k].y >> 2) & 1;
-          const double X = ket ? sel3(j, QG.ABx, QG.ABy, QG.ABz) : sel3(j, PG.ABx, PG.ABy, PG.ABz);
-          hrr[D.nacc + it] = hrr[s1] + X * hrr[s2];
-        }
-      }
+    for (int k = 0; k < W_ACC_MAX; ++k)
+      if (k * 64 + lane < D.nacc) vrr[k * 64 + lane] = acc[k];
+    __syncthreads();
+    for (int x = tid; x < D.nacc; x += nthr) {
+      double sum = vrr_all[x];
+      for (int w = 1; w < nw; ++w) sum += vrr_all[w * D.nvrr + x];
+      hrr[x] = sum;
     }
     __syncthreads();
-  }
-  const ScatterCtx sc = make_scatter(args, qc, P, Q);
-  const uint32_t* OU = W.outs + D.ooff;
-  for (int b = lane; b < D.nout; b += 256) {
-    uint32_t o[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) o[k] = (b + 64 * k < D.nout) ? OU[b + 64 * k] : 0u;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (b + 64 * k < D.nout) {
-        const int off = o[k] & 0xFFFF;
-        const int ma = (o[k] >> 16) & 15, mb = (o[k] >> 20) & 15, mc = (o[k] >> 24) & 15, md = (o[k] >> 28) & 15;
-        const int64_t slot = unique_slot(sc, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md);
-        if (slot >= 0)
-          sc.out[slot] = hrr[off] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
+    // HRR levels: item k writes slot nacc + k.
+    const PairGeom PG = args.geom[qc.bra];
+    const PairGeom QG = args.geom[qc.ket];
+    for (int lv = 0; lv < D.nhlev; ++lv) {
+      const int i1 = D.hlev[lv + 1];
+      for (int it = D.hlev[lv] + tid; it < i1; it += nthr) {
+        const uint2_t o = sH[it];
+        const int s1 = o.x & 0xFFFF, s2 = o.x >> 16;
+        const int j = o.y & 3;
+        const bool ket = (o.y >> 2) & 1;
+        const double X = ket ? sel3(j, QG.ABx, QG.ABy, QG.ABz) : sel3(j, PG.ABx, PG.ABy, PG.ABz);
+        hrr[D.nacc + it] = hrr[s1] + X * hrr[s2];
       }
+      __syncthreads();
     }
+    const ScatterCtx sc = make_scatter(args, qc, P, Q);
+    for (int x = tid; x < D.nout; x += nthr) {
+      const uint32_t o = sO[x];
+      const int off = o & 0xFFFF;
+      const int ma = (o >> 16) & 15, mb = (o >> 20) & 15, mc = (o >> 24) & 15, md = (o >> 28) & 15;
+      const int64_t slot = unique_slot(sc, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md);
+      if (slot >= 0)
+        sc.out[slot] = hrr[off] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
+    }
+    __syncthreads();
   }
 }
 
@@ -262,23 +260,14 @@ const WClassDesc& wclass(int c) { return kWClassDesc[c]; }
 bool has_register_kernel(int c) { return kRegisterKernels[c] != nullptr; }
 
 struct WTablesHost {
-  uint2_t* v = nullptr;
-  uint32_t* a = nullptr;
-  uint2_t* h = nullptr;
-  uint32_t* o = nullptr;
+  uint32_t* blob = nullptr;
 };
 
 hipError_t upload_wtables(void** handle) {
   WTablesHost* t = new WTablesHost();
   hipError_t e;
-  if ((e = hipMalloc(&t->v, sizeof(kWVItems))) != hipSuccess) return e;
-  if ((e = hipMalloc(&t->a, sizeof(kWAccMap))) != hipSuccess) return e;
-  if ((e = hipMalloc(&t->h, sizeof(kWHItems))) != hipSuccess) return e;
-  if ((e = hipMalloc(&t->o, sizeof(kWOut))) != hipSuccess) return e;
-  if ((e = hipMemcpy(t->v, kWVItems, sizeof(kWVItems), hipMemcpyHostToDevice)) != hipSuccess) return e;
-  if ((e = hipMemcpy(t->a, kWAccMap, sizeof(kWAccMap), hipMemcpyHostToDevice)) != hipSuccess) return e;
-  if ((e = hipMemcpy(t->h, kWHItems, sizeof(kWHItems), hipMemcpyHostToDevice)) != hipSuccess) return e;
-  if ((e = hipMemcpy(t->o, kWOut, sizeof(kWOut), hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMalloc(&t->blob, sizeof(kWBlob))) != hipSuccess) return e;
+  if ((e = hipMemcpy(t->blob, kWBlob, sizeof(kWBlob), hipMemcpyHostToDevice)) != hipSuccess) return e;
   *handle = t;
   return hipSuccess;
 }
@@ -286,10 +275,7 @@ hipError_t upload_wtables(void** handle) {
 void free_wtables(void* handle) {
   WTablesHost* t = static_cast<WTablesHost*>(handle);
   if (!t) return;
-  hipFree(t->v);
-  hipFree(t->a);
-  hipFree(t->h);
-  hipFree(t->o);
+  hipFree(t->blob);
   delete t;
 }
 
@@ -309,19 +295,32 @@ hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStr
   return hipLaunchKernel(fn, dim3(nblocks), dim3(256), kargs, 0, stream);
 }
 
-size_t wclass_lds_bytes(int c) {
+size_t wclass_lds_bytes(int c, int nw) {
   const WClassDesc& D = kWClassDesc[c];
-  return sizeof(double) * (size_t)(D.nvrr + D.nhrr + 4 * W_SCL);
+  return sizeof(double) * (size_t)(w_desc_doubles(D) + D.nhrr + 4 * W_SCL + (size_t)nw * D.nvrr);
+}
+
+int wave_kernel_blocks_per_cu(int c, int nw) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)hgp_w_kernel, 64 * nw,
+                                                   wclass_lds_bytes(c, nw)) != hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
 }
 
 bool wclass_fits(int c) { return kWClassDesc[c].nacc <= 64 * W_ACC_MAX; }
 
-hipError_t launch_class_wave(int c, const EriArgs& args, int nblocks, void* wtables, hipStream_t stream) {
+bool wclass_fits_nw(int c, int nw, size_t lds_limit) {
+  return wclass_fits(c) && nw >= 1 && nw <= 4 && wclass_lds_bytes(c, nw) <= lds_limit;
+}
+
+hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
+                             hipStream_t stream) {
   const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
-  WTablesDev W{t->v, t->a, t->h, t->o};
+  WTablesDev W{t->blob};
   const WClassDesc D = kWClassDesc[c];
-  const size_t lds = wclass_lds_bytes(c);
-  hipLaunchKernelGGL(hgp_w_kernel, dim3(nblocks), dim3(64), lds, stream, args, D, W);
+  const size_t lds = wclass_lds_bytes(c, nw);
+  hipLaunchKernelGGL(hgp_w_kernel, dim3(nblocks), dim3(64 * nw), lds, stream, args, D, W, nq_launch);
   return hipGetLastError();
 }
 

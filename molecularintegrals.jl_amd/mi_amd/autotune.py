@@ -2,10 +2,10 @@
 
 For every quartet class it picks, on the GPU it runs on:
   * the kernel variant: generated register kernel (lane group per shell
-    quartet) or the LDS wave kernel (one wavefront per shell quartet), and
+    quartet) or the LDS wave kernel (1, 2 or 4 wavefronts per shell quartet), and
   * for the register kernel, the lanes per quartet G as a function of the
     contraction depth: G = pow2floor(K_bra * K_ket / tau), clamped to [1, 64],
-    with tau chosen per class.
+    with tau chosen per class; for the wave kernel, the wavefront count.
 Method: each candidate setting is applied to all classes at once, one
 execute is profiled with HIP events between launches (median of ``reps``),
 and each class keeps its fastest candidate.  The result is written as
@@ -37,14 +37,18 @@ def pow2floor(x):
     return g
 
 
-def rows_for(cls_choice, has_reg):
+def rows_for(cls_choice, has_reg, w_ok):
+    """cls_choice[c] = (variant, tau) for the register kernel or (0, nw) for
+    the wave kernel with nw wavefronts per quartet."""
     rows = []
     for c, (var, tau) in cls_choice.items():
         if var == 1 and not has_reg[c]:
-            var = 0
+            var, tau = 0, max([n for n in (1, 2, 4) if w_ok[c].get(n)] or [1])
+        if var == 0 and not w_ok[c].get(int(tau), False):
+            tau = max([n for n in (1, 2, 4) if w_ok[c].get(n)] or [1])
         for bb in range(NBUCKET):
             for kb in range(NBUCKET):
-                G = pow2floor(max(1.0, BUCKET_K[bb] * BUCKET_K[kb] / tau)) if var == 1 else 1
+                G = pow2floor(max(1.0, BUCKET_K[bb] * BUCKET_K[kb] / tau)) if var == 1 else int(tau)
                 rows.append((c, bb, kb, var, G))
     return rows
 
@@ -70,24 +74,31 @@ def autotune(workloads, reps=7, verbose=True):
     cid = {c: k for k, c in enumerate(classes)}
     ctx = Context(0)
     ctx.set_streams(1)
-    # which classes have register kernels: probe the library by setting a row
-    has_reg = {}
+    # which variants each class has: probe the library by setting a row
+    has_reg, w_ok = {}, {}
     for k in range(len(classes)):
         try:
             ctx.set_tuning([(k, 0, 0, 1, 1)])
             has_reg[k] = True
         except Exception:
             has_reg[k] = False
-    cands = [(0, None)] + [(1, t) for t in TAUS]
+        w_ok[k] = {}
+        for nw in (1, 2, 4):
+            try:
+                ctx.set_tuning([(k, 0, 0, 0, nw)])
+                w_ok[k][nw] = True
+            except Exception:
+                w_ok[k][nw] = False
+    cands = [(0, 1), (0, 2), (0, 4)] + [(1, t) for t in TAUS]
     total = {k: {} for k in range(len(classes))}
     for var, tau in cands:
-        choice = {k: (var, tau if tau else 1) for k in range(len(classes))}
-        rows = rows_for(choice, has_reg)
+        choice = {k: (var, tau) for k in range(len(classes))}
+        rows = rows_for(choice, has_reg, w_ok)
         for fb in workloads:
             per = class_times(ctx, fb, rows, reps)
             for cls, t in per.items():
                 k = cid[cls]
-                if var == 1 and not has_reg[k]:
+                if (var == 1 and not has_reg[k]) or (var == 0 and not w_ok[k].get(tau)):
                     continue
                 total[k][(var, tau)] = total[k].get((var, tau), 0.0) + t
     best = {}
@@ -96,14 +107,14 @@ def autotune(workloads, reps=7, verbose=True):
             best[k] = (1 if has_reg[k] else 0, 4)
             continue
         (var, tau), t = min(total[k].items(), key=lambda kv: kv[1])
-        best[k] = (var, tau if tau else 1)
+        best[k] = (var, tau)
         if verbose:
-            alt = ", ".join(f"{'W' if v == 0 else 'R/' + str(tt if tt < 1e6 else 'inf')}:{x * 1e3:.1f}"
+            alt = ", ".join(f"{'W' + str(tt) if v == 0 else 'R/' + str(tt if tt < 1e6 else 'inf')}:{x * 1e3:.1f}"
                             for (v, tt), x in sorted(total[k].items(), key=lambda kv: kv[1])[:4])
             print(f"[autotune] {''.join('spdf'[x] for x in classes[k])}: "
-                  f"{'W' if var == 0 else 'R tau=' + str(tau)}  ({alt} us)", flush=True)
+                  f"{'W nw=' + str(tau) if var == 0 else 'R tau=' + str(tau)}  ({alt} us)", flush=True)
     ctx.close()
-    return rows_for(best, has_reg), best
+    return rows_for(best, has_reg, w_ok), best
 
 
 def main():

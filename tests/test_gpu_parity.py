@@ -179,7 +179,7 @@ def test_errors_cross_the_abi_as_codes(ctx):
     assert rc == MI_ESIZE
 
 
-@pytest.mark.parametrize("variant,G", [(0, 1), (1, 1), (1, 4), (1, 64)])
+@pytest.mark.parametrize("variant,G", [(0, 1), (0, 2), (0, 4), (1, 1), (1, 4), (1, 64)])
 def test_every_kernel_variant(variant, G):
     """Force one kernel variant for every class (the autotuner's search space)
     and check the full array; register kernels also at G lanes per quartet."""
@@ -191,15 +191,14 @@ def test_every_kernel_variant(variant, G):
         for bb in range(5):
             for kb in range(5):
                 rows.append((k, bb, kb, variant, G))
-    if variant == 1:  # classes without a register kernel keep the wave kernel
-        ok = []
-        for r in rows:
-            try:
-                c.set_tuning([r])
-                ok.append(r)
-            except Exception:
-                ok.append((r[0], r[1], r[2], 0, 1))
-        rows = ok
+    ok = []  # classes without the variant keep a wave kernel that fits
+    for r in rows:
+        try:
+            c.set_tuning([r])
+            ok.append(r)
+        except Exception:
+            ok.append((r[0], r[1], r[2], 0, 1))
+    rows = ok
     c.set_tuning(rows)
     rng = np.random.Generator(np.random.PCG64(900 + G))
     fb = flat_from_shells(random_spec(rng, 8, kmax=4))
