@@ -113,6 +113,32 @@ int mi_eri_unique(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, doubl
 int mi_plan_create(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, mi_plan **out);
 int mi_plan_execute(mi_plan *plan, double *out_device, int64_t out_len, void *stream);
 void mi_plan_destroy(mi_plan *plan);
+/* Batched plan over nmol molecules (configuration "many molecules of uneven
+ * size in one launch sequence"; no counterpart in the reference, which
+ * handles one Basis per call).  Shell pairs and quartets stay inside each
+ * molecule; the output is the molecules' unique arrays back to back, molecule
+ * m starting at offsets[m] (mi_plan_mol_offsets, nmol+1 entries). */
+int mi_batch_plan_create(mi_ctx *ctx, const mi_basis *bases, int32_t nmol, const mi_opts *opts, mi_plan **out);
+int64_t mi_plan_out_len(const mi_plan *plan);
+int mi_plan_mol_offsets(const mi_plan *plan, int64_t *offsets);
+
+/* Direct Coulomb/exchange for one molecule, the ERI tensor never stored:
+ *   J_ij = sum_kl (ij|kl) D_kl,  K_ij = sum_kl (ik|jl) D_kl  (D symmetric),
+ * the index algebra of make2JmK (src/ERI.jl:114-128; G = 2J - K there).
+ * D, J, K are nbf x nbf row-major.  opts->schwarz_cut > 0 skips quartets with
+ * Q_ab Q_cd below it (Q = sqrt(max |(ab|ab)|)); MI_PREC_MIXED runs the VRR of
+ * quartets with Q_ab Q_cd < tau in FP32.  Accumulation uses FP64 atomics, so
+ * J and K agree across runs to rounding, not bit for bit.
+ * mi_jk: host arrays.  mi_plan_jk_execute: device arrays, async on stream. */
+int mi_jk(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, const double *D, double *J, double *K);
+int mi_plan_jk_execute(mi_plan *plan, const double *D_device, double *J_device, double *K_device, void *stream);
+
+/* Capture one execute (writing out_device) into a HIP graph owned by the
+ * plan; mi_plan_graph_launch replays it on any stream.  The graph holds no
+ * memset nodes. */
+int mi_plan_graph_capture(mi_plan *plan, double *out_device, int64_t out_len);
+int mi_plan_graph_launch(mi_plan *plan, void *stream);
+
 /* Plan statistics: [0]=nbf [1]=shell pairs [2]=primitive pairs
  * [3]=shell quartets [4]=primitive quartets [5]=unique ERIs
  * [6]=kernel launches per execute. */

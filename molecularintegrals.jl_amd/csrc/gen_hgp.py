@@ -46,6 +46,65 @@ def vname(e, f, m):
     return f"v{e}_{f}_{m}"
 
 
+def vrr_block(g, ctype, indent):
+    """Straight-line VRR for one primitive quartet in arithmetic type ctype;
+    returns (coefficient definitions, statements, target accumulations)."""
+    uses = set()
+    body = []
+    cast = "" if ctype == "double" else "(float)"
+    for (e, f) in g.nodes:
+        rec = g.recurrence(e, f)
+        for m in range(g.maxm[(e, f)] + 1):
+            if rec[0] == "F":
+                body.append(f"{indent}const {ctype} {vname(e, f, m)} = {cast}(pref * F[{m}]);")
+            elif rec[0] == "A":
+                _, i, e1, e2, n1 = rec
+                x = XYZ[i]
+                uses.update({f"PA{x}", f"WP{x}"})
+                expr = f"PA{x} * {vname(e1, 0, m)} + WP{x} * {vname(e1, 0, m + 1)}"
+                if e2 >= 0:
+                    uses.update({"oo2z", "eta_s"})
+                    coef = "oo2z" if n1 == 1 else f"({cast}{float(n1)} * oo2z)"
+                    expr += f" + {coef} * ({vname(e2, 0, m)} - eta_s * {vname(e2, 0, m + 1)})"
+                body.append(f"{indent}const {ctype} {vname(e, f, m)} = {expr};")
+            else:
+                _, i, f1, f2, e1, ne, nf = rec
+                x = XYZ[i]
+                uses.update({f"QC{x}", f"WQ{x}"})
+                expr = f"QC{x} * {vname(e, f1, m)} + WQ{x} * {vname(e, f1, m + 1)}"
+                if e1 >= 0:
+                    uses.add("oo2s")
+                    coef = "oo2s" if ne == 1 else f"({cast}{float(ne)} * oo2s)"
+                    expr += f" + {coef} * {vname(e1, f1, m + 1)}"
+                if f2 >= 0:
+                    uses.update({"oo2e", "zeta_s"})
+                    coef = "oo2e" if nf == 1 else f"({cast}{float(nf)} * oo2e)"
+                    expr += f" + {coef} * ({vname(e, f2, m)} - zeta_s * {vname(e, f2, m + 1)})"
+                body.append(f"{indent}const {ctype} {vname(e, f, m)} = {expr};")
+    defs = []
+    for x in XYZ:
+        if f"PA{x}" in uses:
+            defs.append(f"{indent}const {ctype} PA{x} = {cast}bp.PA{x};")
+        if f"WP{x}" in uses:
+            defs.append(f"{indent}const {ctype} WP{x} = {cast}(-eta * inv_s * PQ{x});")
+        if f"QC{x}" in uses:
+            defs.append(f"{indent}const {ctype} QC{x} = {cast}kp.PA{x};")
+        if f"WQ{x}" in uses:
+            defs.append(f"{indent}const {ctype} WQ{x} = {cast}(zeta * inv_s * PQ{x});")
+    if "oo2z" in uses:
+        defs.append(f"{indent}const {ctype} oo2z = {cast}bp.oo2z;")
+    if "oo2e" in uses:
+        defs.append(f"{indent}const {ctype} oo2e = {cast}kp.oo2z;")
+    if "oo2s" in uses:
+        defs.append(f"{indent}const {ctype} oo2s = {cast}(0.5 * inv_s);")
+    if "eta_s" in uses:
+        defs.append(f"{indent}const {ctype} eta_s = {cast}(eta * inv_s);")
+    if "zeta_s" in uses:
+        defs.append(f"{indent}const {ctype} zeta_s = {cast}(zeta * inv_s);")
+    accs = [f"{indent}acc{e}_{f} += (double){vname(e, f, 0)};" for (e, f) in g.targets]
+    return defs, body, accs
+
+
 def emit_rkernel(c):
     la, lb, lc, ld = c
     g = VrrGraph(*c)
@@ -63,12 +122,14 @@ def emit_rkernel(c):
     w(f"  if (!decode_quartet(args, qc)) return;")
     w(f"  const PairTopo P = args.topo[qc.bra];")
     w(f"  const PairTopo Q = args.topo[qc.ket];")
+    w(f"  const int qmode = quartet_mode(args, qc);  // 0 FP64 VRR, 1 FP32 VRR, 2 screened")
+    w(f"  if (qmode == 2 && args.mode != MI_MODE_UNIQUE) return;")
     for (e, f) in g.targets:
         w(f"  double acc{e}_{f} = 0.0;")
-    w(f"  const int nu = P.npp, nv = Q.npp, ntot = nu * nv;")
+    w(f"  const int nu = P.npp, nv = qmode == 2 ? 0 : Q.npp, ntot = nu * nv;")
     w(f"  const int G = qc.G;")
-    w(f"  int u = qc.g / nv, v = qc.g - (qc.g / nv) * nv;")
-    w(f"  const int du = G / nv, dv = G - du * nv;")
+    w(f"  int u = qc.g / Q.npp, v = qc.g - (qc.g / Q.npp) * Q.npp;")
+    w(f"  const int du = G / Q.npp, dv = G - du * Q.npp;")
     w(f"  for (int t = qc.g; t < ntot; t += G) {{")
     w(f"    const PrimPair bp = args.pp[P.ppoff + u];")
     w(f"    const PrimPair kp = args.pp[Q.ppoff + v];")
@@ -80,67 +141,32 @@ def emit_rkernel(c):
     w(f"    const double pref = bp.K * kp.K * sqrt(inv_s);")
     w(f"    double F[{mmax}];")
     w(f"    boys_eval<{mmax}>(T, sboys, F);")
-    uses = set()
-    body = []
-    for (e, f) in g.nodes:
-        rec = g.recurrence(e, f)
-        for m in range(g.maxm[(e, f)] + 1):
-            if rec[0] == "F":
-                body.append(f"    const double {vname(e, f, m)} = pref * F[{m}];")
-            elif rec[0] == "A":
-                _, i, e1, e2, n1 = rec
-                x = XYZ[i]
-                uses.update({f"PA{x}", f"WP{x}"})
-                expr = f"PA{x} * {vname(e1, 0, m)} + WP{x} * {vname(e1, 0, m + 1)}"
-                if e2 >= 0:
-                    uses.update({"oo2z", "eta_s"})
-                    coef = "oo2z" if n1 == 1 else f"({float(n1)} * oo2z)"
-                    expr += f" + {coef} * ({vname(e2, 0, m)} - eta_s * {vname(e2, 0, m + 1)})"
-                body.append(f"    const double {vname(e, f, m)} = {expr};")
-            else:
-                _, i, f1, f2, e1, ne, nf = rec
-                x = XYZ[i]
-                uses.update({f"QC{x}", f"WQ{x}"})
-                expr = f"QC{x} * {vname(e, f1, m)} + WQ{x} * {vname(e, f1, m + 1)}"
-                if e1 >= 0:
-                    uses.add("oo2s")
-                    coef = "oo2s" if ne == 1 else f"({float(ne)} * oo2s)"
-                    expr += f" + {coef} * {vname(e1, f1, m + 1)}"
-                if f2 >= 0:
-                    uses.update({"oo2e", "zeta_s"})
-                    coef = "oo2e" if nf == 1 else f"({float(nf)} * oo2e)"
-                    expr += f" + {coef} * ({vname(e, f2, m)} - zeta_s * {vname(e, f2, m + 1)})"
-                body.append(f"    const double {vname(e, f, m)} = {expr};")
-    for x in XYZ:
-        X = x
-        if f"PA{x}" in uses:
-            w(f"    const double PA{x} = bp.PA{X};")
-        if f"WP{x}" in uses:
-            w(f"    const double WP{x} = -eta * inv_s * PQ{X};")
-        if f"QC{x}" in uses:
-            w(f"    const double QC{x} = kp.PA{X};")
-        if f"WQ{x}" in uses:
-            w(f"    const double WQ{x} = zeta * inv_s * PQ{X};")
-    if "oo2z" in uses:
-        w("    const double oo2z = bp.oo2z;")
-    if "oo2e" in uses:
-        w("    const double oo2e = kp.oo2z;")
-    if "oo2s" in uses:
-        w("    const double oo2s = 0.5 * inv_s;")
-    if "eta_s" in uses:
-        w("    const double eta_s = eta * inv_s;")
-    if "zeta_s" in uses:
-        w("    const double zeta_s = zeta * inv_s;")
-    L.extend(body)
-    for (e, f) in g.targets:
-        w(f"    acc{e}_{f} += {vname(e, f, 0)};")
+    if mmax > 1:
+        w(f"    if (qmode == 1) {{  // FP32 VRR (Boys and prefactor stay FP64)")
+        d, bdy, acc = vrr_block(g, "float", "      ")
+        L.extend(d + bdy + acc)
+        w(f"    }} else {{")
+        d, bdy, acc = vrr_block(g, "double", "      ")
+        L.extend(d + bdy + acc)
+        w(f"    }}")
+    else:
+        d, bdy, acc = vrr_block(g, "double", "    ")
+        L.extend(d + bdy + acc)
     w("  }")
     # cross-lane reduction within the lane group (fixed butterfly: deterministic)
     w("  for (int s = G >> 1; s > 0; s >>= 1) {")
     for (e, f) in g.targets:
         w(f"    acc{e}_{f} += __shfl_xor(acc{e}_{f}, s, 64);")
     w("  }")
-    # HRR
+    diag = [(e, f) for (e, f) in g.targets if e == f]
+    if diag:
+        w("  if (args.mode == MI_MODE_SCHWARZ && qc.g == 0) {  // bound of the VRR outputs (e0|e0)")
+        w("    double vmax = 0.0;")
+        for (e, f) in diag:
+            w(f"    vmax = fmax(vmax, fabs(acc{e}_{f}));")
+        w("    record_vmax(&args.geom[qc.bra], vmax);")
+        w("  }")
+    # HRR (always FP64)
     if lb or ld:
         w("  const PairGeom PG = args.geom[qc.bra];")
         w("  const PairGeom QG = args.geom[qc.ket];")
@@ -154,7 +180,6 @@ def emit_rkernel(c):
             w(f"  const double {nm} = h{s1[0]}_{s1[1]}_{s1[2]}_{s1[3]} + PG.AB{XYZ[j]} * h{s2[0]}_{s2[1]}_{s2[2]}_{s2[3]};")
         else:
             w(f"  const double {nm} = h{s1[0]}_{s1[1]}_{s1[2]}_{s1[3]} + QG.AB{XYZ[j]} * h{s2[0]}_{s2[1]}_{s2[2]}_{s2[3]};")
-    # scales
     w(f"  double sA[{ncart(la)}], sB[{ncart(lb)}], sC[{ncart(lc)}], sD[{ncart(ld)}];")
     w(f"  load_scales<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, P, Q, sA, sB, sC, sD);")
     w("  const ScatterCtx sc = make_scatter(args, qc, P, Q);")
@@ -163,7 +188,7 @@ def emit_rkernel(c):
         for mb, b in enumerate(shell_aos(lb)):
             for mc, cc in enumerate(shell_aos(lc)):
                 for md, d in enumerate(shell_aos(ld)):
-                    w(f"  scatter_one(sc, {idx}, {ma}, {mb}, {mc}, {md}, "
+                    w(f"  scatter_one(args, sc, {idx}, {ma}, {mb}, {mc}, {md}, "
                       f"h{a}_{b}_{cc}_{d} * (sA[{ma}] * sB[{mb}]) * (sC[{mc}] * sD[{md}]));")
                     idx += 1
     w("}")
@@ -237,7 +262,7 @@ def w_tables(c):
                 hbase[n] = len(acc_nodes) + len(order)
                 order.append(n)
     nhrr = len(acc_nodes) + len(order)
-    accsrc = [base[(n[0], n[2])] for n in acc_nodes]
+    accsrc = [base[(n[0], n[2])] | ((1 << 31) if n[0] == n[2] else 0) for n in acc_nodes]
     hitems, hlev = [], []
     for lev in hlevels:
         hlev.append(len(hitems))

@@ -31,7 +31,10 @@ struct PairTopo {
 };
 
 struct PairGeom {
-  double ABx, ABy, ABz, Q;  // Q: Schwarz bound sqrt(max |(ab|ab)|), 0 until computed
+  double ABx, ABy, ABz;
+  double Q;   // max |(ab|ab)| over the pair's functions (Schwarz pass), 0 until computed
+  double V;   // max |(e0|e0)| over the pair's contracted VRR outputs: bounds the HRR inputs
+  double pad;
 };
 
 struct __attribute__((aligned(16))) PrimPair {
@@ -47,28 +50,43 @@ struct TaskDesc {
   int32_t blockStart;  // first block of this task in its launch
   int32_t cls;         // quartet class id
   int32_t variant;     // 1 register kernel, 0 LDS wave kernel
-  int32_t pad;
+  int32_t bf0;         // first function of the task's molecule (pairs use global indices)
   int64_t nq;          // shell quartets in the task
   int64_t nprim;       // primitive quartets in the task
+  int64_t obase;       // output offset of the task's molecule
+  int64_t M;           // nbf(nbf+1)/2 of the task's molecule
 };
+
+// What a launch does with each computed block element.
+#define MI_MODE_UNIQUE 0   // write the unique (ij|kl) array
+#define MI_MODE_SCHWARZ 1  // diagonal (ab|ab): atomic max of |value| into PairGeom.Q
+#define MI_MODE_JK 2       // accumulate Coulomb/exchange against a density
 
 struct EriArgs {
   const PairTopo* topo;
-  const PairGeom* geom;
+  PairGeom* geom;
   const PrimPair* pp;
   const double* bf_scale;
   const double* boys;
   double* out;
   const TaskDesc* tasks;
-  int64_t M;           // nbf(nbf+1)/2
   int32_t order;       // 0 iindex, 1 chrr
   int32_t ntasks;
+  int32_t mode;        // MI_MODE_*
+  int32_t nbf;         // J/K: matrix dimension
+  double tau2;         // FP32 VRR when Qmax_ab * Qmax_cd < tau2 (0: never)
+  double cut2;         // skip when Qmax_ab * Qmax_cd < cut2 (0: never)
+  const double* D;     // J/K: density, nbf x nbf row-major
+  double* Jacc;        // J/K: unsymmetrised accumulators (J = Jacc + Jacc^T)
+  double* Kacc;
 };
 
 struct QuartetCtx {
   int bra, ket;
   int G, g;
   bool same_pair;
+  int bf0;
+  int64_t obase, M;
 };
 
 struct ScatterCtx {
@@ -78,6 +96,7 @@ struct ScatterCtx {
   int fa0, fb0, fc0, fd0;
   bool braSame, ketSame, quartetSame;
   int gmask, g;
+  int bra;
 };
 
 // Wave-cooperative class descriptor (tables generated by gen_hgp.py).
@@ -143,22 +162,25 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
   }
 }
 
-// Copy n 16-byte chunks global -> LDS with all threads of the block, eight
-// independent loads in flight per thread before the stores.
+// Copy n 16-byte chunks global -> LDS with all threads of the block: the main
+// loop issues eight unconditional loads per thread before its stores (plain
+// registers, all eight in flight), the tail moves the remainder one by one.
 __device__ __forceinline__ void stage_chunks(uint4* __restrict__ dst, const uint4* __restrict__ src, int n) {
-  for (int base = threadIdx.x; base < n; base += 8 * blockDim.x) {
-    uint4 r[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int i = base + k * blockDim.x;
-      if (i < n) r[k] = src[i];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int i = base + k * blockDim.x;
-      if (i < n) dst[i] = r[k];
-    }
+  const int st = blockDim.x;
+  int i = threadIdx.x;
+  for (; i + 7 * st < n; i += 8 * st) {
+    const uint4 r0 = src[i], r1 = src[i + st], r2 = src[i + 2 * st], r3 = src[i + 3 * st];
+    const uint4 r4 = src[i + 4 * st], r5 = src[i + 5 * st], r6 = src[i + 6 * st], r7 = src[i + 7 * st];
+    dst[i] = r0;
+    dst[i + st] = r1;
+    dst[i + 2 * st] = r2;
+    dst[i + 3 * st] = r3;
+    dst[i + 4 * st] = r4;
+    dst[i + 5 * st] = r5;
+    dst[i + 6 * st] = r6;
+    dst[i + 7 * st] = r7;
   }
+  for (; i < n; i += st) dst[i] = src[i];
 }
 
 // Copy the Boys slice of one top order into LDS (all threads of the block).
@@ -184,7 +206,9 @@ __device__ __forceinline__ int find_task(const EriArgs& a) {
 }
 
 __device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, QuartetCtx& qc) {
-  if (T.same) {
+  if (T.same == 2) {  // Schwarz pass: the diagonal quartets (p|p)
+    qc.bra = qc.ket = T.braStart + (int)qi;
+  } else if (T.same) {
     int64_t q = (int64_t)((sqrt(8.0 * (double)qi + 1.0) - 1.0) * 0.5);
     while (q * (q + 1) / 2 > qi) --q;
     while ((q + 1) * (q + 2) / 2 <= qi) ++q;
@@ -198,6 +222,9 @@ __device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, Quar
     qc.ket = T.ketStart + (int)q;
   }
   qc.same_pair = (qc.bra == qc.ket);
+  qc.bf0 = T.bf0;
+  qc.obase = T.obase;
+  qc.M = T.M;
 }
 
 // Lane-group decode for the register kernels: G lanes per quartet.
@@ -229,18 +256,19 @@ __device__ __forceinline__ void load_scales(const EriArgs& a, const PairTopo& P,
 __device__ __forceinline__ ScatterCtx make_scatter(const EriArgs& a, const QuartetCtx& qc, const PairTopo& P,
                                                    const PairTopo& Q) {
   ScatterCtx s;
-  s.out = a.out;
-  s.M = a.M;
+  s.out = a.out + qc.obase;
+  s.M = qc.M;
   s.order = a.order;
-  s.fa0 = P.bfA;
-  s.fb0 = P.bfB;
-  s.fc0 = Q.bfA;
-  s.fd0 = Q.bfB;
+  s.fa0 = P.bfA - qc.bf0;
+  s.fb0 = P.bfB - qc.bf0;
+  s.fc0 = Q.bfA - qc.bf0;
+  s.fd0 = Q.bfB - qc.bf0;
   s.braSame = (P.shA == P.shB);
   s.ketSame = (Q.shA == Q.shB);
   s.quartetSame = qc.same_pair;
   s.gmask = qc.G - 1;
   s.g = qc.g;
+  s.bra = qc.bra;
   return s;
 }
 
@@ -259,9 +287,68 @@ __device__ __forceinline__ int64_t unique_slot(const ScatterCtx& s, int fa, int 
   return (lo - 1) * (s.M + 1) - lo * (lo - 1) / 2 + hi - lo;
 }
 
-__device__ __forceinline__ void scatter_one(const ScatterCtx& s, int idx, int ma, int mb, int mc, int md,
-                                            double val) {
+// 0: FP64 VRR, 1: FP32 VRR, 2: screened.  Screening uses the Schwarz bound of
+// the integrals, Q_ab Q_cd < cut; FP32 VRR uses the Schwarz bound of the VRR
+// outputs (e0|f0) themselves, V_ab V_cd < tau (each (e0|f0) is an integral
+// (e_A s_B|f_C s_D), so |(e0|f0)| <= sqrt(V_ab V_cd)): HRR cancellation can
+// make (e0|f0) much larger than the final integral, so bounding only the
+// final value would not bound the FP32 rounding.
+__device__ __forceinline__ int quartet_mode(const EriArgs& a, const QuartetCtx& qc) {
+  if (a.tau2 <= 0.0 && a.cut2 <= 0.0) return 0;
+  const PairGeom& gb = a.geom[qc.bra];
+  const PairGeom& gk = a.geom[qc.ket];
+  if (gb.Q * gk.Q < a.cut2) return 2;
+  return gb.V * gk.V < a.tau2 ? 1 : 0;
+}
+
+// Schwarz pass: fold |x| into a pair's V (non-negative doubles order like their bits).
+__device__ __forceinline__ void record_vmax(PairGeom* g, double x) {
+  atomicMax(reinterpret_cast<unsigned long long*>(&g->V), (unsigned long long)__double_as_longlong(fabs(x)));
+}
+
+// One element (fa fb|fc fd) of a shell-quartet block (molecule-local indices).
+__device__ __forceinline__ void scatter_elem(const EriArgs& a, const ScatterCtx& s, int bra, int fa, int fb, int fc,
+                                             int fd, double val) {
+  if (a.mode == MI_MODE_UNIQUE) {
+    const int64_t slot = unique_slot(s, fa, fb, fc, fd);
+    if (slot >= 0) s.out[slot] = val;
+    return;
+  }
+  if (a.mode == MI_MODE_SCHWARZ) {
+    if (fa == fc && fb == fd)  // non-negative doubles order like their bit patterns
+      atomicMax(reinterpret_cast<unsigned long long*>(&a.geom[bra].Q),
+                (unsigned long long)__double_as_longlong(fabs(val)));
+    return;
+  }
+  // Coulomb/exchange (index algebra of make2JmK, src/ERI.jl:114-128): each
+  // unique (ij|kl) stands for its 8 permutations; with the degeneracy weight
+  // w = v / (2^#coincidences) they contribute Jacc[i][j] += 2w D_kl,
+  // Jacc[k][l] += 2w D_ij, Kacc[i][k] += w D_jl, Kacc[j][k] += w D_il,
+  // Kacc[i][l] += w D_jk, Kacc[j][l] += w D_ik, and J = Jacc + Jacc^T,
+  // K = Kacc + Kacc^T.
+  if (unique_slot(s, fa, fb, fc, fd) < 0) return;
+  int i = fa > fb ? fa : fb, j = fa > fb ? fb : fa, k = fc > fd ? fc : fd, l = fc > fd ? fd : fc;
+  const int64_t ij = (int64_t)i * (i + 1) / 2 + j, kl = (int64_t)k * (k + 1) / 2 + l;
+  if (ij > kl) {
+    int t = i; i = k; k = t;
+    t = j; j = l; l = t;
+  }
+  double w = val;
+  if (i == j) w *= 0.5;
+  if (k == l) w *= 0.5;
+  if (i == k && j == l) w *= 0.5;
+  const int n = a.nbf;
+  const double* D = a.D;
+  atomicAdd(&a.Jacc[(int64_t)i * n + j], 2.0 * w * D[(int64_t)k * n + l]);
+  atomicAdd(&a.Jacc[(int64_t)k * n + l], 2.0 * w * D[(int64_t)i * n + j]);
+  atomicAdd(&a.Kacc[(int64_t)i * n + k], w * D[(int64_t)j * n + l]);
+  atomicAdd(&a.Kacc[(int64_t)j * n + k], w * D[(int64_t)i * n + l]);
+  atomicAdd(&a.Kacc[(int64_t)i * n + l], w * D[(int64_t)j * n + k]);
+  atomicAdd(&a.Kacc[(int64_t)j * n + l], w * D[(int64_t)i * n + k]);
+}
+
+__device__ __forceinline__ void scatter_one(const EriArgs& a, const ScatterCtx& s, int idx, int ma, int mb, int mc,
+                                            int md, double val) {
   if ((idx & s.gmask) != s.g) return;
-  const int64_t slot = unique_slot(s, s.fa0 + ma, s.fb0 + mb, s.fc0 + mc, s.fd0 + md);
-  if (slot >= 0) s.out[slot] = val;
+  scatter_elem(a, s, s.bra, s.fa0 + ma, s.fb0 + mb, s.fc0 + mc, s.fd0 + md, val);
 }

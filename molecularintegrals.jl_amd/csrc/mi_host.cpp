@@ -172,6 +172,8 @@ struct mi_plan {
   int nshell = 0, nbf = 0, npair = 0, npp = 0;
   int64_t nq_shell = 0, nq_prim = 0, n_unique = 0;
   int order = MI_ORDER_CHRR;
+  int nmol = 1;
+  std::vector<int64_t> mol_off;  // output offset of each molecule, [nmol+1]
   // device buffers
   double* d_shell_xyz = nullptr;
   int32_t* d_shell_prim_off = nullptr;
@@ -183,8 +185,12 @@ struct mi_plan {
   PrimPair* d_pp = nullptr;
   int32_t* d_pp_pair = nullptr;
   TaskDesc* d_tasks = nullptr;
-  std::vector<Launch> launches;  // in issue order (cost descending)
+  std::vector<Launch> launches;   // in issue order (cost descending)
+  std::vector<Launch> slaunches;  // Schwarz pass (diagonal quartets), run before the main launches
+  bool need_schwarz = false;
+  double tau2 = 0.0, cut2 = 0.0;
   int nlanes = 1;
+  hipGraphExec_t gexec = nullptr;  // captured execute (mi_plan_graph_capture)
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[MI_MAX_STREAMS] = {};
 };
@@ -224,6 +230,7 @@ int upload(mi_ctx* ctx, T** dst, const T* src, size_t n) {
 }
 
 void free_plan_buffers(mi_plan* p) {
+  if (p->gexec) hipGraphExecDestroy(p->gexec);
   if (p->ev_fork) hipEventDestroy(p->ev_fork);
   for (auto& e : p->ev_join)
     if (e) hipEventDestroy(e);
@@ -304,140 +311,44 @@ void mi_destroy(mi_ctx* ctx) {
   delete ctx;
 }
 
-int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan** out) {
-  if (!ctx || !out) return MI_EINVAL;
-  *out = nullptr;
-  int rc = validate(ctx, b);
-  if (rc) return rc;
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
-  mi_plan* P = new mi_plan();
-  P->ctx = ctx;
-  P->order = opts ? opts->order : MI_ORDER_CHRR;
-  if (P->order != MI_ORDER_CHRR && P->order != MI_ORDER_IINDEX) {
-    delete P;
-    return set_err(ctx, MI_EINVAL, "unknown order %d", P->order);
-  }
-  const int ns = b->nshell;
-  P->nshell = ns;
-  P->nbf = b->nbf;
-  P->n_unique = mi_n_unique(b->nbf);
-  std::vector<int> bfoff(ns + 1, 0), np(ns);
-  for (int s = 0; s < ns; ++s) {
-    bfoff[s + 1] = bfoff[s] + ncart(b->shell_l[s]);
-    np[s] = b->shell_prim_off[s + 1] - b->shell_prim_off[s];
-  }
-  // Shell pairs (I >= J), canonical shell order L_A >= L_B.
-  struct PairKey {
-    int pc, bucket, pidx, A, B, K;
-  };
-  std::vector<PairKey> keys;
-  keys.reserve((size_t)ns * (ns + 1) / 2);
-  for (int I = 0; I < ns; ++I)
-    for (int J = 0; J <= I; ++J) {
-      int A = I, B = J;
-      if (b->shell_l[J] > b->shell_l[I]) std::swap(A, B);
-      const int K = np[A] * np[B];
-      keys.push_back({pair_class(b->shell_l[A], b->shell_l[B]), kbucket(K), I * (I + 1) / 2 + J, A, B, K});
+}  // extern "C"
+
+// Plan over one or more molecules: shells are concatenated, shell pairs and
+// quartets stay inside a molecule, and every task carries its molecule's
+// output offset, pair count and first function, so the slot algebra of the
+// kernels is per molecule.
+// Kernel choice for a task from the tuning table, else the heuristic:
+// register kernel when one exists with lanes ~ primitive quartets / 4, else the
+// wave kernel with as many wavefronts (<= 4) as the LDS budget allows.
+static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg) {
+  const TuneEntry& te = ctx->tune[tune_index(t.cls, bb, kb)];
+  int variant, G;
+  if (te.variant >= 0) {
+    variant = te.variant;
+    G = te.G;
+  } else {
+    variant = mi::has_register_kernel(t.cls) ? 1 : 0;
+    if (variant) {
+      G = pow2floor(std::max(1.0, kavg / 4.0));
+      if (mi::wclass(t.cls).nacc > 16) G = std::min(G, 4);
+    } else {
+      G = 4;
+      while (G > 1 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) G >>= 1;
     }
-  std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
-    if (x.pc != y.pc) return x.pc < y.pc;
-    if (x.bucket != y.bucket) return x.bucket < y.bucket;
-    return x.pidx < y.pidx;
-  });
-  const int npair = (int)keys.size();
-  std::vector<PairTopo> topo(npair);
-  std::vector<int32_t> pp_pair;
-  int ppoff = 0;
-  for (int r = 0; r < npair; ++r) {
-    const PairKey& k = keys[r];
-    topo[r] = {k.A, k.B, bfoff[k.A], bfoff[k.B], ppoff, k.K, np[k.B], k.pidx};
-    for (int t = 0; t < k.K; ++t) pp_pair.push_back(r);
-    ppoff += k.K;
   }
-  P->npair = npair;
-  P->npp = ppoff;
-  // Contiguous lists of equal (pc, bucket).
-  struct List {
-    int start, n, pc, Ksum, Kmax;
-  };
-  std::vector<List> lists;
-  for (int r = 0; r < npair;) {
-    int e = r;
-    int Ksum = 0, Kmax = 0;
-    while (e < npair && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket) {
-      Ksum += keys[e].K;
-      Kmax = std::max(Kmax, keys[e].K);
-      ++e;
-    }
-    lists.push_back({r, e - r, keys[r].pc, Ksum, Kmax});
-    r = e;
-  }
-  // Tasks grouped by quartet class.
-  std::vector<std::vector<TaskDesc>> per_class(mi::nclass());
-  std::vector<int64_t> cls_shell(mi::nclass(), 0), cls_prim(mi::nclass(), 0);
-  int64_t nq_shell = 0, nq_prim = 0;
-  for (size_t x = 0; x < lists.size(); ++x)
-    for (size_t y = x; y < lists.size(); ++y) {
-      const List* bra = &lists[x];
-      const List* ket = &lists[y];
-      if (ket->pc > bra->pc) std::swap(bra, ket);
-      TaskDesc t{};
-      t.braStart = bra->start;
-      t.nBra = bra->n;
-      t.ketStart = ket->start;
-      t.nKet = ket->n;
-      t.same = (x == y);
-      t.cls = quartet_class(bra->pc, ket->pc);
-      t.nq = t.same ? (int64_t)bra->n * (bra->n + 1) / 2 : (int64_t)bra->n * ket->n;
-      const double kavg = ((double)bra->Ksum / bra->n) * ((double)ket->Ksum / ket->n);
-      const int bb = keys[bra->start].bucket, kb = keys[ket->start].bucket;
-      const TuneEntry& te = ctx->tune[tune_index(t.cls, bb, kb)];
-      int variant, G;
-      if (te.variant >= 0) {
-        variant = te.variant;
-        G = te.G;
-      } else {
-        // heuristic: register kernel when one exists, lanes ~ primitive quartets / 4;
-        // wave kernel with as many wavefronts (<= 4) as the LDS budget allows
-        variant = mi::has_register_kernel(t.cls) ? 1 : 0;
-        if (variant) {
-          G = pow2floor(std::max(1.0, kavg / 4.0));
-          if (mi::wclass(t.cls).nacc > 16) G = std::min(G, 4);
-        } else {
-          G = 4;
-          while (G > 1 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) G >>= 1;
-        }
-      }
-      if (variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) {
-        delete P;
-        return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
-      }
-      t.G = G;
-      t.variant = variant;
-      int64_t tprim;
-      if (t.same) {
-        // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
-        int64_t sk = 0, sk2 = 0;
-        for (int r = bra->start; r < bra->start + bra->n; ++r) {
-          sk += keys[r].K;
-          sk2 += (int64_t)keys[r].K * keys[r].K;
-        }
-        tprim = (sk * sk + sk2) / 2;
-      } else {
-        tprim = (int64_t)bra->Ksum * ket->Ksum;
-      }
-      t.nprim = tprim;
-      nq_shell += t.nq;
-      nq_prim += tprim;
-      cls_shell[t.cls] += t.nq;
-      cls_prim[t.cls] += tprim;
-      per_class[t.cls].push_back(t);
-    }
-  P->nq_shell = nq_shell;
-  P->nq_prim = nq_prim;
+  if (variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit))
+    return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
+  t.G = G;
+  t.variant = variant;
+  return MI_OK;
+}
+
+// One launch per (class, kernel variant, wavefronts); tasks appended to
+// all_tasks, launches get task ranges, grid sizes and a cost estimate.
+static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskDesc>>& per_class,
+                          std::vector<TaskDesc>& all_tasks, std::vector<Launch>& launches) {
   // One launch per (class, variant); within a launch, tasks with the most
   // work per block come first so the longest blocks start earliest.
-  std::vector<TaskDesc> all_tasks;
   for (int c = 0; c < mi::nclass(); ++c) {
     for (int vk = 0; vk < 4; ++vk) {  // 0: register kernel; 1,2,3: wave kernel with 1,2,4 wavefronts
       const int var = vk == 0 ? 1 : 0, nw = vk == 0 ? 0 : (1 << (vk - 1));
@@ -462,7 +373,6 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
       }
       if (!L.reg) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
       if (blocks > INT32_MAX || qs > INT32_MAX) {
-        delete P;
         return set_err(ctx, MI_ENOMEM, "class %d: %lld quartets exceed one launch", c, (long long)qs);
       }
       L.nblocks = (int)blocks;
@@ -470,7 +380,213 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
       L.nq_prim = 0;
       for (const TaskDesc& t : ts) L.nq_prim += t.nprim;
       L.cost = 0.0;
-      P->launches.push_back(L);
+      launches.push_back(L);
+    }
+  }
+  for (Launch& L : launches) {
+    const WClassDesc& D = mi::wclass(L.cls);
+    L.cost = (double)L.nq_prim * (D.nvrr + D.nacc) + (double)L.nq_shell * (D.nhrr + D.nout) +
+             2.0e4 * (D.nvlev + D.nhlev + 1);
+  }
+  (void)P;
+  return MI_OK;
+}
+
+static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, mi_plan** out) {
+  if (!ctx || !out || !bases || nmol <= 0) return set_err(ctx, MI_EINVAL, "bad plan arguments");
+  *out = nullptr;
+  int rc;
+  for (int m = 0; m < nmol; ++m)
+    if ((rc = validate(ctx, &bases[m]))) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  mi_plan* P = new mi_plan();
+  P->ctx = ctx;
+  P->order = opts ? opts->order : MI_ORDER_CHRR;
+  if (P->order != MI_ORDER_CHRR && P->order != MI_ORDER_IINDEX) {
+    delete P;
+    return set_err(ctx, MI_EINVAL, "unknown order %d", P->order);
+  }
+  // Concatenated basis.
+  std::vector<double> xyz, pexp, pcn, bfs;
+  std::vector<int32_t> L, poff(1, 0), shmol;
+  std::vector<int> mol_sh0(nmol + 1, 0), mol_bf0(nmol + 1, 0);
+  P->mol_off.assign(nmol + 1, 0);
+  for (int m = 0; m < nmol; ++m) {
+    const mi_basis& b = bases[m];
+    mol_sh0[m] = (int)L.size();
+    mol_bf0[m] = (int)bfs.size();
+    const int pbase = poff.back();
+    for (int s2 = 0; s2 < b.nshell; ++s2) {
+      for (int k = 0; k < 3; ++k) xyz.push_back(b.shell_xyz[3 * s2 + k]);
+      L.push_back(b.shell_l[s2]);
+      shmol.push_back(m);
+      poff.push_back(pbase + b.shell_prim_off[s2 + 1]);
+    }
+    for (int p2 = 0; p2 < b.shell_prim_off[b.nshell]; ++p2) {
+      pexp.push_back(b.prim_exp[p2]);
+      pcn.push_back(b.prim_coef[p2] * b.prim_norm[p2]);
+    }
+    for (int f = 0; f < b.nbf; ++f) bfs.push_back(b.bf_scale[f]);
+    P->mol_off[m + 1] = P->mol_off[m] + mi_n_unique(b.nbf);
+  }
+  mol_sh0[nmol] = (int)L.size();
+  mol_bf0[nmol] = (int)bfs.size();
+  const int ns = (int)L.size();
+  P->nshell = ns;
+  P->nbf = nmol == 1 ? bases[0].nbf : (int)bfs.size();
+  P->nmol = nmol;
+  P->n_unique = P->mol_off[nmol];
+  std::vector<int> bfoff(ns + 1, 0), np(ns);
+  for (int s2 = 0; s2 < ns; ++s2) {
+    bfoff[s2 + 1] = bfoff[s2] + ncart(L[s2]);
+    np[s2] = poff[s2 + 1] - poff[s2];
+  }
+  // Shell pairs (I >= J within a molecule), canonical shell order L_A >= L_B.
+  struct PairKey {
+    int mol, pc, bucket, pidx, A, B, K;
+  };
+  std::vector<PairKey> keys;
+  for (int m = 0; m < nmol; ++m)
+    for (int I = mol_sh0[m]; I < mol_sh0[m + 1]; ++I)
+      for (int J = mol_sh0[m]; J <= I; ++J) {
+        int A = I, B = J;
+        if (L[J] > L[I]) std::swap(A, B);
+        const int K = np[A] * np[B];
+        const int i = I - mol_sh0[m], j = J - mol_sh0[m];
+        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K});
+      }
+  std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
+    if (x.mol != y.mol) return x.mol < y.mol;
+    if (x.pc != y.pc) return x.pc < y.pc;
+    if (x.bucket != y.bucket) return x.bucket < y.bucket;
+    return x.pidx < y.pidx;
+  });
+  const int npair = (int)keys.size();
+  std::vector<PairTopo> topo(npair);
+  std::vector<int32_t> pp_pair;
+  int64_t ppoff = 0;
+  for (int r = 0; r < npair; ++r) {
+    const PairKey& k = keys[r];
+    topo[r] = {k.A, k.B, bfoff[k.A], bfoff[k.B], (int32_t)ppoff, k.K, np[k.B], k.pidx};
+    for (int t = 0; t < k.K; ++t) pp_pair.push_back(r);
+    ppoff += k.K;
+  }
+  if (ppoff > INT32_MAX) {
+    delete P;
+    return set_err(ctx, MI_ENOMEM, "too many primitive pairs (%lld)", (long long)ppoff);
+  }
+  P->npair = npair;
+  P->npp = (int)ppoff;
+  // Contiguous lists of equal (molecule, pc, bucket).
+  struct List {
+    int start, n, pc, Ksum, Kmax, mol;
+  };
+  std::vector<List> lists;
+  std::vector<int> mol_list0(nmol + 1, 0);
+  for (int r = 0; r < npair;) {
+    int e = r;
+    int Ksum = 0, Kmax = 0;
+    while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket) {
+      Ksum += keys[e].K;
+      Kmax = std::max(Kmax, keys[e].K);
+      ++e;
+    }
+    lists.push_back({r, e - r, keys[r].pc, Ksum, Kmax, keys[r].mol});
+    r = e;
+  }
+  for (size_t x = 0, m = 0; m <= (size_t)nmol; ++m) {
+    while (x < lists.size() && lists[x].mol < (int)m) ++x;
+    mol_list0[m] = (int)x;
+  }
+  // Tasks grouped by quartet class.
+  std::vector<std::vector<TaskDesc>> per_class(mi::nclass());
+  std::vector<int64_t> cls_shell(mi::nclass(), 0), cls_prim(mi::nclass(), 0);
+  int64_t nq_shell = 0, nq_prim = 0;
+  for (int m = 0; m < nmol; ++m)
+  for (int x = mol_list0[m]; x < mol_list0[m + 1]; ++x)
+    for (int y = x; y < mol_list0[m + 1]; ++y) {
+      const List* bra = &lists[x];
+      const List* ket = &lists[y];
+      if (ket->pc > bra->pc) std::swap(bra, ket);
+      TaskDesc t{};
+      t.braStart = bra->start;
+      t.nBra = bra->n;
+      t.ketStart = ket->start;
+      t.nKet = ket->n;
+      t.same = (x == y);
+      t.cls = quartet_class(bra->pc, ket->pc);
+      t.nq = t.same ? (int64_t)bra->n * (bra->n + 1) / 2 : (int64_t)bra->n * ket->n;
+      t.bf0 = mol_bf0[m];
+      t.obase = P->mol_off[m];
+      {
+        const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
+        t.M = nbm * (nbm + 1) / 2;
+      }
+      const double kavg = ((double)bra->Ksum / bra->n) * ((double)ket->Ksum / ket->n);
+      const int bb = keys[bra->start].bucket, kb = keys[ket->start].bucket;
+      if ((rc = choose_kernel(ctx, t, bb, kb, kavg))) {
+        delete P;
+        return rc;
+      }
+      int64_t tprim;
+      if (t.same) {
+        // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
+        int64_t sk = 0, sk2 = 0;
+        for (int r = bra->start; r < bra->start + bra->n; ++r) {
+          sk += keys[r].K;
+          sk2 += (int64_t)keys[r].K * keys[r].K;
+        }
+        tprim = (sk * sk + sk2) / 2;
+      } else {
+        tprim = (int64_t)bra->Ksum * ket->Ksum;
+      }
+      t.nprim = tprim;
+      nq_shell += t.nq;
+      nq_prim += tprim;
+      cls_shell[t.cls] += t.nq;
+      cls_prim[t.cls] += tprim;
+      per_class[t.cls].push_back(t);
+    }
+  P->nq_shell = nq_shell;
+  P->nq_prim = nq_prim;
+  std::vector<TaskDesc> all_tasks;
+  if ((rc = group_launches(ctx, P, per_class, all_tasks, P->launches))) {
+    delete P;
+    return rc;
+  }
+  // Schwarz pass: diagonal quartets (p|p) of every shell pair, needed for
+  // FP32-VRR selection (MI_PREC_MIXED) and for screening (schwarz_cut > 0).
+  P->tau2 = (opts && opts->precision == MI_PREC_MIXED) ? opts->tau * opts->tau : 0.0;
+  P->cut2 = (opts && opts->schwarz_cut > 0) ? opts->schwarz_cut * opts->schwarz_cut : 0.0;
+  P->need_schwarz = P->tau2 > 0.0 || P->cut2 > 0.0;
+  if (P->need_schwarz) {
+    std::vector<std::vector<TaskDesc>> diag(mi::nclass());
+    for (int m = 0; m < nmol; ++m)
+      for (int x = mol_list0[m]; x < mol_list0[m + 1]; ++x) {
+        const List& li = lists[x];
+        TaskDesc t{};
+        t.braStart = t.ketStart = li.start;
+        t.nBra = t.nKet = li.n;
+        t.same = 2;
+        t.cls = quartet_class(li.pc, li.pc);
+        t.nq = li.n;
+        t.bf0 = mol_bf0[m];
+        t.obase = P->mol_off[m];
+        const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
+        t.M = nbm * (nbm + 1) / 2;
+        const int bk = keys[li.start].bucket;
+        if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n)))) {
+          delete P;
+          return rc;
+        }
+        int64_t sk2 = 0;
+        for (int r = li.start; r < li.start + li.n; ++r) sk2 += (int64_t)keys[r].K * keys[r].K;
+        t.nprim = sk2;
+        diag[t.cls].push_back(t);
+      }
+    if ((rc = group_launches(ctx, P, diag, all_tasks, P->slaunches))) {
+      delete P;
+      return rc;
     }
   }
   // primitive-quartet counts per launch and a cost estimate (VRR+contraction
@@ -495,14 +611,12 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
     }
   }
   // Device buffers.
-  std::vector<double> cn(b->shell_prim_off[ns]);
-  for (int p = 0; p < b->shell_prim_off[ns]; ++p) cn[p] = b->prim_coef[p] * b->prim_norm[p];
-  const int nprim = b->shell_prim_off[ns];
-  if ((rc = upload(ctx, &P->d_shell_xyz, b->shell_xyz, (size_t)3 * ns)) ||
-      (rc = upload(ctx, &P->d_shell_prim_off, b->shell_prim_off, (size_t)ns + 1)) ||
-      (rc = upload(ctx, &P->d_prim_exp, b->prim_exp, (size_t)nprim)) ||
-      (rc = upload(ctx, &P->d_prim_cn, cn.data(), (size_t)nprim)) ||
-      (rc = upload(ctx, &P->d_bf_scale, b->bf_scale, (size_t)b->nbf)) ||
+  const int nprim = poff[ns];
+  if ((rc = upload(ctx, &P->d_shell_xyz, xyz.data(), (size_t)3 * ns)) ||
+      (rc = upload(ctx, &P->d_shell_prim_off, poff.data(), (size_t)ns + 1)) ||
+      (rc = upload(ctx, &P->d_prim_exp, pexp.data(), (size_t)nprim)) ||
+      (rc = upload(ctx, &P->d_prim_cn, pcn.data(), (size_t)nprim)) ||
+      (rc = upload(ctx, &P->d_bf_scale, bfs.data(), bfs.size())) ||
       (rc = upload(ctx, &P->d_topo, topo.data(), (size_t)npair)) ||
       (rc = upload<PairGeom>(ctx, &P->d_geom, nullptr, (size_t)npair)) ||
       (rc = upload<PrimPair>(ctx, &P->d_pp, nullptr, (size_t)P->npp)) ||
@@ -526,18 +640,59 @@ int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan*
   return MI_OK;
 }
 
-static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev) {
+extern "C" {
+
+int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan** out) {
+  return plan_build(ctx, b, 1, opts, out);
+}
+
+int mi_batch_plan_create(mi_ctx* ctx, const mi_basis* bases, int32_t nmol, const mi_opts* opts, mi_plan** out) {
+  return plan_build(ctx, bases, nmol, opts, out);
+}
+
+int64_t mi_plan_out_len(const mi_plan* P) { return P ? P->n_unique : -1; }
+
+int mi_plan_mol_offsets(const mi_plan* P, int64_t* offsets) {
+  if (!P || !offsets) return MI_EINVAL;
+  for (int m = 0; m <= P->nmol; ++m) offsets[m] = P->mol_off[m];
+  return MI_OK;
+}
+
+
+struct JKArgs {
+  const double* D;
+  double* J;
+  double* K;
+};
+
+static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const TaskDesc* d_tasks, hipStream_t ls) {
+  a.tasks = d_tasks + L.task_off;
+  a.ntasks = L.ntasks;
+  if (L.reg)
+    HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
+  else
+    HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
+  return MI_OK;
+}
+
+// One execute on `stream`: shell-pair kernel, Schwarz pass when the plan
+// needs bounds, then the class launches (forked over the plan's stream lanes
+// unless ev != NULL, which times each launch on one stream).  jk != NULL
+// accumulates Coulomb/exchange instead of writing the unique array.
+static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev,
+                    const JKArgs* jk = nullptr) {
   if (!P) return MI_EINVAL;
   mi_ctx* ctx = P->ctx;
-  if (!out) return set_err(ctx, MI_EINVAL, "output pointer is NULL");
-  if (out_len != P->n_unique)
-    return set_err(ctx, MI_ESIZE, "output length %lld, basis needs %lld", (long long)out_len,
-                   (long long)P->n_unique);
+  if (!jk) {
+    if (!out) return set_err(ctx, MI_EINVAL, "output pointer is NULL");
+    if (out_len != P->n_unique)
+      return set_err(ctx, MI_ESIZE, "output length %lld, basis needs %lld", (long long)out_len,
+                     (long long)P->n_unique);
+  }
   hipStream_t s = (hipStream_t)stream;
   if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], s));
   HIP_TRY(ctx, mi::launch_pair_prims(P->npp, P->d_pp_pair, P->d_topo, P->d_shell_xyz, P->d_shell_prim_off,
                                      P->d_prim_exp, P->d_prim_cn, P->d_pp, P->d_geom, s));
-  if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], s));
   EriArgs a{};
   a.topo = P->d_topo;
   a.geom = P->d_geom;
@@ -545,11 +700,30 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
   a.bf_scale = P->d_bf_scale;
   a.boys = ctx->d_boys;
   a.out = out;
-  a.M = (int64_t)P->nbf * (P->nbf + 1) / 2;
   a.order = P->order;
+  a.nbf = P->nbf;
+  if (P->need_schwarz) {
+    EriArgs sa = a;
+    sa.mode = MI_MODE_SCHWARZ;
+    for (const Launch& L : P->slaunches) {
+      int rc = launch_one(ctx, L, sa, P->d_tasks, s);
+      if (rc) return rc;
+    }
+  }
+  if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], s));
+  a.tau2 = P->tau2;
+  a.cut2 = P->cut2;
+  if (jk) {
+    a.mode = MI_MODE_JK;
+    a.D = jk->D;
+    a.Jacc = jk->J;
+    a.Kacc = jk->K;
+    const int64_t nn = (int64_t)P->nbf * P->nbf;
+    HIP_TRY(ctx, mi::launch_zero(jk->J, nn, s));
+    HIP_TRY(ctx, mi::launch_zero(jk->K, nn, s));
+  }
   // Class launches are independent (disjoint output slots): fork them over the
-  // plan's stream lanes and join back into the caller's stream.  Profiling
-  // (ev != NULL) keeps them on one stream to time each launch.
+  // plan's stream lanes and join back into the caller's stream.
   const bool fork = !ev && P->nlanes > 1;
   hipStream_t lanes[MI_MAX_STREAMS] = {s};
   if (fork) {
@@ -561,13 +735,8 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
   }
   for (size_t i = 0; i < P->launches.size(); ++i) {
     const Launch& L = P->launches[i];
-    hipStream_t ls = fork ? lanes[L.lane] : s;
-    a.tasks = P->d_tasks + L.task_off;
-    a.ntasks = L.ntasks;
-    if (L.reg)
-      HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
-    else
-      HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
+    int rc = launch_one(ctx, L, a, P->d_tasks, fork ? lanes[L.lane] : s);
+    if (rc) return rc;
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
   }
   if (fork)
@@ -575,6 +744,7 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
       HIP_TRY(ctx, hipEventRecord(P->ev_join[k], lanes[k]));
       HIP_TRY(ctx, hipStreamWaitEvent(s, P->ev_join[k], 0));
     }
+  if (jk) HIP_TRY(ctx, mi::launch_symmetrize(jk->J, jk->K, P->nbf, s));
   return MI_OK;
 }
 
@@ -605,6 +775,76 @@ int mi_ctx_set_streams(mi_ctx* ctx, int32_t nstreams) {
   while (avail < MI_MAX_STREAMS && ctx->aux[avail - 1]) ++avail;
   ctx->nstreams = std::min<int>(nstreams, avail);
   return MI_OK;
+}
+
+int mi_plan_graph_capture(mi_plan* P, double* out, int64_t out_len) {
+  if (!P) return MI_EINVAL;
+  mi_ctx* ctx = P->ctx;
+  if (out_len != P->n_unique) return set_err(ctx, MI_ESIZE, "output length does not match the plan");
+  hipStream_t cs = nullptr;
+  HIP_TRY(ctx, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = MI_OK;
+  if (e == hipSuccess) {
+    rc = plan_run(P, out, out_len, cs, nullptr);
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    if (rc == MI_OK && e2 != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "capture failed: %s", hipGetErrorString(e2));
+  } else {
+    rc = set_err(ctx, MI_EDEVICE, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+  }
+  if (rc == MI_OK) {
+    if (P->gexec) hipGraphExecDestroy(P->gexec);
+    P->gexec = nullptr;
+    e = hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  if (g) hipGraphDestroy(g);
+  hipStreamDestroy(cs);
+  return rc;
+}
+
+int mi_plan_graph_launch(mi_plan* P, void* stream) {
+  if (!P) return MI_EINVAL;
+  if (!P->gexec) return set_err(P->ctx, MI_EINVAL, "no captured graph: call mi_plan_graph_capture first");
+  HIP_TRY(P->ctx, hipGraphLaunch(P->gexec, (hipStream_t)stream));
+  return MI_OK;
+}
+
+int mi_plan_jk_execute(mi_plan* P, const double* D, double* J, double* K, void* stream) {
+  if (!P) return MI_EINVAL;
+  if (P->nmol != 1) return set_err(P->ctx, MI_EINVAL, "Coulomb/exchange plans hold one molecule");
+  if (!D || !J || !K) return set_err(P->ctx, MI_EINVAL, "NULL density or output");
+  JKArgs jk{D, J, K};
+  return plan_run(P, nullptr, 0, stream, nullptr, &jk);
+}
+
+int mi_jk(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, const double* D, double* J, double* K) {
+  if (!ctx) return MI_EINVAL;
+  if (!b || !D || !J || !K) return set_err(ctx, MI_EINVAL, "NULL argument");
+  mi_plan* P = nullptr;
+  int rc = mi_plan_create(ctx, b, opts, &P);
+  if (rc) return rc;
+  const size_t nn = (size_t)b->nbf * b->nbf;
+  double *dD = nullptr, *dJ = nullptr, *dK = nullptr;
+  hipStream_t s = opts ? (hipStream_t)opts->stream : nullptr;
+  hipError_t e = hipMalloc(&dD, sizeof(double) * nn);
+  if (e == hipSuccess) e = hipMalloc(&dJ, sizeof(double) * nn);
+  if (e == hipSuccess) e = hipMalloc(&dK, sizeof(double) * nn);
+  if (e == hipSuccess) e = hipMemcpyAsync(dD, D, sizeof(double) * nn, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) rc = set_err(ctx, MI_ENOMEM, "device buffers: %s", hipGetErrorString(e));
+  if (rc == MI_OK) rc = mi_plan_jk_execute(P, dD, dJ, dK, s);
+  if (rc == MI_OK) {
+    e = hipMemcpyAsync(J, dJ, sizeof(double) * nn, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(K, dK, sizeof(double) * nn, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "copy back failed: %s", hipGetErrorString(e));
+  }
+  hipFree(dD);
+  hipFree(dJ);
+  hipFree(dK);
+  mi_plan_destroy(P);
+  return rc;
 }
 
 int mi_plan_nlaunch(const mi_plan* P) { return P ? (int)P->launches.size() : -1; }

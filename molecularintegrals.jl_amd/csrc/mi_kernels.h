@@ -25,5 +25,7 @@ bool wclass_fits_nw(int c, int nw, size_t lds_limit);
 size_t wclass_lds_bytes(int c, int nw);
 bool wclass_fits(int c);
 hipError_t set_wave_kernel_lds_limit(size_t bytes);
+hipError_t launch_zero(double* x, int64_t n, hipStream_t stream);
+hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream);
 
 }  // namespace mi

@@ -9,6 +9,8 @@
 //                          per shell quartet, VRR/HRR staged in LDS).
 // MFMA is not used: HGP's VRR/HRR are short dependent FMA chains with
 // per-quartet coefficients, not dense contractions.
+#include <algorithm>
+
 #include "mi_device.h"
 #include "mi_kernels.h"
 
@@ -57,7 +59,7 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
   pp[r] = o;
   if (local == 0) {
     PairGeom g;
-    g.ABx = ABx; g.ABy = ABy; g.ABz = ABz; g.Q = 0.0;
+    g.ABx = ABx; g.ABy = ABy; g.ABz = ABz; g.Q = 0.0; g.V = 0.0; g.pad = 0.0;
     geom[p] = g;
   }
 }
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
 #pragma unroll
   for (int k = 0; k < W_ACC_MAX; ++k) {
     const int x = k * 64 + lane;
-    asrc[k] = x < D.nacc ? (int)sA[x] : 0;
+    asrc[k] = x < D.nacc ? (int)(sA[x] & 0xFFFF) : 0;  // bit 31 marks a diagonal (e0|e0)
   }
   int ti = 0;
   for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
@@ -132,6 +134,8 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     decode_index(T, q - T.blockStart, qc);
     qc.G = 1;
     qc.g = 0;
+    const int qmode = quartet_mode(args, qc);  // the wave kernel keeps FP64 VRR; 2 = screened
+    if (qmode == 2 && args.mode != MI_MODE_UNIQUE) continue;  // uniform per block
     const PairTopo P = args.topo[qc.bra];
     const PairTopo Q = args.topo[qc.ket];
     if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
@@ -141,7 +145,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     double acc[W_ACC_MAX];
 #pragma unroll
     for (int k = 0; k < W_ACC_MAX; ++k) acc[k] = 0.0;
-    const int K = P.npp * Q.npp;
+    const int K = qmode == 2 ? 0 : P.npp * Q.npp;
     for (int t = wv; t < K; t += nw) {
       const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
       const PrimPair bp = args.pp[P.ppoff + u];
@@ -219,6 +223,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
       double sum = vrr_all[x];
       for (int w = 1; w < nw; ++w) sum += vrr_all[w * D.nvrr + x];
       hrr[x] = sum;
+      if (args.mode == MI_MODE_SCHWARZ && (sA[x] >> 31)) record_vmax(&args.geom[qc.bra], sum);
     }
     __syncthreads();
     // HRR levels: item k writes slot nacc + k.
@@ -241,11 +246,33 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
       const uint32_t o = sO[x];
       const int off = o & 0xFFFF;
       const int ma = (o >> 16) & 15, mb = (o >> 20) & 15, mc = (o >> 24) & 15, md = (o >> 28) & 15;
-      const int64_t slot = unique_slot(sc, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md);
-      if (slot >= 0)
-        sc.out[slot] = hrr[off] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
+      scatter_elem(args, sc, qc.bra, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md,
+                   hrr[off] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]));
     }
     __syncthreads();
+  }
+}
+
+// Zero a device array (a kernel, not a memset node, so captured graphs replay it safely).
+__global__ void __launch_bounds__(256) zero_kernel(double* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
+// J = Jacc + Jacc^T, K = Kacc + Kacc^T, in place (one lane per i <= j).
+__global__ void __launch_bounds__(256) symmetrize_kernel(double* __restrict__ J, double* __restrict__ K, int n) {
+  const int64_t npair = (int64_t)n * (n + 1) / 2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < npair; t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    const int64_t j = t - i * (i + 1) / 2;  // j <= i
+    const double jv = J[i * n + j] + J[j * n + i];
+    const double kv = K[i * n + j] + K[j * n + i];
+    J[i * n + j] = jv;
+    J[j * n + i] = jv;
+    K[i * n + j] = kv;
+    K[j * n + i] = kv;
   }
 }
 
@@ -321,6 +348,20 @@ hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, in
   const WClassDesc D = kWClassDesc[c];
   const size_t lds = wclass_lds_bytes(c, nw);
   hipLaunchKernelGGL(hgp_w_kernel, dim3(nblocks), dim3(64 * nw), lds, stream, args, D, W, nq_launch);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(double* x, int64_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)nb), dim3(256), 0, stream, x, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream) {
+  const int64_t npair = (int64_t)n * (n + 1) / 2;
+  const int64_t nb = std::min<int64_t>((npair + 255) / 256, 4096);
+  hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(256), 0, stream, J, K, n);
   return hipGetLastError();
 }
 

@@ -20,7 +20,9 @@ MI_PREC_FP64, MI_PREC_MIXED = 0, 1
 # Every symbol include/mi_hgp.h declares (checked by the CPU test suite).
 EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique", "mi_eri_unique",
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
-            "mi_plan_launch_info", "mi_plan_profile", "mi_ctx_set_tuning", "mi_ctx_set_streams"]
+            "mi_plan_launch_info", "mi_plan_profile", "mi_ctx_set_tuning", "mi_ctx_set_streams",
+            "mi_batch_plan_create", "mi_plan_out_len", "mi_plan_mol_offsets", "mi_plan_graph_capture",
+            "mi_plan_graph_launch", "mi_jk", "mi_plan_jk_execute"]
 
 
 class MiBasis(ctypes.Structure):
@@ -102,6 +104,22 @@ def load():
     L.mi_ctx_set_tuning.restype = ctypes.c_int
     L.mi_ctx_set_streams.argtypes = [vp, ctypes.c_int32]
     L.mi_ctx_set_streams.restype = ctypes.c_int
+    L.mi_batch_plan_create.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.c_int32, ctypes.POINTER(MiOpts),
+                                       ctypes.POINTER(vp)]
+    L.mi_batch_plan_create.restype = ctypes.c_int
+    L.mi_plan_out_len.argtypes = [vp]
+    L.mi_plan_out_len.restype = ctypes.c_int64
+    L.mi_plan_mol_offsets.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.mi_plan_mol_offsets.restype = ctypes.c_int
+    L.mi_plan_graph_capture.argtypes = [vp, vp, ctypes.c_int64]
+    L.mi_plan_graph_capture.restype = ctypes.c_int
+    L.mi_plan_graph_launch.argtypes = [vp, vp]
+    L.mi_plan_graph_launch.restype = ctypes.c_int
+    dpp = ctypes.POINTER(ctypes.c_double)
+    L.mi_jk.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts), dpp, dpp, dpp]
+    L.mi_jk.restype = ctypes.c_int
+    L.mi_plan_jk_execute.argtypes = [vp, vp, vp, vp, vp]
+    L.mi_plan_jk_execute.restype = ctypes.c_int
     _lib = L
     return L
 

@@ -58,6 +58,20 @@ class Context:
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
         return out
 
+    def jk(self, fb: FlatBasis, D, precision=_lib.MI_PREC_FP64, tau=0.0, schwarz_cut=0.0):
+        """Direct Coulomb and exchange (host arrays): J_ij = sum (ij|kl) D_kl,
+        K_ij = sum (ik|jl) D_kl; make2JmK's G is 2J - K (src/ERI.jl:114-128)."""
+        args = _lib.BasisArgs(fb)
+        n = fb.nbf
+        D = np.ascontiguousarray(D, dtype=np.float64).reshape(n, n)
+        J = np.empty((n, n))
+        K = np.empty((n, n))
+        o = _lib.opts(_lib.MI_ORDER_IINDEX, precision, tau, schwarz_cut)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.check(self.L.mi_jk(self.h, ctypes.byref(args.struct), ctypes.byref(o), D.ctypes.data_as(dp),
+                                J.ctypes.data_as(dp), K.ctypes.data_as(dp)))
+        return J, K
+
     def set_tuning(self, rows):
         """rows: iterable of (class, bra_bucket, ket_bucket, variant, G)."""
         arr = np.ascontiguousarray(np.asarray(list(rows), dtype=np.int32).reshape(-1, 5))
@@ -69,7 +83,12 @@ class Context:
 
     def plan(self, fb: FlatBasis, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
              schwarz_cut=0.0) -> "Plan":
-        return Plan(self, fb, order, precision, tau, schwarz_cut)
+        return Plan(self, [fb], order, precision, tau, schwarz_cut)
+
+    def batch_plan(self, fbs, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
+                   schwarz_cut=0.0) -> "Plan":
+        """One plan over many molecules (outputs back to back, see Plan.mol_offsets)."""
+        return Plan(self, list(fbs), order, precision, tau, schwarz_cut)
 
 
 class Plan:
@@ -77,15 +96,37 @@ class Plan:
     writes all unique ERIs into a caller-owned device buffer (e.g. a torch
     tensor's ``data_ptr()``) on a given HIP stream."""
 
-    def __init__(self, ctx: Context, fb: FlatBasis, order, precision, tau, schwarz_cut):
+    def __init__(self, ctx: Context, fbs, order, precision, tau, schwarz_cut):
         self.ctx = ctx
-        self.fb = fb
-        self._args = _lib.BasisArgs(fb)
+        self.fbs = fbs
+        self.fb = fbs[0]
+        self._args = [_lib.BasisArgs(fb) for fb in fbs]
+        arr = (_lib.MiBasis * len(fbs))(*[a.struct for a in self._args])
         o = _lib.opts(order, precision, tau, schwarz_cut)
         h = ctypes.c_void_p()
-        ctx.check(ctx.L.mi_plan_create(ctx.h, ctypes.byref(self._args.struct), ctypes.byref(o), ctypes.byref(h)))
+        if len(fbs) == 1:
+            ctx.check(ctx.L.mi_plan_create(ctx.h, arr, ctypes.byref(o), ctypes.byref(h)))
+        else:
+            ctx.check(ctx.L.mi_batch_plan_create(ctx.h, arr, len(fbs), ctypes.byref(o), ctypes.byref(h)))
         self.h = h
-        self.n_unique = int(ctx.L.mi_n_unique(fb.nbf))
+        self.n_unique = int(ctx.L.mi_plan_out_len(h))
+
+    def jk_execute(self, D_ptr: int, J_ptr: int, K_ptr: int, stream: int = 0):
+        """Device-buffer Coulomb/exchange for a one-molecule plan."""
+        self.ctx.check(self.ctx.L.mi_plan_jk_execute(self.h, ctypes.c_void_p(D_ptr), ctypes.c_void_p(J_ptr),
+                                                     ctypes.c_void_p(K_ptr), ctypes.c_void_p(stream)))
+
+    def mol_offsets(self) -> np.ndarray:
+        off = np.zeros(len(self.fbs) + 1, dtype=np.int64)
+        self.ctx.check(self.ctx.L.mi_plan_mol_offsets(self.h, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return off
+
+    def capture(self, out_ptr: int):
+        """Capture one execute into a HIP graph (replay with ``replay``)."""
+        self.ctx.check(self.ctx.L.mi_plan_graph_capture(self.h, ctypes.c_void_p(out_ptr), self.n_unique))
+
+    def replay(self, stream: int = 0):
+        self.ctx.check(self.ctx.L.mi_plan_graph_launch(self.h, ctypes.c_void_p(stream)))
 
     def execute(self, out_ptr: int, stream: int = 0):
         self.ctx.check(self.ctx.L.mi_plan_execute(self.h, ctypes.c_void_p(out_ptr), self.n_unique,

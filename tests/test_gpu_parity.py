@@ -219,3 +219,109 @@ def test_stream_fork_join_matches(nstreams):
     ref = O.eri_unique_hgp(fb, nthreads=8)
     assert_eri_close(a, ref, f"streams={nstreams}", fb, O.ORDER_CHRR)
     c.close()
+
+
+def test_batch_plan_matches_each_molecule(ctx):
+    """Many molecules of uneven size in one plan: each molecule's slice of the
+    output equals its own oracle array (both orders)."""
+    from mi_amd import configs
+    mols = configs.batch_molecules(5, nmol=6, natoms_lo=2, natoms_hi=6)
+    fbs = [m.flat() for m in mols]
+    import torch
+    for order in (O.ORDER_CHRR, O.ORDER_IINDEX):
+        plan = ctx.batch_plan(fbs, order=order)
+        off = plan.mol_offsets()
+        assert off[-1] == plan.n_unique == sum(O.n_unique(fb.nbf) for fb in fbs)
+        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+        plan.execute(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        got = out.cpu().numpy()
+        for m, fb in enumerate(fbs):
+            ref = O.eri_unique_hgp(fb, order=order)
+            assert_eri_close(got[off[m]:off[m + 1]], ref, f"batch molecule {m} order={order}", fb, order)
+        plan.close()
+
+
+def test_graph_replay_equals_eager(ctx):
+    """The captured execute, replayed several times with eager work and
+    synchronisations between replays, reproduces the eager output bit for bit."""
+    import torch
+    from mi_amd import configs
+    mols = configs.batch_molecules(6, nmol=4, natoms_lo=3, natoms_hi=7)
+    plan = ctx.batch_plan([m.flat() for m in mols])
+    s = torch.cuda.current_stream()
+    eager = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+    plan.execute(eager.data_ptr(), s.cuda_stream)
+    torch.cuda.synchronize()
+    ref = eager.cpu().numpy()
+    out = torch.empty_like(eager)
+    plan.capture(out.data_ptr())
+    for k in range(3):
+        out.fill_(float("nan"))
+        plan.replay(s.cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().view(np.int64), ref.view(np.int64)), f"replay {k}"
+        plan.execute(eager.data_ptr(), s.cuda_stream)  # eager work between replays
+        torch.cuda.synchronize()
+
+
+def _density(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = rng.normal(size=(n, n)) / n
+    return 0.5 * (d + d.T)
+
+
+@pytest.mark.parametrize("which", ["minimal", "random"])
+def test_jk_matches_oracle(ctx, which):
+    """Direct J/K against the oracle's make2JmK index algebra applied to the
+    oracle's unique array (src/ERI.jl:114-128)."""
+    from mi_amd import configs, flat_from_shells
+    if which == "minimal":
+        fb = configs.ethane_minimal(4).flat()
+    else:
+        fb = flat_from_shells(random_spec(np.random.Generator(np.random.PCG64(77)), 9, kmax=3))
+    D = _density(fb.nbf, 5)
+    J, K = ctx.jk(fb, D)
+    eri = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
+    Jr, Kr = O.jk_from_unique(fb.nbf, eri, D)
+    scale = max(1.0, np.abs(Jr).max())
+    assert np.abs(J - Jr).max() <= 1e-11 * scale
+    assert np.abs(K - Kr).max() <= 1e-11 * scale
+    assert np.array_equal(J, J.T) and np.array_equal(K, K.T)
+
+
+def test_schwarz_screening_bounds_the_error(ctx):
+    """Quartets with Q_ab Q_cd < cut are skipped (written as 0): every
+    integral stays within cut of the oracle, and many are skipped."""
+    from mi_amd import configs
+    fb = configs.random_molecule(np.random.Generator(np.random.PCG64(11)), 10).flat()
+    ref = O.eri_unique_hgp(fb, order=O.ORDER_CHRR)
+    for cut in (1e-10, 1e-12):
+        got = ctx.eri_unique(fb, order=O.ORDER_CHRR, schwarz_cut=cut)
+        assert np.abs(got - ref).max() <= cut
+        zeroed = np.count_nonzero((got == 0.0) & (ref != 0.0))
+        print(f"cut={cut:g}: {zeroed} of {ref.size} integrals screened, max |d| {np.abs(got - ref).max():.2e}")
+        assert zeroed > 0
+
+
+@pytest.mark.parametrize("tau", [1e-8, 1e-6])
+def test_mixed_precision_error(ctx, tau):
+    """FP32 VRR for quartets with Q_ab Q_cd < tau: max |dERI| against the FP64
+    oracle stays <= 1e-10 Eh (the bar for the mixed form)."""
+    from mi_amd import configs
+    for seed in (0, 1):
+        fb = configs.random_molecule(np.random.Generator(np.random.PCG64(20 + seed)), 12).flat()
+        ref = O.eri_unique_hgp(fb, order=O.ORDER_CHRR, nthreads=8)
+        got = ctx.eri_unique(fb, order=O.ORDER_CHRR, precision=1, tau=tau)
+        err = np.abs(got - ref).max()
+        print(f"tau={tau:g} seed={seed}: max |dERI| = {err:.3e}")
+        assert err <= 1e-10
+
+
+def test_jk_mixed_and_screened(ctx):
+    from mi_amd import configs
+    fb = configs.random_molecule(np.random.Generator(np.random.PCG64(31)), 10).flat()
+    D = _density(fb.nbf, 6)
+    J0, K0 = ctx.jk(fb, D)
+    J1, K1 = ctx.jk(fb, D, precision=1, tau=1e-8, schwarz_cut=1e-12)
+    print(f"J/K mixed+screened vs FP64: {np.abs(J1 - J0).max():.2e} {np.abs(K1 - K0).max():.2e}")
+    assert np.abs(J1 - J0).max() <= 1e-9 and np.abs(K1 - K0).max() <= 1e-9
