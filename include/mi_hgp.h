@@ -133,6 +133,13 @@ int mi_plan_mol_offsets(const mi_plan *plan, int64_t *offsets);
 int mi_jk(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, const double *D, double *J, double *K);
 int mi_plan_jk_execute(mi_plan *plan, const double *D_device, double *J_device, double *K_device, void *stream);
 
+/* Per shell-pair bounds from the last execute of a plan that has a Schwarz
+ * pass (MI_PREC_MIXED or schwarz_cut > 0): Q = sqrt(max |(ab|ab)|),
+ * V = sqrt(max |(e0|e0)|) over the pair's VRR outputs, and the molecule of
+ * each pair; npair entries in the plan's pair order (mi_plan_stats[1]).
+ * Synchronous. */
+int mi_plan_pair_bounds(const mi_plan *plan, double *Q, double *V, int32_t *mol);
+
 /* Capture one execute (writing out_device) into a HIP graph owned by the
  * plan; mi_plan_graph_launch replays it on any stream.  The graph holds no
  * memset nodes. */

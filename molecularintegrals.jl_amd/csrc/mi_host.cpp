@@ -188,6 +188,7 @@ struct mi_plan {
   std::vector<Launch> launches;   // in issue order (cost descending)
   std::vector<Launch> slaunches;  // Schwarz pass (diagonal quartets), run before the main launches
   bool need_schwarz = false;
+  std::vector<int32_t> pair_mol;  // molecule of each pair record
   double tau2 = 0.0, cut2 = 0.0;
   int nlanes = 1;
   hipGraphExec_t gexec = nullptr;  // captured execute (mi_plan_graph_capture)
@@ -477,6 +478,8 @@ static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opt
   }
   P->npair = npair;
   P->npp = (int)ppoff;
+  P->pair_mol.resize(npair);
+  for (int r = 0; r < npair; ++r) P->pair_mol[r] = keys[r].mol;
   // Contiguous lists of equal (molecule, pc, bucket).
   struct List {
     int start, n, pc, Ksum, Kmax, mol;
@@ -845,6 +848,20 @@ int mi_jk(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, const double* D, 
   hipFree(dK);
   mi_plan_destroy(P);
   return rc;
+}
+
+int mi_plan_pair_bounds(const mi_plan* P, double* Q, double* V, int32_t* mol) {
+  if (!P || !Q || !V || !mol) return MI_EINVAL;
+  if (!P->need_schwarz) return set_err(P->ctx, MI_EINVAL, "plan has no Schwarz pass (plain FP64 without screening)");
+  std::vector<PairGeom> g(P->npair);
+  std::vector<PairTopo> t(P->npair);
+  HIP_TRY(P->ctx, hipMemcpy(g.data(), P->d_geom, sizeof(PairGeom) * g.size(), hipMemcpyDeviceToHost));
+  for (int r = 0; r < P->npair; ++r) {
+    Q[r] = std::sqrt(g[r].Q);
+    V[r] = std::sqrt(g[r].V);
+    mol[r] = P->pair_mol[r];
+  }
+  return MI_OK;
 }
 
 int mi_plan_nlaunch(const mi_plan* P) { return P ? (int)P->launches.size() : -1; }

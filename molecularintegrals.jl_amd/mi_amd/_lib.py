@@ -22,7 +22,7 @@ EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique"
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
             "mi_plan_launch_info", "mi_plan_profile", "mi_ctx_set_tuning", "mi_ctx_set_streams",
             "mi_batch_plan_create", "mi_plan_out_len", "mi_plan_mol_offsets", "mi_plan_graph_capture",
-            "mi_plan_graph_launch", "mi_jk", "mi_plan_jk_execute"]
+            "mi_plan_graph_launch", "mi_jk", "mi_plan_jk_execute", "mi_plan_pair_bounds"]
 
 
 class MiBasis(ctypes.Structure):
@@ -120,6 +120,8 @@ def load():
     L.mi_jk.restype = ctypes.c_int
     L.mi_plan_jk_execute.argtypes = [vp, vp, vp, vp, vp]
     L.mi_plan_jk_execute.restype = ctypes.c_int
+    L.mi_plan_pair_bounds.argtypes = [vp, dpp, dpp, ctypes.POINTER(ctypes.c_int32)]
+    L.mi_plan_pair_bounds.restype = ctypes.c_int
     _lib = L
     return L
 

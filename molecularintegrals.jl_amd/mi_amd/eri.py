@@ -116,6 +116,17 @@ class Plan:
         self.ctx.check(self.ctx.L.mi_plan_jk_execute(self.h, ctypes.c_void_p(D_ptr), ctypes.c_void_p(J_ptr),
                                                      ctypes.c_void_p(K_ptr), ctypes.c_void_p(stream)))
 
+    def pair_bounds(self):
+        """(Q, V, mol) per shell pair after an execute with a Schwarz pass."""
+        n = self.stats()["shell_pairs"]
+        Q = np.empty(n)
+        V = np.empty(n)
+        mol = np.empty(n, dtype=np.int32)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.ctx.check(self.ctx.L.mi_plan_pair_bounds(self.h, Q.ctypes.data_as(dp), V.ctypes.data_as(dp),
+                                                      mol.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return Q, V, mol
+
     def mol_offsets(self) -> np.ndarray:
         off = np.zeros(len(self.fbs) + 1, dtype=np.int64)
         self.ctx.check(self.ctx.L.mi_plan_mol_offsets(self.h, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
