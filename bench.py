@@ -127,16 +127,16 @@ def main():
     mean_ms = acc / nprof
     per = []
     step_flops = 0
-    for i, (cls, nqs, nqp, nblk, reg) in enumerate(launches):
+    for i, (cls, nqs, nqp, nblk, kern) in enumerate(launches):
         f = flops.class_flops(cls, nqs, nqp)
         step_flops += f
-        per.append((mean_ms[i + 1], cls, f, nqs, nqp, reg))
+        per.append((mean_ms[i + 1], cls, f, nqs, nqp, kern))
     dom = max(per, key=lambda x: x[0])
     dom_ms, dom_cls, dom_flops = dom[0], dom[1], dom[2]
-    name = ("hgp_r_" if dom[5] else "hgp_w_kernel:") + "".join("spdf"[x] for x in dom_cls)
+    name = {"R": "hgp_r_", "W": "hgp_w_kernel:", "G": "hgp_wg_kernel:"}[dom[5][0]] + "".join("spdf"[x] for x in dom_cls)
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    for ms_, cls, f, nqs, nqp, reg in sorted(per, reverse=True)[:8]:
-        log(f"[bench]   {''.join('spdf'[x] for x in cls)} {'R' if reg else 'W'} {ms_ * 1e3:8.2f} us "
+    for ms_, cls, f, nqs, nqp, kern in sorted(per, reverse=True)[:8]:
+        log(f"[bench]   {''.join('spdf'[x] for x in cls)} {kern} {ms_ * 1e3:8.2f} us "
             f"{f / (ms_ * 1e-3) / 1e12:7.3f} TFLOP/s  quartets={nqs} prim={nqp}")
     log(f"[bench]   pair kernel {mean_ms[0] * 1e3:.2f} us; sum of launches {mean_ms.sum() * 1e3:.1f} us")
 

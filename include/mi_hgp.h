@@ -156,12 +156,18 @@ int mi_plan_stats(const mi_plan *plan, int64_t *stats, int32_t nstats);
  * launches 0..n-1, one per quartet class present. */
 int mi_plan_nlaunch(const mi_plan *plan);
 /* cls4 = (la,lb,lc,ld); counts3 = (shell quartets, primitive quartets,
- * blocks).  Returns 1 for a register-kernel class, 0 for the LDS kernel,
- * MI_EINVAL (negative never) on a bad index -- check with the count first. */
-int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *counts3);
+ * blocks); kernel2 = (variant, wavefronts per block): variant 1 = generated
+ * register kernel (wavefronts 0), 0 = LDS wave kernel, 2 = global-table
+ * kernel.  Returns MI_OK or MI_EINVAL on a bad index. */
+int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *counts3, int32_t *kernel2);
 /* One execute with HIP events between launches on `stream`; synchronises and
  * writes nlaunch+1 durations in ms (ms[0] = shell-pair kernel). */
 int mi_plan_profile(mi_plan *plan, double *out_device, int64_t out_len, void *stream, float *ms);
+/* The same for one Coulomb/exchange execute (device arrays as in
+ * mi_plan_jk_execute); ms[0] includes the Schwarz pass, ms[1] the zeroing
+ * of J and K. */
+int mi_plan_jk_profile(mi_plan *plan, const double *D_device, double *J_device, double *K_device, void *stream,
+                       float *ms);
 
 #ifdef __cplusplus
 }

@@ -183,6 +183,21 @@ def emit_rkernel(c):
     w(f"  double sA[{ncart(la)}], sB[{ncart(lb)}], sC[{ncart(lc)}], sD[{ncart(ld)}];")
     w(f"  load_scales<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, P, Q, sA, sB, sC, sD);")
     w("  const ScatterCtx sc = make_scatter(args, qc, P, Q);")
+    # Coulomb/exchange: contract the scaled block in registers, a few atomics per quartet
+    vals = []
+    for ma, a in enumerate(shell_aos(la)):
+        for mb, b in enumerate(shell_aos(lb)):
+            for mc, cc in enumerate(shell_aos(lc)):
+                for md, d in enumerate(shell_aos(ld)):
+                    vals.append(f"h{a}_{b}_{cc}_{d} * (sA[{ma}] * sB[{mb}]) * (sC[{mc}] * sD[{md}])")
+    w("  if (args.mode == MI_MODE_JK) {")
+    w(f"    const double ov[{len(vals)}] = {{")
+    for k in range(0, len(vals), 2):
+        w("      " + ", ".join(vals[k:k + 2]) + ",")
+    w("    };")
+    w(f"    jk_block_regs<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, sc, ov);")
+    w("    return;")
+    w("  }")
     idx = 0
     for ma, a in enumerate(shell_aos(la)):
         for mb, b in enumerate(shell_aos(lb)):
@@ -201,18 +216,17 @@ def emit_rkernel(c):
 def w_tables(c):
     """Op tables for the wave-cooperative kernel.
 
-    VRR: LDS holds [e|f]^(m) for the pruned (e,f) nodes, m contiguous, nodes in
-    level order; items enumerate the same order, so item k of the class writes
-    LDS slot vt0 + k (targets are implicit).  Item = uint2:
-      x = s1 | s2 << 16
-      y = s3 | dir << 16 | n1 << 18 | n2 << 21 | kind << 24 | has2 << 25 | has3 << 26
+    VRR: the work area holds [e|f]^(m) for the pruned (e,f) nodes, m contiguous,
+    nodes in level order; items enumerate the same order, so item k of the class
+    writes slot vt0 + k (targets are implicit).  Items are returned unpacked as
+    (s1, s2, s3, dir, n1, n2, kind, has2, has3); pack16/pack32 encode them.
     kind 0 (a-build): PA_i [s1] + WP_i [s1+1] + n1/(2 zeta) ([s2] - eta/s [s2+1])
     kind 1 (c-build): QC_i [s1] + WQ_i [s1+1] + n1/(2 s) [s3+1]
                       + n2/(2 eta) ([s2] - zeta/s [s2+1])
-    HRR: LDS slots 0..nacc-1 are the contracted (e0|f0) (accsrc[x] = VRR slot
-    of its m=0 value); item k writes slot nacc + k:
-      x = s1 | s2 << 16,  y = dir | ket << 2     ->  [s1] + X_dir [s2]
-    Out: hrr slot | ma << 16 | mb << 20 | mc << 24 | md << 28.
+    HRR: slots 0..nacc-1 are the contracted (e0|f0) (accsrc[x] = VRR slot
+    of its m=0 value, bit 31 set for a diagonal e == f node); item k writes slot
+    nacc + k as (s1, s2, dir, ket)  ->  [s1] + X_dir [s2].
+    Out: (hrr slot, ma, mb, mc, md).
     """
     la, lb, lc, ld = c
     g = VrrGraph(*c)
@@ -241,7 +255,7 @@ def w_tables(c):
                     s1 = base[(e1, 0)] + m
                     has2 = e2 >= 0
                     s2 = base[(e2, 0)] + m if has2 else 0
-                    vitems.append((s1 | (s2 << 16), (i << 16) | (n1 << 18) | (0 << 24) | (int(has2) << 25)))
+                    vitems.append((s1, s2, 0, i, n1, 0, 0, int(has2), 0))
                 else:
                     _, i, f1, f2, e1, ne, nf = rec
                     s1 = base[(e, f1)] + m
@@ -249,8 +263,7 @@ def w_tables(c):
                     s3 = base[(e1, f1)] + m if has3 else 0
                     has2 = f2 >= 0
                     s2 = base[(e, f2)] + m if has2 else 0
-                    vitems.append((s1 | (s2 << 16), s3 | (i << 16) | (ne << 18) | (nf << 21) | (1 << 24)
-                                   | (int(has2) << 25) | (int(has3) << 26)))
+                    vitems.append((s1, s2, s3, i, ne, nf, 1, int(has2), int(has3)))
     vlev.append(len(vitems))
     acc_nodes = [n for n in h.nodes if h.deps(n)[0] == "acc"]
     hlevels = sorted(set(h.level(n) for n in h.nodes if h.level(n) > 0))
@@ -271,18 +284,59 @@ def w_tables(c):
                 continue
             kind, s1, s2, j = h.deps(n)
             assert hbase[n] == len(acc_nodes) + len(hitems)
-            hitems.append((hbase[s1] | (hbase[s2] << 16), j | ((1 if kind == "ket" else 0) << 2)))
+            hitems.append((hbase[s1], hbase[s2], j, 1 if kind == "ket" else 0))
     hlev.append(len(hitems))
     outs = []
     for ma, a in enumerate(shell_aos(la)):
         for mb, b in enumerate(shell_aos(lb)):
             for mc, cc in enumerate(shell_aos(lc)):
                 for md, d in enumerate(shell_aos(ld)):
-                    outs.append(hbase[(a, b, cc, d)] | (ma << 16) | (mb << 20) | (mc << 24) | (md << 28))
-    assert nvrr < 65536 and nhrr < 65536
+                    outs.append((hbase[(a, b, cc, d)], ma, mb, mc, md))
+    assert len(outs) <= nvrr  # the Coulomb/exchange path stages the block in the VRR area
     desc = dict(la=la, lb=lb, lc=lc, ld=ld, mmax=g.mmax, nvrr=nvrr, nhrr=nhrr, vt0=vt0,
                 vlev=vlev, hlev=hlev, nacc=len(accsrc), nout=len(outs))
     return desc, vitems, accsrc, hitems, outs
+
+
+def pack16(vi, hi, ou, am):
+    """LDS-kernel blob: uint2 items with 16-bit slot fields.
+      VRR  x = s1 | s2 << 16, y = s3 | dir << 16 | n1 << 18 | n2 << 21 | kind << 24 | has2 << 25 | has3 << 26
+      HRR  x = s1 | s2 << 16, y = dir | ket << 2
+      out  slot | ma << 16 | mb << 20 | mc << 24 | md << 28;  acc as is."""
+    words = []
+    for s1, s2, s3, dr, n1, n2, kind, h2, h3 in vi:
+        words += [s1 | (s2 << 16), s3 | (dr << 16) | (n1 << 18) | (n2 << 21) | (kind << 24) | (h2 << 25) | (h3 << 26)]
+    for s1, s2, dr, ket in hi:
+        words += [s1 | (s2 << 16), dr | (ket << 2)]
+    words += [o | (ma << 16) | (mb << 20) | (mc << 24) | (md << 28) for o, ma, mb, mc, md in ou]
+    words += list(am)
+    while len(words) % 4:
+        words.append(0)
+    return words
+
+
+def pack32(vi, hi, ou, am):
+    """Global-kernel tables, 32-bit slot fields; returns (words, offsets).
+      VRR  uint4 (s1, s2, s3, dir | n1 << 2 | n2 << 5 | kind << 8 | has2 << 9 | has3 << 10)
+      HRR  uint4 (s1, s2, dir | ket << 2, 0)
+      out  uint2 (slot, ma | mb << 4 | mc << 8 | md << 12)
+      acc  uint2 (VRR slot, diagonal flag)"""
+    w = []
+    gv = 0
+    for s1, s2, s3, dr, n1, n2, kind, h2, h3 in vi:
+        w += [s1, s2, s3, dr | (n1 << 2) | (n2 << 5) | (kind << 8) | (h2 << 9) | (h3 << 10)]
+    gh = len(w)
+    for s1, s2, dr, ket in hi:
+        w += [s1, s2, dr | (ket << 2), 0]
+    go = len(w)
+    for o, ma, mb, mc, md in ou:
+        w += [o, ma | (mb << 4) | (mc << 8) | (md << 12)]
+    ga = len(w)
+    for a_ in am:
+        w += [a_ & 0x7FFFFFFF, a_ >> 31]
+    while len(w) % 4:
+        w.append(0)
+    return w, (gv, gh, go, ga)
 
 
 # ---------------------------------------------------------------- driver
@@ -316,23 +370,23 @@ def main():
     parts.append("};")
     with open(os.path.join(args.out, "hgp_rkernels.inc"), "w") as f:
         f.write("\n".join(parts) + "\n")
-    # W tables: one contiguous blob per class, 16-byte aligned, laid out as
-    # [VRR items (2 words) | HRR items (2 words) | out words | acc sources],
-    # so a block stages it with wide loads.
-    descs, BLOB = [], []
+    # LDS-kernel tables: one contiguous blob per class, 16-byte aligned, laid
+    # out as [VRR items | HRR items | out words | acc sources], so a block
+    # stages it with wide loads.  Only classes whose slots fit 16 bits and whose
+    # contracted block fits the per-lane accumulators (nacc <= 1024) get one;
+    # boff = -1 otherwise.  Global-kernel tables (pack32) exist for every class.
+    descs, BLOB, GBLOB = [], [], []
     for c in classes:
         d, vi, am, hi, ou = w_tables(c)
-        d["boff"] = len(BLOB) // 4
-        words = []
-        for x, y in vi:
-            words += [x, y]
-        for x, y in hi:
-            words += [x, y]
-        words += list(ou) + list(am)
-        while len(words) % 4:
-            words.append(0)
-        d["bwords"] = len(words)
-        BLOB += words
+        if d["nvrr"] < 65536 and d["nhrr"] < 65536 and d["nacc"] <= 1024:
+            words = pack16(vi, hi, ou, am)
+            d["boff"], d["bwords"] = len(BLOB) // 4, len(words)
+            BLOB += words
+        else:
+            d["boff"], d["bwords"] = -1, 0
+        words, offs = pack32(vi, hi, ou, am)
+        d["gv"], d["gh"], d["go"], d["ga"] = (len(GBLOB) + o for o in offs)
+        GBLOB += words
         descs.append(d)
     maxlev = max(len(d["vlev"]) for d in descs)
     maxhlev = max(len(d["hlev"]) for d in descs)
@@ -347,15 +401,22 @@ def main():
         hl = list(d["hlev"]) + [d["hlev"][-1]] * (maxhlev - len(d["hlev"]))
         t.append(f"  {{{d['la']}, {d['lb']}, {d['lc']}, {d['ld']}, {d['mmax']}, {d['nvrr']}, {d['nhrr']}, "
                  f"{d['vt0']}, {len(d['vlev']) - 1}, {len(d['hlev']) - 1}, {d['nacc']}, {d['nout']}, "
-                 f"{d['boff']}, {d['bwords']}, "
+                 f"{d['boff']}, {d['bwords']}, {d['gv']}, {d['gh']}, {d['go']}, {d['ga']}, "
                  f"{{{', '.join(map(str, vl))}}}, {{{', '.join(map(str, hl))}}}, "
                  f"{1 if c in rcls else 0}}},  // {cname(c)}")
     t.append("};")
     t.append("static const uint32_t kWBlob[] __attribute__((aligned(16))) = {")
     for k in range(0, len(BLOB), 12):
         t.append("  " + ", ".join(f"{x}u" for x in BLOB[k:k + 12]) + ",")
+    if not BLOB:
+        t.append("  0u,")
     t.append("};")
     t.append(f"static const int kWBlobWords = {len(BLOB)};")
+    t.append("static const uint32_t kWGBlob[] __attribute__((aligned(16))) = {")
+    for k in range(0, len(GBLOB), 16):
+        t.append("  " + ",".join(f"{x}u" for x in GBLOB[k:k + 16]) + ",")
+    t.append("};")
+    t.append(f"static const int kWGBlobWords = {len(GBLOB)};")
     with open(os.path.join(args.out, "hgp_wtables.inc"), "w") as f:
         f.write("\n".join(t) + "\n")
     print(f"gen_hgp: {len(classes)} classes, {len(rcls)} register kernels: "

@@ -104,9 +104,10 @@ struct WClassDesc {
   int la, lb, lc, ld, mmax;
   int nvrr, nhrr, vt0;       // vt0: LDS slot written by VRR item 0
   int nvlev, nhlev, nacc, nout;
-  int boff, bwords;          // op-table blob: 16-byte offset and 32-bit word count
-  int vlev[16];
-  int hlev[8];
+  int boff, bwords;          // LDS-kernel op-table blob: 16-byte offset (-1: none) and word count
+  int gv, gh, go, ga;        // global-kernel tables: word offsets of VRR, HRR, out, acc sections
+  int vlev[16];  // level starts (VRR items)
+  int hlev[10];
   int is_register;
 };
 
@@ -345,6 +346,158 @@ __device__ __forceinline__ void scatter_elem(const EriArgs& a, const ScatterCtx&
   atomicAdd(&a.Kacc[(int64_t)j * n + k], w * D[(int64_t)i * n + l]);
   atomicAdd(&a.Kacc[(int64_t)i * n + l], w * D[(int64_t)j * n + k]);
   atomicAdd(&a.Kacc[(int64_t)j * n + l], w * D[(int64_t)i * n + k]);
+}
+
+// Coulomb/exchange from one whole shell-quartet block V[a][b][c][d]
+// (na nb nc nd, scaled values).  Shell-level degeneracy: the block of a
+// unique shell quartet carries every element of its diagonal sub-blocks, so
+// f = 1/2 per coincidence (A = B, C = D, AB = CD) weights each of the 8
+// permutations once (the element-level rule of make2JmK, src/ERI.jl:114-128,
+// summed over the block):
+//   Jacc[ab] += 2f sum_cd V D_cd    Jacc[cd] += 2f sum_ab V D_ab
+//   Kacc[ac] += f sum_bd V D_bd     Kacc[ad] += f sum_bc V D_bc
+//   Kacc[bc] += f sum_ad V D_ad     Kacc[bd] += f sum_ac V D_ac
+// and J = Jacc + Jacc^T, K = Kacc + Kacc^T afterwards.  One atomic per
+// output element of the six contractions instead of six per integral.
+__device__ __forceinline__ double jk_factor(const ScatterCtx& s) {
+  return (s.braSame ? 0.5 : 1.0) * (s.ketSame ? 0.5 : 1.0) * (s.quartetSame ? 0.5 : 1.0);
+}
+
+// Register-resident block (generated kernels): outputs t = 0..T-1 of the six
+// contractions are split over the G lanes of the quartet (t mod G).
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void jk_block_regs(const EriArgs& a, const ScatterCtx& s,
+                                              const double (&v)[NA * NB * NC * ND]) {
+  const double f = jk_factor(s);
+  const int64_t n = a.nbf;
+  const double* __restrict__ D = a.D;
+  const int64_t ra = s.fa0 * n, rb = s.fb0 * n, rc = s.fc0 * n;
+#define V_(i, j, k, l) v[(((i) * NB + (j)) * NC + (k)) * ND + (l)]
+  constexpr int T0 = NA * NB, T1 = T0 + NC * ND, T2 = T1 + NA * NC, T3 = T2 + NA * ND, T4 = T3 + NB * NC;
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (((i * NB + j) & s.gmask) == s.g) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+          for (int l = 0; l < ND; ++l) acc += V_(i, j, k, l) * D[rc + k * n + s.fd0 + l];
+        atomicAdd(&a.Jacc[ra + i * n + s.fb0 + j], 2.0 * f * acc);
+      }
+#pragma unroll
+  for (int k = 0; k < NC; ++k)
+#pragma unroll
+    for (int l = 0; l < ND; ++l)
+      if (((T0 + k * ND + l) & s.gmask) == s.g) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) acc += V_(i, j, k, l) * D[ra + i * n + s.fb0 + j];
+        atomicAdd(&a.Jacc[rc + k * n + s.fd0 + l], 2.0 * f * acc);
+      }
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (((T1 + i * NC + k) & s.gmask) == s.g) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int l = 0; l < ND; ++l) acc += V_(i, j, k, l) * D[rb + j * n + s.fd0 + l];
+        atomicAdd(&a.Kacc[ra + i * n + s.fc0 + k], f * acc);
+      }
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int l = 0; l < ND; ++l)
+      if (((T2 + i * ND + l) & s.gmask) == s.g) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int k = 0; k < NC; ++k) acc += V_(i, j, k, l) * D[rb + j * n + s.fc0 + k];
+        atomicAdd(&a.Kacc[ra + i * n + s.fd0 + l], f * acc);
+      }
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (((T3 + j * NC + k) & s.gmask) == s.g) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int l = 0; l < ND; ++l) acc += V_(i, j, k, l) * D[ra + i * n + s.fd0 + l];
+        atomicAdd(&a.Kacc[rb + j * n + s.fc0 + k], f * acc);
+      }
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+#pragma unroll
+    for (int l = 0; l < ND; ++l)
+      if (((T4 + j * ND + l) & s.gmask) == s.g) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int k = 0; k < NC; ++k) acc += V_(i, j, k, l) * D[ra + i * n + s.fc0 + k];
+        atomicAdd(&a.Kacc[rb + j * n + s.fd0 + l], f * acc);
+      }
+#undef V_
+}
+
+// Block staged in LDS (wave kernels): thread tid of nthr takes outputs
+// tid, tid + nthr, ... of the six contractions.
+__device__ __forceinline__ void jk_block_lds(const EriArgs& a, const ScatterCtx& s, const double* V, int na, int nb,
+                                             int nc, int nd, int tid, int nthr) {
+  const double f = jk_factor(s);
+  const int64_t n = a.nbf;
+  const double* __restrict__ D = a.D;
+  const int nab = na * nb, ncd = nc * nd, nac = na * nc, nad = na * nd, nbc = nb * nc, nbd = nb * nd;
+  const int total = nab + ncd + nac + nad + nbc + nbd;
+  for (int t = tid; t < total; t += nthr) {
+    int u = t;
+    double acc = 0.0, w = f;
+    double* dst;
+    if (u < nab) {
+      const int i = u / nb, j = u - i * nb;
+      for (int k = 0; k < nc; ++k)
+        for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fc0 + k) * n + s.fd0 + l];
+      dst = &a.Jacc[(s.fa0 + i) * n + s.fb0 + j];
+      w = 2.0 * f;
+    } else if ((u -= nab) < ncd) {
+      const int k = u / nd, l = u - k * nd;
+      for (int i = 0; i < na; ++i)
+        for (int j = 0; j < nb; ++j) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fa0 + i) * n + s.fb0 + j];
+      dst = &a.Jacc[(s.fc0 + k) * n + s.fd0 + l];
+      w = 2.0 * f;
+    } else if ((u -= ncd) < nac) {
+      const int i = u / nc, k = u - i * nc;
+      for (int j = 0; j < nb; ++j)
+        for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fb0 + j) * n + s.fd0 + l];
+      dst = &a.Kacc[(s.fa0 + i) * n + s.fc0 + k];
+    } else if ((u -= nac) < nad) {
+      const int i = u / nd, l = u - i * nd;
+      for (int j = 0; j < nb; ++j)
+        for (int k = 0; k < nc; ++k) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fb0 + j) * n + s.fc0 + k];
+      dst = &a.Kacc[(s.fa0 + i) * n + s.fd0 + l];
+    } else if ((u -= nad) < nbc) {
+      const int j = u / nc, k = u - j * nc;
+      for (int i = 0; i < na; ++i)
+        for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fa0 + i) * n + s.fd0 + l];
+      dst = &a.Kacc[(s.fb0 + j) * n + s.fc0 + k];
+    } else {
+      u -= nbc;
+      const int j = u / nd, l = u - j * nd;
+      for (int i = 0; i < na; ++i)
+        for (int k = 0; k < nc; ++k) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fa0 + i) * n + s.fc0 + k];
+      dst = &a.Kacc[(s.fb0 + j) * n + s.fd0 + l];
+    }
+    atomicAdd(dst, w * acc);
+  }
 }
 
 __device__ __forceinline__ void scatter_one(const EriArgs& a, const ScatterCtx& s, int idx, int ma, int mb, int mc,

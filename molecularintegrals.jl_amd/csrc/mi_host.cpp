@@ -30,7 +30,8 @@
 #define MI_MAX_STREAMS 4
 
 // Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
-// generated register kernel with G lanes per quartet, 0 = LDS wave kernel.
+// generated register kernel with G lanes per quartet, 0 = LDS wave kernel with
+// G wavefronts per quartet, 2 = global-table kernel with G wavefronts per block.
 struct TuneEntry {
   int variant = -1;  // -1: not tuned, use the heuristic
   int G = 1;
@@ -151,6 +152,8 @@ void load_tuning_file(mi_ctx* c) {
     if (var == 1 && !mi::has_register_kernel(cls)) continue;
     if (G < 1 || G > 64 || (G & (G - 1))) continue;
     if (var == 0 && !mi::wclass_fits_nw(cls, G, c->lds_limit)) continue;
+    if (var == 2 && !mi::wgclass_fits_nw(cls, G, c->lds_limit)) continue;
+    if (var < 0 || var > 2) continue;
     c->tune[tune_index(cls, bb, kb)] = {var, G};
   }
 }
@@ -159,8 +162,9 @@ void load_tuning_file(mi_ctx* c) {
 
 struct Launch {
   int cls;
-  bool reg;
-  int nw;  // wave kernel: wavefronts per block
+  int var;  // 1 register kernel, 0 LDS wave kernel, 2 global-table kernel
+  int nw;   // wave kernels: wavefronts per block
+  size_t scratch_off;  // global-table kernel: offset of its slice of the plan scratch (doubles)
   int task_off, ntasks, nblocks;
   int64_t nq_shell, nq_prim;
   double cost;  // relative device time estimate (longest launches go first)
@@ -185,6 +189,8 @@ struct mi_plan {
   PrimPair* d_pp = nullptr;
   int32_t* d_pp_pair = nullptr;
   TaskDesc* d_tasks = nullptr;
+  double* d_scratch = nullptr;     // HRR work areas of the global-table launches
+  size_t scratch_doubles = 0;
   std::vector<Launch> launches;   // in issue order (cost descending)
   std::vector<Launch> slaunches;  // Schwarz pass (diagonal quartets), run before the main launches
   bool need_schwarz = false;
@@ -245,6 +251,7 @@ void free_plan_buffers(mi_plan* p) {
   hipFree(p->d_pp);
   hipFree(p->d_pp_pair);
   hipFree(p->d_tasks);
+  hipFree(p->d_scratch);
 }
 
 }  // namespace
@@ -320,7 +327,8 @@ void mi_destroy(mi_ctx* ctx) {
 // kernels is per molecule.
 // Kernel choice for a task from the tuning table, else the heuristic:
 // register kernel when one exists with lanes ~ primitive quartets / 4, else the
-// wave kernel with as many wavefronts (<= 4) as the LDS budget allows.
+// wave kernel with as many wavefronts (<= 4) as the LDS budget allows, else the
+// global-table kernel with 4 wavefronts per block.
 static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg) {
   const TuneEntry& te = ctx->tune[tune_index(t.cls, bb, kb)];
   int variant, G;
@@ -335,9 +343,14 @@ static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg) 
     } else {
       G = 4;
       while (G > 1 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) G >>= 1;
+      if (!mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) {
+        variant = 2;
+        G = 4;
+      }
     }
   }
-  if (variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit))
+  if ((variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) ||
+      (variant == 2 && !mi::wgclass_fits_nw(t.cls, G, ctx->lds_limit)))
     return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
   t.G = G;
   t.variant = variant;
@@ -351,28 +364,33 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
   // One launch per (class, variant); within a launch, tasks with the most
   // work per block come first so the longest blocks start earliest.
   for (int c = 0; c < mi::nclass(); ++c) {
-    for (int vk = 0; vk < 4; ++vk) {  // 0: register kernel; 1,2,3: wave kernel with 1,2,4 wavefronts
-      const int var = vk == 0 ? 1 : 0, nw = vk == 0 ? 0 : (1 << (vk - 1));
+    // vk 0: register kernel; 1-3: wave kernel with 1, 2, 4 wavefronts;
+    // 4-7: global-table kernel with 1, 2, 4, 8 wavefronts
+    for (int vk = 0; vk < 8; ++vk) {
+      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : 2);
+      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (1 << (vk - 4)));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
         if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
       if (ts.empty()) continue;
       Launch L{};
       L.cls = c;
-      L.reg = var == 1;
+      L.var = var;
       L.nw = nw;
+      const bool reg = var == 1;
       L.task_off = (int)all_tasks.size();
       L.ntasks = (int)ts.size();
       int64_t blocks = 0;
       int64_t qs = 0;
       for (TaskDesc& t : ts) {
         // register kernel: first block of the task; wave kernel: first quartet
-        t.blockStart = (int32_t)(L.reg ? blocks : qs);
-        blocks += L.reg ? (t.nq * t.G + 255) / 256 : 0;
+        t.blockStart = (int32_t)(reg ? blocks : qs);
+        blocks += reg ? (t.nq * t.G + 255) / 256 : 0;
         qs += t.nq;
         all_tasks.push_back(t);
       }
-      if (!L.reg) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
+      if (var == 0) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
+      if (var == 2) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wg_kernel_blocks_per_cu(c, nw));
       if (blocks > INT32_MAX || qs > INT32_MAX) {
         return set_err(ctx, MI_ENOMEM, "class %d: %lld quartets exceed one launch", c, (long long)qs);
       }
@@ -381,6 +399,7 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
       L.nq_prim = 0;
       for (const TaskDesc& t : ts) L.nq_prim += t.nprim;
       L.cost = 0.0;
+      L.scratch_off = 0;
       launches.push_back(L);
     }
   }
@@ -602,6 +621,23 @@ static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opt
   }
   std::stable_sort(P->launches.begin(), P->launches.end(),
                    [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
+  // Scratch of the global-table launches: the main launches may run
+  // concurrently (stream lanes), so each gets its own slice; the Schwarz pass
+  // runs before them on the caller's stream and reuses the same memory.
+  {
+    size_t off = 0, soff = 0;
+    for (Launch& L : P->launches)
+      if (L.var == 2) {
+        L.scratch_off = off;
+        off += mi::wg_scratch_doubles(L.cls, L.nblocks);
+      }
+    for (Launch& L : P->slaunches)
+      if (L.var == 2) {
+        L.scratch_off = soff;
+        soff += mi::wg_scratch_doubles(L.cls, L.nblocks);
+      }
+    P->scratch_doubles = std::max(off, soff);
+  }
   P->nlanes = std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
   {
     std::vector<double> load(P->nlanes, 0.0);
@@ -624,7 +660,8 @@ static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opt
       (rc = upload<PairGeom>(ctx, &P->d_geom, nullptr, (size_t)npair)) ||
       (rc = upload<PrimPair>(ctx, &P->d_pp, nullptr, (size_t)P->npp)) ||
       (rc = upload(ctx, &P->d_pp_pair, pp_pair.data(), pp_pair.size())) ||
-      (rc = upload(ctx, &P->d_tasks, all_tasks.data(), all_tasks.size()))) {
+      (rc = upload(ctx, &P->d_tasks, all_tasks.data(), all_tasks.size())) ||
+      (P->scratch_doubles && (rc = upload<double>(ctx, &P->d_scratch, nullptr, P->scratch_doubles)))) {
     free_plan_buffers(P);
     delete P;
     return rc;
@@ -668,13 +705,16 @@ struct JKArgs {
   double* K;
 };
 
-static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const TaskDesc* d_tasks, hipStream_t ls) {
-  a.tasks = d_tasks + L.task_off;
+static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P, hipStream_t ls) {
+  a.tasks = P->d_tasks + L.task_off;
   a.ntasks = L.ntasks;
-  if (L.reg)
+  if (L.var == 1)
     HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
-  else
+  else if (L.var == 0)
     HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
+  else
+    HIP_TRY(ctx, mi::launch_class_wave_global(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables,
+                                              P->d_scratch + L.scratch_off, ls));
   return MI_OK;
 }
 
@@ -709,7 +749,7 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
     EriArgs sa = a;
     sa.mode = MI_MODE_SCHWARZ;
     for (const Launch& L : P->slaunches) {
-      int rc = launch_one(ctx, L, sa, P->d_tasks, s);
+      int rc = launch_one(ctx, L, sa, P, s);
       if (rc) return rc;
     }
   }
@@ -738,7 +778,7 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
   }
   for (size_t i = 0; i < P->launches.size(); ++i) {
     const Launch& L = P->launches[i];
-    int rc = launch_one(ctx, L, a, P->d_tasks, fork ? lanes[L.lane] : s);
+    int rc = launch_one(ctx, L, a, P, fork ? lanes[L.lane] : s);
     if (rc) return rc;
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
   }
@@ -764,8 +804,9 @@ int mi_ctx_set_tuning(mi_ctx* ctx, const int32_t* rows, int32_t nrows) {
     if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET)
       return set_err(ctx, MI_EINVAL, "tuning row %d: bad class/bucket", r);
     if (G < 1 || G > 64 || (G & (G - 1))) return set_err(ctx, MI_EINVAL, "tuning row %d: G=%d", r, G);
-    if ((var != 0 && var != 1) || (var == 1 && !mi::has_register_kernel(cls)) ||
-        (var == 0 && !mi::wclass_fits_nw(cls, G, ctx->lds_limit)))
+    if ((var < 0 || var > 2) || (var == 1 && !mi::has_register_kernel(cls)) ||
+        (var == 0 && !mi::wclass_fits_nw(cls, G, ctx->lds_limit)) ||
+        (var == 2 && !mi::wgclass_fits_nw(cls, G, ctx->lds_limit)))
       return set_err(ctx, MI_EINVAL, "tuning row %d: variant %d (G=%d) unavailable for class %d", r, var, G, cls);
     ctx->tune[tune_index(cls, bb, kb)] = {var, G};
   }
@@ -866,8 +907,8 @@ int mi_plan_pair_bounds(const mi_plan* P, double* Q, double* V, int32_t* mol) {
 
 int mi_plan_nlaunch(const mi_plan* P) { return P ? (int)P->launches.size() : -1; }
 
-int mi_plan_launch_info(const mi_plan* P, int32_t i, int32_t* cls4, int64_t* counts3) {
-  if (!P || i < 0 || i >= (int)P->launches.size() || !cls4 || !counts3) return MI_EINVAL;
+int mi_plan_launch_info(const mi_plan* P, int32_t i, int32_t* cls4, int64_t* counts3, int32_t* kernel2) {
+  if (!P || i < 0 || i >= (int)P->launches.size() || !cls4 || !counts3 || !kernel2) return MI_EINVAL;
   const Launch& L = P->launches[i];
   const WClassDesc& D = mi::wclass(L.cls);
   cls4[0] = D.la;
@@ -877,16 +918,18 @@ int mi_plan_launch_info(const mi_plan* P, int32_t i, int32_t* cls4, int64_t* cou
   counts3[0] = L.nq_shell;
   counts3[1] = L.nq_prim;
   counts3[2] = L.nblocks;
-  return L.reg ? 1 : 0;
+  kernel2[0] = L.var;
+  kernel2[1] = L.nw;
+  return MI_OK;
 }
 
-int mi_plan_profile(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms) {
+static int profile_run(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms, const JKArgs* jk) {
   if (!P || !ms) return MI_EINVAL;
   mi_ctx* ctx = P->ctx;
   const size_t nev = P->launches.size() + 2;
   std::vector<hipEvent_t> ev(nev, nullptr);
   for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
-  int rc = plan_run(P, out, out_len, stream, ev.data());
+  int rc = plan_run(P, out, out_len, stream, ev.data(), jk);
   if (rc == MI_OK) {
     hipError_t e = hipEventSynchronize(ev[nev - 1]);
     if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "event sync failed: %s", hipGetErrorString(e));
@@ -897,6 +940,18 @@ int mi_plan_profile(mi_plan* P, double* out, int64_t out_len, void* stream, floa
   }
   for (auto& e : ev) hipEventDestroy(e);
   return rc;
+}
+
+int mi_plan_profile(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms) {
+  return profile_run(P, out, out_len, stream, ms, nullptr);
+}
+
+int mi_plan_jk_profile(mi_plan* P, const double* D, double* J, double* K, void* stream, float* ms) {
+  if (!P) return MI_EINVAL;
+  if (P->nmol != 1) return set_err(P->ctx, MI_EINVAL, "Coulomb/exchange plans hold one molecule");
+  if (!D || !J || !K) return set_err(P->ctx, MI_EINVAL, "NULL density or output");
+  JKArgs jk{D, J, K};
+  return profile_run(P, nullptr, 0, stream, ms, &jk);
 }
 
 void mi_plan_destroy(mi_plan* P) {

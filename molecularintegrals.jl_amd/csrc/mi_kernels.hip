@@ -3,10 +3,12 @@
 // Per execute:
 //   1. pair_prims_kernel   one lane per primitive pair: P, zeta, PA, folded
 //                          prefactor K, 1/(2 zeta); lane 0 of a pair writes AB.
-//   2. per quartet class   either a generated register kernel (hgp_r_<class>,
-//                          lane group per shell quartet) or the table-driven
-//                          wave-cooperative kernel hgp_w_kernel (one wavefront
-//                          per shell quartet, VRR/HRR staged in LDS).
+//   2. per quartet class   a generated register kernel (hgp_r_<class>, lane
+//                          group per shell quartet), the table-driven wave
+//                          kernel hgp_w_kernel (1-4 wavefronts per shell
+//                          quartet, tables and VRR/HRR in LDS), or, for the
+//                          largest f classes, hgp_wg_kernel (a block per shell
+//                          quartet, VRR in LDS, HRR in global scratch).
 // MFMA is not used: HGP's VRR/HRR are short dependent FMA chains with
 // per-quartet coefficients, not dense contractions.
 #include <algorithm>
@@ -88,6 +90,35 @@ __device__ __forceinline__ double sel3(int i, double x, double y, double z) { re
 
 __device__ __forceinline__ int ncart_dev(int l) { return (l + 1) * (l + 2) / 2; }
 
+// F_m(T) for one order m <= mtop: grid Taylor value of the top order, then
+// downward recursion to m (T < 30); asymptotic form with upward recursion
+// otherwise (src/Boys.jl:19-40).  btab is the grid of the top order mtop.
+__device__ __forceinline__ double boys_lane(const double* __restrict__ btab, double T2, int mtop, int m) {
+  double fm;
+  if (T2 < BOYS_TMAX) {
+    const int k = (int)(T2 * 16.0 + 0.5);
+    const double d = (double)k * 0.0625 - T2;
+    const double* r = btab + k * 8;
+    double f = r[6];
+    f = fma(f, d, r[5]);
+    f = fma(f, d, r[4]);
+    f = fma(f, d, r[3]);
+    f = fma(f, d, r[2]);
+    f = fma(f, d, r[1]);
+    f = fma(f, d, r[0]);
+    const double eT = r[7] * exp_small(d);
+    const double t2 = T2 + T2;
+    fm = f;
+    for (int j = mtop - 1; j >= m; --j) fm = fma(t2, fm, eT) * kInvOdd[j];
+  } else {
+    const double rT = 1.0 / T2;
+    fm = 0.88622692545275801365 * sqrt(rT);
+    const double o2T = 0.5 * rT;
+    for (int j = 1; j <= m; ++j) fm *= (2 * j - 1) * o2T;
+  }
+  return fm;
+}
+
 // LDS layout of a wave-kernel block: the class's op tables (staged once per
 // block, then read at LDS latency), followed by the per-quartet work areas.
 __host__ __device__ __forceinline__ int w_desc_doubles(const WClassDesc& D) { return D.bwords / 2; }
@@ -156,31 +187,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
       const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
       const double pref = bp.K * kp.K * sqrt(inv_s);
       // Boys: lane m (< vt0) evaluates the top order and recurses down to m.
-      if (lane < D.vt0) {
-        double fm;
-        if (T2 < BOYS_TMAX) {
-          const int k = (int)(T2 * 16.0 + 0.5);
-          const double d = (double)k * 0.0625 - T2;
-          const double* r = btab + k * 8;
-          double f = r[6];
-          f = fma(f, d, r[5]);
-          f = fma(f, d, r[4]);
-          f = fma(f, d, r[3]);
-          f = fma(f, d, r[2]);
-          f = fma(f, d, r[1]);
-          f = fma(f, d, r[0]);
-          const double eT = r[7] * exp_small(d);
-          const double t2 = T2 + T2;
-          fm = f;
-          for (int m = mtop - 1; m >= lane; --m) fm = fma(t2, fm, eT) * kInvOdd[m];
-        } else {
-          const double rT = 1.0 / T2;
-          fm = 0.88622692545275801365 * sqrt(rT);
-          const double o2T = 0.5 * rT;
-          for (int m = 1; m <= lane; ++m) fm *= (2 * m - 1) * o2T;
-        }
-        vrr[lane] = pref * fm;
-      }
+      if (lane < D.vt0) vrr[lane] = pref * boys_lane(btab, T2, mtop, lane);
       __syncwarp();
       const double oo2z = bp.oo2z, oo2e = kp.oo2z, oo2s = 0.5 * inv_s;
       const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
@@ -242,12 +249,160 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
       __syncthreads();
     }
     const ScatterCtx sc = make_scatter(args, qc, P, Q);
+    if (args.mode == MI_MODE_JK) {  // uniform per block
+      // scaled block into the (now free) VRR areas, then six contractions
+      for (int x = tid; x < D.nout; x += nthr) {
+        const uint32_t o = sO[x];
+        const int ma = (o >> 16) & 15, mb = (o >> 20) & 15, mc = (o >> 24) & 15, md = (o >> 28) & 15;
+        vrr_all[x] = hrr[o & 0xFFFF] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
+      }
+      __syncthreads();
+      jk_block_lds(args, sc, vrr_all, na, nb, nc, nd, tid, nthr);
+      __syncthreads();
+      continue;
+    }
     for (int x = tid; x < D.nout; x += nthr) {
       const uint32_t o = sO[x];
       const int off = o & 0xFFFF;
       const int ma = (o >> 16) & 15, mb = (o >> 20) & 15, mc = (o >> 24) & 15, md = (o >> 28) & 15;
       scatter_elem(args, sc, qc.bra, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md,
                    hrr[off] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]));
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Global-table block kernel for the classes whose contracted block or HRR
+// work area does not fit the wave kernel (f classes with nacc > 1024 or more
+// than 65535 HRR slots).  All threads of a block cooperate on one primitive
+// quartet at a time: the VRR work area of the class is in LDS (nvrr <= 17273
+// doubles for (ff|ff)), the op tables are read from global memory with 32-bit
+// slot fields, and the contracted (e0|f0) block plus the HRR levels live in a
+// per-block slice of a global scratch buffer (nhrr doubles per block; each
+// accumulator slot is owned by one thread across the primitive loop).  Blocks
+// are persistent over the launch's quartets like the wave kernel.
+// LDS: [scales: 4*W_SCL][vrr: nvrr]
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const WClassDesc D,
+                                                     const uint32_t* __restrict__ gblob,
+                                                     double* __restrict__ scratch, int64_t nq_launch) {
+  extern __shared__ double lds[];
+  double* scl = lds;
+  double* vrr = lds + 4 * W_SCL;
+  const uint4* gV = reinterpret_cast<const uint4*>(gblob + D.gv);
+  const uint4* gH = reinterpret_cast<const uint4*>(gblob + D.gh);
+  const uint2_t* gO = reinterpret_cast<const uint2_t*>(gblob + D.go);
+  const uint2_t* gA = reinterpret_cast<const uint2_t*>(gblob + D.ga);
+  double* hrr = scratch + (size_t)blockIdx.x * D.nhrr;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
+  const int mtop = D.mmax - 1;
+  const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
+  int ti = 0;
+  for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
+    while (ti + 1 < args.ntasks && args.tasks[ti + 1].blockStart <= q) ++ti;
+    const TaskDesc T = args.tasks[ti];
+    QuartetCtx qc;
+    decode_index(T, q - T.blockStart, qc);
+    qc.G = 1;
+    qc.g = 0;
+    const int qmode = quartet_mode(args, qc);  // FP64 VRR throughout; 2 = screened (uniform per block)
+    const PairTopo P = args.topo[qc.bra];
+    const PairTopo Q = args.topo[qc.ket];
+    if (qmode == 2) {
+      if (args.mode == MI_MODE_UNIQUE) {
+        const ScatterCtx sc = make_scatter(args, qc, P, Q);
+        for (int x = tid; x < D.nout; x += nthr) {
+          const uint2_t o = gO[x];
+          scatter_elem(args, sc, qc.bra, sc.fa0 + (o.y & 15), sc.fb0 + ((o.y >> 4) & 15),
+                       sc.fc0 + ((o.y >> 8) & 15), sc.fd0 + ((o.y >> 12) & 15), 0.0);
+        }
+      }
+      continue;
+    }
+    if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
+    if (tid < nb) scl[W_SCL + tid] = args.bf_scale[P.bfB + tid];
+    if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
+    if (tid < nd) scl[3 * W_SCL + tid] = args.bf_scale[Q.bfB + tid];
+    const int K = P.npp * Q.npp;
+    for (int t = 0; t < K; ++t) {
+      const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
+      const PrimPair bp = args.pp[P.ppoff + u];
+      const PrimPair kp = args.pp[Q.ppoff + v];
+      const double zeta = bp.zeta, eta = kp.zeta;
+      const double inv_s = 1.0 / (zeta + eta);
+      const double PQx = bp.Px - kp.Px, PQy = bp.Py - kp.Py, PQz = bp.Pz - kp.Pz;
+      const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = bp.K * kp.K * sqrt(inv_s);
+      if (tid < D.vt0) vrr[tid] = pref * boys_lane(btab, T2, mtop, tid);
+      __syncthreads();
+      const double oo2z = bp.oo2z, oo2e = kp.oo2z, oo2s = 0.5 * inv_s;
+      const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
+      const double WPx = -eta_s * PQx, WPy = -eta_s * PQy, WPz = -eta_s * PQz;
+      const double WQx = zeta_s * PQx, WQy = zeta_s * PQy, WQz = zeta_s * PQz;
+      for (int lv = 0; lv < D.nvlev; ++lv) {
+        const int i1 = D.vlev[lv + 1];
+        for (int it = D.vlev[lv] + tid; it < i1; it += nthr) {
+          const uint4 o = gV[it];
+          const int s1 = o.x, s2 = o.y, s3 = o.z;
+          const int dir = o.w & 3;
+          const double n1 = (double)((o.w >> 2) & 7), n2 = (double)((o.w >> 5) & 7);
+          const bool cb = (o.w >> 8) & 1;
+          const double X1 = cb ? sel3(dir, kp.PAx, kp.PAy, kp.PAz) : sel3(dir, bp.PAx, bp.PAy, bp.PAz);
+          const double X2 = cb ? sel3(dir, WQx, WQy, WQz) : sel3(dir, WPx, WPy, WPz);
+          const double c3 = cb ? n1 * oo2s : 0.0;
+          const double c2 = cb ? n2 * oo2e : n1 * oo2z;
+          const double r2 = cb ? zeta_s : eta_s;
+          double val = X1 * vrr[s1] + X2 * vrr[s1 + 1];
+          val += c3 * vrr[s3 + 1];
+          val += c2 * (vrr[s2] - r2 * vrr[s2 + 1]);
+          vrr[D.vt0 + it] = val;
+        }
+        __syncthreads();
+      }
+      // contraction: slot x is always added by the same thread
+      for (int x = tid; x < D.nacc; x += nthr) {
+        const double v0 = vrr[gA[x].x];
+        hrr[x] = t == 0 ? v0 : hrr[x] + v0;
+      }
+      __syncthreads();  // the next primitive overwrites vrr
+    }
+    if (args.mode == MI_MODE_SCHWARZ)
+      for (int x = tid; x < D.nacc; x += nthr)
+        if (gA[x].y) record_vmax(&args.geom[qc.bra], hrr[x]);
+    // HRR levels: item k writes slot nacc + k.  __syncthreads orders the
+    // block's global-memory writes and reads (workgroup scope, one CU).
+    const PairGeom PG = args.geom[qc.bra];
+    const PairGeom QG = args.geom[qc.ket];
+    for (int lv = 0; lv < D.nhlev; ++lv) {
+      const int i1 = D.hlev[lv + 1];
+      for (int it = D.hlev[lv] + tid; it < i1; it += nthr) {
+        const uint4 o = gH[it];
+        const int j = o.z & 3;
+        const bool ket = (o.z >> 2) & 1;
+        const double X = ket ? sel3(j, QG.ABx, QG.ABy, QG.ABz) : sel3(j, PG.ABx, PG.ABy, PG.ABz);
+        hrr[D.nacc + it] = hrr[o.x] + X * hrr[o.y];
+      }
+      __syncthreads();
+    }
+    const ScatterCtx sc = make_scatter(args, qc, P, Q);
+    if (args.mode == MI_MODE_JK) {  // uniform per block
+      for (int x = tid; x < D.nout; x += nthr) {
+        const uint2_t o = gO[x];
+        const int ma = o.y & 15, mb = (o.y >> 4) & 15, mc = (o.y >> 8) & 15, md = (o.y >> 12) & 15;
+        vrr[x] = hrr[o.x] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
+      }
+      __syncthreads();
+      jk_block_lds(args, sc, vrr, na, nb, nc, nd, tid, nthr);
+      __syncthreads();
+      continue;
+    }
+    for (int x = tid; x < D.nout; x += nthr) {
+      const uint2_t o = gO[x];
+      const int ma = o.y & 15, mb = (o.y >> 4) & 15, mc = (o.y >> 8) & 15, md = (o.y >> 12) & 15;
+      scatter_elem(args, sc, qc.bra, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md,
+                   hrr[o.x] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]));
     }
     __syncthreads();
   }
@@ -287,7 +442,8 @@ const WClassDesc& wclass(int c) { return kWClassDesc[c]; }
 bool has_register_kernel(int c) { return kRegisterKernels[c] != nullptr; }
 
 struct WTablesHost {
-  uint32_t* blob = nullptr;
+  uint32_t* blob = nullptr;   // kWBlob (LDS wave kernel)
+  uint32_t* gblob = nullptr;  // kWGBlob (global-table kernel)
 };
 
 hipError_t upload_wtables(void** handle) {
@@ -295,6 +451,8 @@ hipError_t upload_wtables(void** handle) {
   hipError_t e;
   if ((e = hipMalloc(&t->blob, sizeof(kWBlob))) != hipSuccess) return e;
   if ((e = hipMemcpy(t->blob, kWBlob, sizeof(kWBlob), hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMalloc(&t->gblob, sizeof(kWGBlob))) != hipSuccess) return e;
+  if ((e = hipMemcpy(t->gblob, kWGBlob, sizeof(kWGBlob), hipMemcpyHostToDevice)) != hipSuccess) return e;
   *handle = t;
   return hipSuccess;
 }
@@ -303,6 +461,7 @@ void free_wtables(void* handle) {
   WTablesHost* t = static_cast<WTablesHost*>(handle);
   if (!t) return;
   hipFree(t->blob);
+  hipFree(t->gblob);
   delete t;
 }
 
@@ -335,7 +494,7 @@ int wave_kernel_blocks_per_cu(int c, int nw) {
   return nb > 0 ? nb : 1;
 }
 
-bool wclass_fits(int c) { return kWClassDesc[c].nacc <= 64 * W_ACC_MAX; }
+bool wclass_fits(int c) { return kWClassDesc[c].boff >= 0 && kWClassDesc[c].nacc <= 64 * W_ACC_MAX; }
 
 bool wclass_fits_nw(int c, int nw, size_t lds_limit) {
   return wclass_fits(c) && nw >= 1 && nw <= 4 && wclass_lds_bytes(c, nw) <= lds_limit;
@@ -348,6 +507,31 @@ hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, in
   const WClassDesc D = kWClassDesc[c];
   const size_t lds = wclass_lds_bytes(c, nw);
   hipLaunchKernelGGL(hgp_w_kernel, dim3(nblocks), dim3(64 * nw), lds, stream, args, D, W, nq_launch);
+  return hipGetLastError();
+}
+
+size_t wgclass_lds_bytes(int c) { return sizeof(double) * (size_t)(4 * W_SCL + kWClassDesc[c].nvrr); }
+
+bool wgclass_fits_nw(int c, int nw, size_t lds_limit) {
+  return (nw == 1 || nw == 2 || nw == 4 || nw == 8) && wgclass_lds_bytes(c) <= lds_limit;
+}
+
+int wg_kernel_blocks_per_cu(int c, int nw) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)hgp_wg_kernel, 64 * nw,
+                                                   wgclass_lds_bytes(c)) != hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
+size_t wg_scratch_doubles(int c, int nblocks) { return (size_t)nblocks * kWClassDesc[c].nhrr; }
+
+hipError_t launch_class_wave_global(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch,
+                                    void* wtables, double* scratch, hipStream_t stream) {
+  const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
+  const WClassDesc D = kWClassDesc[c];
+  hipLaunchKernelGGL(hgp_wg_kernel, dim3(nblocks), dim3(64 * nw), wgclass_lds_bytes(c), stream, args, D, t->gblob,
+                     scratch, nq_launch);
   return hipGetLastError();
 }
 
@@ -366,7 +550,9 @@ hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream) {
 }
 
 hipError_t set_wave_kernel_lds_limit(size_t bytes) {
-  return hipFuncSetAttribute((const void*)hgp_w_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = hipFuncSetAttribute((const void*)hgp_w_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)hgp_wg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 }  // namespace mi

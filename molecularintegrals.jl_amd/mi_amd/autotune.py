@@ -63,7 +63,7 @@ def class_times(ctx, fb, rows, reps):
     prof = np.array([plan.profile(out.data_ptr(), sptr) for _ in range(reps)])
     med = np.median(prof, axis=0)
     per = {}
-    for i, (cls, nqs, nqp, nblk, reg) in enumerate(launches):
+    for i, (cls, nqs, nqp, nblk, kern) in enumerate(launches):
         per[cls] = per.get(cls, 0.0) + float(med[i + 1])
     plan.close()
     return per

@@ -152,14 +152,18 @@ class Plan:
 
     def launches(self) -> list:
         """Per class launch: ((la,lb,lc,ld), shell quartets, primitive quartets,
-        blocks, register-kernel flag)."""
+        blocks, kernel), kernel = "R" (generated register kernel), "W<nw>" (LDS
+        wave kernel, nw wavefronts per quartet) or "G<nw>" (global-table kernel,
+        nw wavefronts per block)."""
         out = []
         n = self.ctx.L.mi_plan_nlaunch(self.h)
         for i in range(n):
             c4 = (ctypes.c_int32 * 4)()
             n3 = (ctypes.c_int64 * 3)()
-            r = self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3)
-            out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), bool(r == 1)))
+            k2 = (ctypes.c_int32 * 2)()
+            self.ctx.check(self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3, k2))
+            kern = "R" if k2[0] == 1 else ("W" if k2[0] == 0 else "G") + str(k2[1])
+            out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), kern))
         return out
 
     def profile(self, out_ptr: int, stream: int = 0) -> list:
@@ -168,6 +172,14 @@ class Plan:
         ms = (ctypes.c_float * n)()
         self.ctx.check(self.ctx.L.mi_plan_profile(self.h, ctypes.c_void_p(out_ptr), self.n_unique,
                                                   ctypes.c_void_p(stream), ms))
+        return [float(x) for x in ms]
+
+    def jk_profile(self, D_ptr: int, J_ptr: int, K_ptr: int, stream: int = 0) -> list:
+        """Per-launch durations (ms) of one Coulomb/exchange execute."""
+        n = self.ctx.L.mi_plan_nlaunch(self.h) + 1
+        ms = (ctypes.c_float * n)()
+        self.ctx.check(self.ctx.L.mi_plan_jk_profile(self.h, ctypes.c_void_p(D_ptr), ctypes.c_void_p(J_ptr),
+                                                     ctypes.c_void_p(K_ptr), ctypes.c_void_p(stream), ms))
         return [float(x) for x in ms]
 
     def close(self):

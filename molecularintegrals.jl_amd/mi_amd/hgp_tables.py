@@ -11,7 +11,7 @@ from __future__ import annotations
 import sys
 from collections import OrderedDict
 
-LMAX_SHELL = 2           # shells up to d (f classes need the global-scratch W path)
+LMAX_SHELL = 3           # shells up to f
 LMAX_TAB = 2 * 3 + 1     # AO tables cover HRR intermediates of f shells too
 
 # ---------------------------------------------------------------- AO tables

@@ -236,6 +236,24 @@ def test_hgp_equals_tho_on_random_bases(seed):
         assert np.max(np.abs(h - t)) <= 1e-13 * max(1.0, np.max(np.abs(t)))
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_hgp_equals_tho_with_f_shells(seed):
+    """f shells (la+lb up to 6, the range the reference's nao table cannot
+    reach): HGP VRR/HRR against the independent THO expansion."""
+    from mi_amd import flat_from_shells
+    rng = np.random.Generator(np.random.PCG64(40 + seed))
+    cents = rng.uniform(-1.5, 1.5, size=(3, 3))
+    spec = []
+    for L in (3, 3, 1):
+        k = int(rng.integers(1, 3))
+        ex = np.exp(rng.uniform(np.log(0.2), np.log(8.0), size=k))
+        spec.append((cents[int(rng.integers(0, 3))], L, ex.tolist(), rng.uniform(0.2, 1, size=k).tolist()))
+    fb = flat_from_shells(spec)
+    h = O.eri_unique_hgp(fb, order=O.ORDER_CHRR)
+    t = O.eri_unique_tho(fb, order=O.ORDER_CHRR)
+    assert np.max(np.abs(h - t)) <= 2e-12 * max(1.0, np.max(np.abs(t)))
+
+
 def test_hgp_equals_tho_ethane_minimal():
     from mi_amd import configs
     fb = configs.ethane_minimal(0).flat()
