@@ -31,6 +31,11 @@ sys.path.insert(0, os.path.join(ROOT, "molecularintegrals.jl_amd"))
 METRIC = "unique ERIs/s, 1 GPU (plain vs batched+mixed @1e-10 Eh); p50/p99 latency"
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of a kernel symbol on the default workload, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
+# a 256 MiB device copy gives); profiles/hbm_traffic_ethane_split_valence_60bf.md
+# holds the commands and the per-kernel table.
+MEASURED_TRAFFIC_BYTES = {"hgp_w_kernel": 1230314}
 
 
 def log(*a):
@@ -118,7 +123,11 @@ def main():
     value = n / (ms_per_step * 1e-3)
     log(f"[bench] {ms_per_step * 1e3:.1f} us/step, {value:.3e} unique ERIs/s")
 
-    # Per-launch durations (HIP events between launches, same stream).
+    # Per-launch durations (HIP events between launches, same stream).  The
+    # roofline is taken for the dominant kernel SYMBOL: hgp_r_<class> is one
+    # class, hgp_w_kernel / hgp_wg_kernel serve several classes, so their time
+    # and FLOPs are summed over the launches of one step and the mean launch
+    # duration is total / launches (what rocprofv3 --stats reports per symbol).
     launches = plan.launches()
     nprof = max(5, min(args.steps, 50))
     acc = np.zeros(len(launches) + 1)
@@ -127,17 +136,21 @@ def main():
     mean_ms = acc / nprof
     per = []
     step_flops = 0
+    sym = {}
     for i, (cls, nqs, nqp, nblk, kern) in enumerate(launches):
         f = flops.class_flops(cls, nqs, nqp)
         step_flops += f
         per.append((mean_ms[i + 1], cls, f, nqs, nqp, kern))
-    dom = max(per, key=lambda x: x[0])
-    dom_ms, dom_cls, dom_flops = dom[0], dom[1], dom[2]
-    name = {"R": "hgp_r_", "W": "hgp_w_kernel:", "G": "hgp_wg_kernel:"}[dom[5][0]] + "".join("spdf"[x] for x in dom_cls)
-    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    for ms_, cls, f, nqs, nqp, kern in sorted(per, reverse=True)[:8]:
+        name = {"R": "hgp_r_" + "".join("spdf"[x] for x in cls), "W": "hgp_w_kernel", "G": "hgp_wg_kernel"}[kern[0]]
+        t, fl, nl = sym.get(name, (0.0, 0, 0))
+        sym[name] = (t + mean_ms[i + 1], fl + f, nl + 1)
+    name, (dom_ms_total, dom_flops, dom_n) = max(sym.items(), key=lambda kv: kv[1][0])
+    achieved = dom_flops / (dom_ms_total * 1e-3) / 1e12
+    for ms_, cls, f, nqs, nqp, kern in sorted(per, key=lambda x: -x[0])[:8]:
         log(f"[bench]   {''.join('spdf'[x] for x in cls)} {kern} {ms_ * 1e3:8.2f} us "
             f"{f / (ms_ * 1e-3) / 1e12:7.3f} TFLOP/s  quartets={nqs} prim={nqp}")
+    for nm, (t, fl, nl) in sorted(sym.items(), key=lambda kv: -kv[1][0])[:5]:
+        log(f"[bench]   symbol {nm}: {nl} launch(es), {t * 1e3:.2f} us/step, {fl / (t * 1e-3) / 1e12:.3f} TFLOP/s")
     log(f"[bench]   pair kernel {mean_ms[0] * 1e3:.2f} us; sum of launches {mean_ms.sum() * 1e3:.1f} us")
 
     # parity spot check of the timed output against the committed fixture
@@ -164,8 +177,10 @@ def main():
                    "primitive_quartets": plan.stats()["primitive_quartets"], "precision": "fp64",
                    "order": "chrr"},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
-                     "kernel": name, "kernel_mean_us": dom_ms * 1e3, "kernel_algorithmic_flop": dom_flops,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC_BYTES.get(name) if args.seed == 0 else None,
+                     "kernel": name, "kernel_launches_per_step": dom_n,
+                     "kernel_mean_us": dom_ms_total * 1e3 / dom_n, "kernel_algorithmic_flop": dom_flops / dom_n,
                      "step_algorithmic_flop": step_flops,
                      "step_achieved_tflops": step_flops / (ms_per_step * 1e-3) / 1e12,
                      "step_output_write_gbs": n * 8 / (ms_per_step * 1e-3) / 1e9},
