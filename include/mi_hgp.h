@@ -169,6 +169,55 @@ int mi_plan_profile(mi_plan *plan, double *out_device, int64_t out_len, void *st
 int mi_plan_jk_profile(mi_plan *plan, const double *D_device, double *J_device, double *K_device, void *stream,
                        float *ms);
 
+/* ---------------------------------------------------------------------------
+ * Serving engine (no counterpart in the reference, whose drivers evaluate one
+ * Basis per blocking call; callers: request servers that take ERI or J/K
+ * requests for molecules of different sizes as they arrive).
+ *
+ * Requests are submitted with caller-owned DEVICE buffers (out_device of
+ * mi_n_unique(nbf) doubles for MI_REQ_ERI; D, J, K of nbf*nbf for MI_REQ_JK)
+ * and queued in arrival order; the basis is copied at submit.  mi_server_step
+ * is the engine and never blocks: it retires finished batches (event query),
+ * then admits the head of the queue while fewer than max_inflight batches are
+ * on the device and the request's device bytes (plan, HRR scratch, output)
+ * fit hbm_budget -- the head waits for memory otherwise.  ERI requests with
+ * equal options at the head are merged (up to max_batch and the budget) into
+ * one batched plan; a J/K request is a batch of its own.  Each batch runs on
+ * its own stream, so requests admitted while others run overlap with them.
+ * Device memory is allocated and freed stream-ordered on the batch stream.
+ * A server and its requests belong to one host thread.
+ * ------------------------------------------------------------------------- */
+typedef struct mi_server mi_server;
+typedef struct {
+  int32_t max_inflight; /* batches on the device at once, 1..8 */
+  int32_t max_batch;    /* ERI requests merged into one batch */
+  int64_t hbm_budget;   /* device bytes the engine may hold for batches */
+} mi_server_opts;
+#define MI_REQ_ERI 0
+#define MI_REQ_JK 1
+/* request states (mi_server_poll) */
+#define MI_REQ_QUEUED 0
+#define MI_REQ_RUNNING 1
+#define MI_REQ_DONE 2
+#define MI_REQ_FAILED 3
+int mi_server_create(int32_t device, const mi_server_opts *opts, mi_server **out);
+void mi_server_destroy(mi_server *server);
+const char *mi_server_last_error(const mi_server *server);
+/* opts: order, precision, tau, schwarz_cut as for mi_eri_unique / mi_jk
+ * (opts->stream is ignored).  *id receives the request id. */
+int mi_server_submit(mi_server *server, const mi_basis *basis, const mi_opts *opts, int32_t kind,
+                     const double *D_device, double *out_device, double *J_device, double *K_device, int64_t *id);
+int mi_server_step(mi_server *server);
+int mi_server_poll(mi_server *server, int64_t id, int32_t *state);
+/* Forget a finished request (its id becomes unknown). */
+int mi_server_release(mi_server *server, int64_t id);
+/* Run the engine until the queue is empty and nothing is in flight
+ * (blocking); MI_EDEVICE if any request failed. */
+int mi_server_drain(mi_server *server);
+/* st[0..6] = queued, running, done, failed, batches launched, device bytes
+ * in use, peak device bytes. */
+int mi_server_stats(const mi_server *server, int64_t *stats, int32_t nstats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,7 @@ def build(verbose: bool = True) -> str:
     run([sys.executable, os.path.join(CSRC, "gen_hgp.py"), "--out", gen_dir])
     objs = []
     jobs = []
-    for src in ("mi_kernels.hip", "mi_host.cpp"):
+    for src in ("mi_kernels.hip", "mi_host.cpp", "mi_server.cpp"):
         obj = os.path.join(gen_dir, src + ".o")
         cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):

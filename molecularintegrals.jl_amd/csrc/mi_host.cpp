@@ -23,35 +23,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mi_hgp.h"
-#include "mi_kernels.h"
+#include "mi_internal.h"
 
-#define MI_NBUCKET 5
-#define MI_MAX_STREAMS 4
-
-// Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
-// generated register kernel with G lanes per quartet, 0 = LDS wave kernel with
-// G wavefronts per quartet, 2 = global-table kernel with G wavefronts per block.
-struct TuneEntry {
-  int variant = -1;  // -1: not tuned, use the heuristic
-  int G = 1;
-};
-
-struct mi_ctx {
-  int device = 0;
-  double* d_boys = nullptr;
-  void* wtables = nullptr;
-  std::string err;
-  size_t lds_limit = 0;
-  int ncu = 256;
-  int nstreams = MI_MAX_STREAMS;              // class launches spread over this many streams
-  hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
-  std::vector<TuneEntry> tune;                // [class][bra bucket][ket bucket]
-};
-
-namespace {
-
-int set_err(mi_ctx* c, int code, const char* fmt, ...) {
+int mi_set_err(mi_ctx* c, int code, const char* fmt, ...) {
   if (c) {
     char buf[512];
     va_list ap;
@@ -62,6 +36,10 @@ int set_err(mi_ctx* c, int code, const char* fmt, ...) {
   }
   return code;
 }
+
+namespace {
+
+#define set_err mi_set_err
 
 #define HIP_TRY(ctx, expr)                                                                   \
   do {                                                                                       \
@@ -160,48 +138,6 @@ void load_tuning_file(mi_ctx* c) {
 
 }  // namespace
 
-struct Launch {
-  int cls;
-  int var;  // 1 register kernel, 0 LDS wave kernel, 2 global-table kernel
-  int nw;   // wave kernels: wavefronts per block
-  size_t scratch_off;  // global-table kernel: offset of its slice of the plan scratch (doubles)
-  int task_off, ntasks, nblocks;
-  int64_t nq_shell, nq_prim;
-  double cost;  // relative device time estimate (longest launches go first)
-  int lane;     // stream lane: 0 = caller's stream, 1.. = context aux streams
-};
-
-struct mi_plan {
-  mi_ctx* ctx = nullptr;
-  int nshell = 0, nbf = 0, npair = 0, npp = 0;
-  int64_t nq_shell = 0, nq_prim = 0, n_unique = 0;
-  int order = MI_ORDER_CHRR;
-  int nmol = 1;
-  std::vector<int64_t> mol_off;  // output offset of each molecule, [nmol+1]
-  // device buffers
-  double* d_shell_xyz = nullptr;
-  int32_t* d_shell_prim_off = nullptr;
-  double* d_prim_exp = nullptr;
-  double* d_prim_cn = nullptr;
-  double* d_bf_scale = nullptr;
-  PairTopo* d_topo = nullptr;
-  PairGeom* d_geom = nullptr;
-  PrimPair* d_pp = nullptr;
-  int32_t* d_pp_pair = nullptr;
-  TaskDesc* d_tasks = nullptr;
-  double* d_scratch = nullptr;     // HRR work areas of the global-table launches
-  size_t scratch_doubles = 0;
-  std::vector<Launch> launches;   // in issue order (cost descending)
-  std::vector<Launch> slaunches;  // Schwarz pass (diagonal quartets), run before the main launches
-  bool need_schwarz = false;
-  std::vector<int32_t> pair_mol;  // molecule of each pair record
-  double tau2 = 0.0, cut2 = 0.0;
-  int nlanes = 1;
-  hipGraphExec_t gexec = nullptr;  // captured execute (mi_plan_graph_capture)
-  hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[MI_MAX_STREAMS] = {};
-};
-
 namespace {
 
 int validate(mi_ctx* ctx, const mi_basis* b) {
@@ -228,11 +164,22 @@ int validate(mi_ctx* ctx, const mi_basis* b) {
   return MI_OK;
 }
 
+// Device buffer of n elements, filled from src when given.  With a plan
+// alloc_stream the allocation and copy are ordered on that stream (src is
+// host data the plan keeps until it is destroyed).
 template <typename T>
-int upload(mi_ctx* ctx, T** dst, const T* src, size_t n) {
+int upload(mi_plan* P, T** dst, const T* src, size_t n) {
+  mi_ctx* ctx = P->ctx;
   if (n == 0) n = 1;
-  HIP_TRY(ctx, hipMalloc((void**)dst, sizeof(T) * n));
-  if (src) HIP_TRY(ctx, hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  const size_t bytes = sizeof(T) * n;
+  if (P->alloc_stream) {
+    HIP_TRY(ctx, hipMallocAsync((void**)dst, bytes, P->alloc_stream));
+    if (src) HIP_TRY(ctx, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, P->alloc_stream));
+  } else {
+    HIP_TRY(ctx, hipMalloc((void**)dst, bytes));
+    if (src) HIP_TRY(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  }
+  P->device_bytes += bytes;
   return MI_OK;
 }
 
@@ -241,17 +188,15 @@ void free_plan_buffers(mi_plan* p) {
   if (p->ev_fork) hipEventDestroy(p->ev_fork);
   for (auto& e : p->ev_join)
     if (e) hipEventDestroy(e);
-  hipFree(p->d_shell_xyz);
-  hipFree(p->d_shell_prim_off);
-  hipFree(p->d_prim_exp);
-  hipFree(p->d_prim_cn);
-  hipFree(p->d_bf_scale);
-  hipFree(p->d_topo);
-  hipFree(p->d_geom);
-  hipFree(p->d_pp);
-  hipFree(p->d_pp_pair);
-  hipFree(p->d_tasks);
-  hipFree(p->d_scratch);
+  void* bufs[] = {p->d_shell_xyz, p->d_shell_prim_off, p->d_prim_exp, p->d_prim_cn, p->d_bf_scale, p->d_topo,
+                  p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch};
+  for (void* b : bufs) {
+    if (!b) continue;
+    if (p->alloc_stream)
+      hipFreeAsync(b, p->alloc_stream);
+    else
+      hipFree(b);
+  }
 }
 
 }  // namespace
@@ -412,7 +357,28 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
   return MI_OK;
 }
 
-static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, mi_plan** out) {
+int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
+  int rc;
+  const mi_plan::HostData& h = P->h;
+  P->alloc_stream = alloc_stream;
+  P->device_bytes = 0;
+  if ((rc = upload(P, &P->d_shell_xyz, h.xyz.data(), h.xyz.size())) ||
+      (rc = upload(P, &P->d_shell_prim_off, h.poff.data(), h.poff.size())) ||
+      (rc = upload(P, &P->d_prim_exp, h.pexp.data(), h.pexp.size())) ||
+      (rc = upload(P, &P->d_prim_cn, h.pcn.data(), h.pcn.size())) ||
+      (rc = upload(P, &P->d_bf_scale, h.bfs.data(), h.bfs.size())) ||
+      (rc = upload(P, &P->d_topo, h.topo.data(), h.topo.size())) ||
+      (rc = upload<PairGeom>(P, &P->d_geom, nullptr, (size_t)P->npair)) ||
+      (rc = upload<PrimPair>(P, &P->d_pp, nullptr, (size_t)P->npp)) ||
+      (rc = upload(P, &P->d_pp_pair, h.pp_pair.data(), h.pp_pair.size())) ||
+      (rc = upload(P, &P->d_tasks, h.tasks.data(), h.tasks.size())) ||
+      (P->scratch_doubles && (rc = upload<double>(P, &P->d_scratch, nullptr, P->scratch_doubles))))
+    return rc;
+  return MI_OK;
+}
+
+int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, bool defer_upload,
+                  mi_plan** out) {
   if (!ctx || !out || !bases || nmol <= 0) return set_err(ctx, MI_EINVAL, "bad plan arguments");
   *out = nullptr;
   int rc;
@@ -649,19 +615,25 @@ static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opt
       load[best] += L.cost;
     }
   }
-  // Device buffers.
-  const int nprim = poff[ns];
-  if ((rc = upload(ctx, &P->d_shell_xyz, xyz.data(), (size_t)3 * ns)) ||
-      (rc = upload(ctx, &P->d_shell_prim_off, poff.data(), (size_t)ns + 1)) ||
-      (rc = upload(ctx, &P->d_prim_exp, pexp.data(), (size_t)nprim)) ||
-      (rc = upload(ctx, &P->d_prim_cn, pcn.data(), (size_t)nprim)) ||
-      (rc = upload(ctx, &P->d_bf_scale, bfs.data(), bfs.size())) ||
-      (rc = upload(ctx, &P->d_topo, topo.data(), (size_t)npair)) ||
-      (rc = upload<PairGeom>(ctx, &P->d_geom, nullptr, (size_t)npair)) ||
-      (rc = upload<PrimPair>(ctx, &P->d_pp, nullptr, (size_t)P->npp)) ||
-      (rc = upload(ctx, &P->d_pp_pair, pp_pair.data(), pp_pair.size())) ||
-      (rc = upload(ctx, &P->d_tasks, all_tasks.data(), all_tasks.size())) ||
-      (P->scratch_doubles && (rc = upload<double>(ctx, &P->d_scratch, nullptr, P->scratch_doubles)))) {
+  // Host images of the device buffers; uploaded now, or by the caller of a
+  // deferred build (the serving engine admits a plan by its device bytes).
+  {
+    mi_plan::HostData& h = P->h;
+    h.xyz = std::move(xyz);
+    h.poff = std::move(poff);
+    h.pexp = std::move(pexp);
+    h.pcn = std::move(pcn);
+    h.bfs = std::move(bfs);
+    h.topo = std::move(topo);
+    h.pp_pair = std::move(pp_pair);
+    h.tasks = std::move(all_tasks);
+    auto nb = [](size_t n, size_t sz) { return sz * std::max<size_t>(n, 1); };
+    P->device_bytes = nb(h.xyz.size(), 8) + nb(h.poff.size(), 4) + nb(h.pexp.size(), 8) + nb(h.pcn.size(), 8) +
+                      nb(h.bfs.size(), 8) + nb(h.topo.size(), sizeof(PairTopo)) + nb(npair, sizeof(PairGeom)) +
+                      nb(P->npp, sizeof(PrimPair)) + nb(h.pp_pair.size(), 4) + nb(h.tasks.size(), sizeof(TaskDesc)) +
+                      (P->scratch_doubles ? 8 * P->scratch_doubles : 0);
+  }
+  if (!defer_upload && (rc = mi_plan_upload(P, nullptr))) {
     free_plan_buffers(P);
     delete P;
     return rc;
@@ -683,11 +655,11 @@ static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opt
 extern "C" {
 
 int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan** out) {
-  return plan_build(ctx, b, 1, opts, out);
+  return mi_plan_build(ctx, b, 1, opts, false, out);
 }
 
 int mi_batch_plan_create(mi_ctx* ctx, const mi_basis* bases, int32_t nmol, const mi_opts* opts, mi_plan** out) {
-  return plan_build(ctx, bases, nmol, opts, out);
+  return mi_plan_build(ctx, bases, nmol, opts, false, out);
 }
 
 int64_t mi_plan_out_len(const mi_plan* P) { return P ? P->n_unique : -1; }
@@ -699,11 +671,9 @@ int mi_plan_mol_offsets(const mi_plan* P, int64_t* offsets) {
 }
 
 
-struct JKArgs {
-  const double* D;
-  double* J;
-  double* K;
-};
+
+
+}  // extern "C"
 
 static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P, hipStream_t ls) {
   a.tasks = P->d_tasks + L.task_off;
@@ -722,8 +692,7 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
 // needs bounds, then the class launches (forked over the plan's stream lanes
 // unless ev != NULL, which times each launch on one stream).  jk != NULL
 // accumulates Coulomb/exchange instead of writing the unique array.
-static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev,
-                    const JKArgs* jk = nullptr) {
+int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev, const JKArgs* jk) {
   if (!P) return MI_EINVAL;
   mi_ctx* ctx = P->ctx;
   if (!jk) {
@@ -791,8 +760,10 @@ static int plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipE
   return MI_OK;
 }
 
+extern "C" {
+
 int mi_plan_execute(mi_plan* P, double* out, int64_t out_len, void* stream) {
-  return plan_run(P, out, out_len, stream, nullptr);
+  return mi_plan_run(P, out, out_len, stream, nullptr, nullptr);
 }
 
 int mi_ctx_set_tuning(mi_ctx* ctx, const int32_t* rows, int32_t nrows) {
@@ -831,7 +802,7 @@ int mi_plan_graph_capture(mi_plan* P, double* out, int64_t out_len) {
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   int rc = MI_OK;
   if (e == hipSuccess) {
-    rc = plan_run(P, out, out_len, cs, nullptr);
+    rc = mi_plan_run(P, out, out_len, cs, nullptr, nullptr);
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (rc == MI_OK && e2 != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "capture failed: %s", hipGetErrorString(e2));
   } else {
@@ -860,7 +831,7 @@ int mi_plan_jk_execute(mi_plan* P, const double* D, double* J, double* K, void* 
   if (P->nmol != 1) return set_err(P->ctx, MI_EINVAL, "Coulomb/exchange plans hold one molecule");
   if (!D || !J || !K) return set_err(P->ctx, MI_EINVAL, "NULL density or output");
   JKArgs jk{D, J, K};
-  return plan_run(P, nullptr, 0, stream, nullptr, &jk);
+  return mi_plan_run(P, nullptr, 0, stream, nullptr, &jk);
 }
 
 int mi_jk(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, const double* D, double* J, double* K) {
@@ -929,7 +900,7 @@ static int profile_run(mi_plan* P, double* out, int64_t out_len, void* stream, f
   const size_t nev = P->launches.size() + 2;
   std::vector<hipEvent_t> ev(nev, nullptr);
   for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
-  int rc = plan_run(P, out, out_len, stream, ev.data(), jk);
+  int rc = mi_plan_run(P, out, out_len, stream, ev.data(), jk);
   if (rc == MI_OK) {
     hipError_t e = hipEventSynchronize(ev[nev - 1]);
     if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "event sync failed: %s", hipGetErrorString(e));

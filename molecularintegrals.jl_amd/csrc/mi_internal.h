@@ -1,0 +1,108 @@
+// mi_internal.h -- library-internal types shared by mi_host.cpp (plans, C
+// ABI) and mi_server.cpp (serving engine).  Not installed; the ABI is
+// include/mi_hgp.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mi_hgp.h"
+#include "mi_kernels.h"
+
+#define MI_NBUCKET 5
+#define MI_MAX_STREAMS 4
+
+// Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
+// generated register kernel with G lanes per quartet, 0 = LDS wave kernel with
+// G wavefronts per quartet, 2 = global-table kernel with G wavefronts per block.
+struct TuneEntry {
+  int variant = -1;  // -1: not tuned, use the heuristic
+  int G = 1;
+};
+
+struct mi_ctx {
+  int device = 0;
+  double* d_boys = nullptr;
+  void* wtables = nullptr;
+  std::string err;
+  size_t lds_limit = 0;
+  int ncu = 256;
+  int nstreams = MI_MAX_STREAMS;              // class launches spread over this many streams
+  hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
+  std::vector<TuneEntry> tune;                // [class][bra bucket][ket bucket]
+};
+
+struct Launch {
+  int cls;
+  int var;  // 1 register kernel, 0 LDS wave kernel, 2 global-table kernel
+  int nw;   // wave kernels: wavefronts per block
+  size_t scratch_off;  // global-table kernel: offset of its slice of the plan scratch (doubles)
+  int task_off, ntasks, nblocks;
+  int64_t nq_shell, nq_prim;
+  double cost;  // relative device time estimate (longest launches go first)
+  int lane;     // stream lane: 0 = caller's stream, 1.. = context aux streams
+};
+
+struct mi_plan {
+  mi_ctx* ctx = nullptr;
+  int nshell = 0, nbf = 0, npair = 0, npp = 0;
+  int64_t nq_shell = 0, nq_prim = 0, n_unique = 0;
+  int order = MI_ORDER_CHRR;
+  int nmol = 1;
+  std::vector<int64_t> mol_off;  // output offset of each molecule, [nmol+1]
+  // device buffers
+  double* d_shell_xyz = nullptr;
+  int32_t* d_shell_prim_off = nullptr;
+  double* d_prim_exp = nullptr;
+  double* d_prim_cn = nullptr;
+  double* d_bf_scale = nullptr;
+  PairTopo* d_topo = nullptr;
+  PairGeom* d_geom = nullptr;
+  PrimPair* d_pp = nullptr;
+  int32_t* d_pp_pair = nullptr;
+  TaskDesc* d_tasks = nullptr;
+  double* d_scratch = nullptr;     // HRR work areas of the global-table launches
+  size_t scratch_doubles = 0;
+  std::vector<Launch> launches;   // in issue order (cost descending)
+  std::vector<Launch> slaunches;  // Schwarz pass (diagonal quartets), run before the main launches
+  bool need_schwarz = false;
+  std::vector<int32_t> pair_mol;  // molecule of each pair record
+  double tau2 = 0.0, cut2 = 0.0;
+  int nlanes = 1;
+  hipGraphExec_t gexec = nullptr;  // captured execute (mi_plan_graph_capture)
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[MI_MAX_STREAMS] = {};
+  // Stream-ordered allocation (serving): device buffers come from
+  // hipMallocAsync on this stream and go back with hipFreeAsync, so building
+  // and retiring a plan never waits for work on other streams.  The host
+  // copies of uploaded data live as long as the plan.
+  hipStream_t alloc_stream = nullptr;
+  size_t device_bytes = 0;  // bytes of device memory the plan holds (or needs, before upload)
+  struct HostData {         // host images of the uploaded buffers
+    std::vector<double> xyz, pexp, pcn, bfs;
+    std::vector<int32_t> poff, pp_pair;
+    std::vector<PairTopo> topo;
+    std::vector<TaskDesc> tasks;
+  } h;
+};
+
+struct JKArgs {
+  const double* D;
+  double* J;
+  double* K;
+};
+
+// Error text into the context; returns code.
+int mi_set_err(mi_ctx* c, int code, const char* fmt, ...);
+
+// Plan over nmol molecules.  defer_upload: compute everything on the host
+// (P->device_bytes is then what mi_plan_upload will allocate) and leave the
+// device buffers to mi_plan_upload; alloc_stream != nullptr there selects
+// stream-ordered allocation (hipMallocAsync / hipFreeAsync) on that stream.
+int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, bool defer_upload,
+                  mi_plan** out);
+int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream);
+// One execute on stream (ev: per-launch timing events or nullptr; jk: the
+// Coulomb/exchange arrays or nullptr for the unique array).
+int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev, const JKArgs* jk);

@@ -22,7 +22,12 @@ EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique"
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
             "mi_plan_launch_info", "mi_plan_profile", "mi_plan_jk_profile", "mi_ctx_set_tuning", "mi_ctx_set_streams",
             "mi_batch_plan_create", "mi_plan_out_len", "mi_plan_mol_offsets", "mi_plan_graph_capture",
-            "mi_plan_graph_launch", "mi_jk", "mi_plan_jk_execute", "mi_plan_pair_bounds"]
+            "mi_plan_graph_launch", "mi_jk", "mi_plan_jk_execute", "mi_plan_pair_bounds",
+            "mi_server_create", "mi_server_destroy", "mi_server_last_error", "mi_server_submit", "mi_server_step",
+            "mi_server_poll", "mi_server_release", "mi_server_drain", "mi_server_stats"]
+
+MI_REQ_ERI, MI_REQ_JK = 0, 1
+MI_REQ_QUEUED, MI_REQ_RUNNING, MI_REQ_DONE, MI_REQ_FAILED = 0, 1, 2, 3
 
 
 class MiBasis(ctypes.Structure):
@@ -43,6 +48,12 @@ class MiOpts(ctypes.Structure):
                 ("tau", ctypes.c_double),
                 ("schwarz_cut", ctypes.c_double),
                 ("stream", ctypes.c_void_p)]
+
+
+class MiServerOpts(ctypes.Structure):
+    _fields_ = [("max_inflight", ctypes.c_int32),
+                ("max_batch", ctypes.c_int32),
+                ("hbm_budget", ctypes.c_int64)]
 
 
 class MiError(RuntimeError):
@@ -124,6 +135,25 @@ def load():
     L.mi_plan_jk_execute.restype = ctypes.c_int
     L.mi_plan_pair_bounds.argtypes = [vp, dpp, dpp, ctypes.POINTER(ctypes.c_int32)]
     L.mi_plan_pair_bounds.restype = ctypes.c_int
+    L.mi_server_create.argtypes = [ctypes.c_int32, ctypes.POINTER(MiServerOpts), ctypes.POINTER(vp)]
+    L.mi_server_create.restype = ctypes.c_int
+    L.mi_server_destroy.argtypes = [vp]
+    L.mi_server_destroy.restype = None
+    L.mi_server_last_error.argtypes = [vp]
+    L.mi_server_last_error.restype = ctypes.c_char_p
+    L.mi_server_submit.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts), ctypes.c_int32, vp, vp, vp,
+                                   vp, ctypes.POINTER(ctypes.c_int64)]
+    L.mi_server_submit.restype = ctypes.c_int
+    L.mi_server_step.argtypes = [vp]
+    L.mi_server_step.restype = ctypes.c_int
+    L.mi_server_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]
+    L.mi_server_poll.restype = ctypes.c_int
+    L.mi_server_release.argtypes = [vp, ctypes.c_int64]
+    L.mi_server_release.restype = ctypes.c_int
+    L.mi_server_drain.argtypes = [vp]
+    L.mi_server_drain.restype = ctypes.c_int
+    L.mi_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
+    L.mi_server_stats.restype = ctypes.c_int
     _lib = L
     return L
 
