@@ -8,6 +8,16 @@ device-resident buffer.  Inputs (basis topology and primitives) are resident
 in HBM before the timed region.
 
 Prints ONE JSON line.  Extra keys:
+  legs          the other workloads, each timed the same way (HIP events,
+                inputs resident on the device):
+                  batched_molecules   512 molecules of 3-24 atoms: plain
+                                      per-molecule plans vs one batched plan
+                                      replayed as a graph, FP64 and mixed
+                  jk_cluster_40atoms  direct J/K, 920 functions with f shells:
+                                      plain FP64 vs screened vs mixed+screened
+                  serving             seeded Poisson schedule through the
+                                      continuous-batching server: p50/p99
+                                      latency and throughput, FP64 and mixed
   roofline      dominant kernel: algorithmic FP64 FLOP per launch (FLOP model
                 of mi_amd/flops.py) / its mean HIP-event duration, against the
                 FP64 vector peak (the recurrences do not use MFMA).
@@ -69,6 +79,156 @@ def cpu_baseline(fb, n_unique, budget_s=12.0):
             "single_thread_value": n_unique / float(np.median(t1))}
 
 
+def timed(fn, reps, stream):
+    """Mean ms of fn() over reps calls, HIP events on stream (one warm-up)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def leg_batched(nmol, seed=0):
+    """Many molecules of uneven size (3-24 atoms, split-valence recipe):
+    the plain path (one plan per molecule, executed one after the other) vs
+    one batched plan replayed as a captured graph, FP64 and mixed (FP32 VRR
+    below tau = 1e-8).  Errors: mixed vs FP64 over every integral (per
+    molecule, on the device); FP64 vs the CPU oracle on the three smallest
+    molecules."""
+    import torch
+    from mi_amd import Context, configs
+    from oracle import oracle as O
+
+    mols = configs.batch_molecules(seed, nmol=512)[:nmol]
+    fbs = [m.flat() for m in mols]
+    ctx = Context(0)
+    st = torch.cuda.current_stream()
+    sp = st.cuda_stream
+    t = time.time()
+    bplan = ctx.batch_plan(fbs)
+    t_plan = time.time() - t
+    n = bplan.n_unique
+    off = bplan.mol_offsets()
+    log(f"[bench] batched: {nmol} molecules, {n:.3e} unique ERIs ({n * 8 / 1e9:.1f} GB), plan {t_plan:.2f} s")
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    bplan.capture(out.data_ptr())
+    ms_fp64 = timed(lambda: bplan.replay(sp), 2, st)
+    # plain: per-molecule plans, eager execute into the same slices
+    plans = [ctx.plan(fb) for fb in fbs]
+    base = out.data_ptr()
+    def plain():
+        for m, p in enumerate(plans):
+            p.execute(base + 8 * int(off[m]), sp)
+    ms_plain = timed(plain, 1, st)
+    # CPU oracle spot check of the FP64 results
+    small = sorted(range(nmol), key=lambda m: fbs[m].nbf)[:3]
+    err_oracle = 0.0
+    for m in small:
+        ref = O.eri_unique_hgp(fbs[m], order=O.ORDER_CHRR, nthreads=8)
+        got = out[int(off[m]):int(off[m + 1])].cpu().numpy()
+        err_oracle = max(err_oracle, float(np.abs(got - ref).max()))
+    # mixed
+    mplan = ctx.batch_plan(fbs, precision=1, tau=1e-8)
+    mplan.capture(out.data_ptr())
+    ms_mixed = timed(lambda: mplan.replay(sp), 2, st)
+    scratch = torch.empty(int(max(off[m + 1] - off[m] for m in range(nmol))), dtype=torch.float64, device="cuda")
+    err_mixed = 0.0
+    for m, p in enumerate(plans):
+        k = int(off[m + 1] - off[m])
+        p.execute(scratch.data_ptr(), sp)
+        err_mixed = max(err_mixed, float((out[int(off[m]):int(off[m]) + k] - scratch[:k]).abs().max()))
+    for p in plans:
+        p.close()
+    bplan.close()
+    mplan.close()
+    del out, scratch
+    torch.cuda.empty_cache()
+    ctx.close()
+    r = {"molecules": nmol, "unique_eris": int(n), "output_gb": n * 8 / 1e9,
+         "plain_per_molecule": {"ms": ms_plain, "value": n / (ms_plain * 1e-3)},
+         "batched_fp64_graph": {"ms": ms_fp64, "value": n / (ms_fp64 * 1e-3)},
+         "batched_mixed_graph": {"ms": ms_mixed, "value": n / (ms_mixed * 1e-3), "tau": 1e-8},
+         "speedup_batched_mixed_vs_plain": ms_plain / ms_mixed,
+         "max_abs_err_mixed_vs_fp64": err_mixed,
+         "max_abs_err_fp64_vs_cpu_oracle_3_smallest": err_oracle}
+    log(f"[bench] batched: plain {ms_plain:.1f} ms, batched fp64 {ms_fp64:.1f} ms, mixed {ms_mixed:.1f} ms, "
+        f"speedup {ms_plain / ms_mixed:.2f}x, err mixed {err_mixed:.2e}, err vs oracle {err_oracle:.2e}")
+    return r
+
+
+def leg_jk(seed=0):
+    """Direct J/K on the 40-atom cluster (920 functions, l <= 3): plain FP64
+    (no screening) vs screening at 1e-12 vs mixed (FP32 VRR below 1e-8) with
+    screening; errors of J and K against the plain FP64 GPU result."""
+    import torch
+    from mi_amd import Context, configs
+
+    wl, D = configs.cluster_40(seed)
+    fb = wl.flat()
+    n = fb.nbf
+    nu = n * (n + 1) * (n * n + n + 2) // 8
+    ctx = Context(0)
+    st = torch.cuda.current_stream()
+    sp = st.cuda_stream
+    dD = torch.from_numpy(D).cuda()
+    J = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    K = torch.empty_like(J)
+    res, mats = {}, {}
+    for name, kw, reps in (("plain_fp64", {}, 1), ("fp64_screened", dict(schwarz_cut=1e-12), 2),
+                           ("mixed_screened", dict(precision=1, tau=1e-8, schwarz_cut=1e-12), 2)):
+        plan = ctx.plan(fb, order=0, **kw)
+        ms = timed(lambda: plan.jk_execute(dD.data_ptr(), J.data_ptr(), K.data_ptr(), sp), reps, st)
+        mats[name] = (J.cpu().numpy(), K.cpu().numpy())
+        res[name] = {"ms": ms, "value": nu / (ms * 1e-3)}
+        plan.close()
+        log(f"[bench] J/K {name}: {ms:.1f} ms, {nu / (ms * 1e-3):.3e} unique ERIs/s (never stored)")
+    J0, K0 = mats["plain_fp64"]
+    for name in ("fp64_screened", "mixed_screened"):
+        res[name]["max_abs_err_J"] = float(np.abs(mats[name][0] - J0).max())
+        res[name]["max_abs_err_K"] = float(np.abs(mats[name][1] - K0).max())
+    ctx.close()
+    res.update({"nbf": n, "unique_eris": nu, "screening": 1e-12,
+                "speedup_mixed_screened_vs_plain": res["plain_fp64"]["ms"] / res["mixed_screened"]["ms"]})
+    return res
+
+
+def leg_serving(nreq, rate, seed=0):
+    """Poisson request schedule (2-60 atoms, split-valence or cluster recipe
+    50/50; unique ERIs up to 120 functions, J/K beyond) replayed in real time
+    through the continuous-batching server; plain FP64 and mixed form."""
+    from mi_amd import configs
+    from mi_amd.serving import serve
+
+    reqs = configs.serving_requests(seed=seed, nreq=nreq, rate_per_s=rate)
+    out = {"requests": nreq, "rate_per_s": rate, "eri_requests": sum(r.kind == "eri" for r in reqs),
+           "jk_requests": sum(r.kind == "jk" for r in reqs),
+           "nbf_range": [min(r.workload.nbf for r in reqs), max(r.workload.nbf for r in reqs)]}
+    res = {}
+    for name, mixed in (("mixed", True), ("plain_fp64", False)):
+        r = serve(reqs, mixed=mixed)
+        res[name] = r
+        out[name] = {"value": r.throughput, "unit": "unique ERIs/s", "p50_ms": r.p(0.5) * 1e3,
+                     "p99_ms": r.p(0.99) * 1e3, "makespan_s": r.makespan_s, "batches": r.stats["batches"],
+                     "peak_device_gb": r.stats["peak_bytes"] / 1e9}
+        log(f"[bench] serving {name}: p50 {r.p(0.5) * 1e3:.1f} ms p99 {r.p(0.99) * 1e3:.1f} ms "
+            f"{r.throughput:.3e} unique ERIs/s, makespan {r.makespan_s:.2f} s")
+    err = 0.0
+    for q in reqs:
+        a = [t.cpu().numpy() for t in res["plain_fp64"].outputs[q.rid]]
+        b = [t.cpu().numpy() for t in res["mixed"].outputs[q.rid]]
+        k = 0 if q.kind == "eri" else 1
+        err = max(err, max(float(np.abs(x - y).max()) for x, y in zip(a[k:], b[k:])))
+    out["mixed"]["max_abs_err_vs_fp64"] = err
+    del res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +237,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="only the headline workload")
+    ap.add_argument("--batch-mols", type=int, default=512)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -189,6 +353,21 @@ def main():
     if not args.no_cpu:
         log("[bench] timing CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(fb, n, args.cpu_budget_s)
+    plan.close()
+    del out
+    torch.cuda.empty_cache()
+    if not args.no_legs:
+        legs = {}
+        for name, fn in (("batched_molecules", lambda: leg_batched(args.batch_mols)),
+                         ("jk_cluster_40atoms", leg_jk),
+                         ("serving", lambda: leg_serving(args.serve_requests, args.serve_rate))):
+            try:
+                legs[name] = fn()
+            except Exception as e:  # report, keep the headline line
+                legs[name] = {"error": f"{type(e).__name__}: {e}"}
+                log(f"[bench] leg {name} failed: {e}")
+            torch.cuda.empty_cache()
+        result["legs"] = legs
     print(json.dumps(result), flush=True)
 
 
