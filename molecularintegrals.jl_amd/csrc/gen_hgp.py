@@ -123,10 +123,10 @@ def emit_rkernel(c):
     w(f"  const PairTopo P = args.topo[qc.bra];")
     w(f"  const PairTopo Q = args.topo[qc.ket];")
     w(f"  const int qmode = quartet_mode(args, qc);  // 0 FP64 VRR, 1 FP32 VRR, 2 screened")
-    w(f"  if (qmode == 2 && args.mode != MI_MODE_UNIQUE) return;")
+    w("  if (qmode == 2) return;  // screened: output pre-zeroed")
     for (e, f) in g.targets:
         w(f"  double acc{e}_{f} = 0.0;")
-    w(f"  const int nu = P.npp, nv = qmode == 2 ? 0 : Q.npp, ntot = nu * nv;")
+    w(f"  const int nu = P.npp, nv = Q.npp, ntot = nu * nv;")
     w(f"  const int G = qc.G;")
     w(f"  int u = qc.g / Q.npp, v = qc.g - (qc.g / Q.npp) * Q.npp;")
     w(f"  const int du = G / Q.npp, dv = G - du * Q.npp;")

@@ -47,7 +47,7 @@ struct TaskDesc {
   int32_t braStart, nBra, ketStart, nKet;
   int32_t same;        // bra list == ket list: triangular quartet space
   int32_t G;           // lanes per quartet (power of two, <= 64)
-  int32_t blockStart;  // first block of this task in its launch
+  int32_t blockStart;  // first shell quartet of this task in its launch (launch-flat quartet index)
   int32_t cls;         // quartet class id
   int32_t variant;     // 1 register kernel, 0 LDS wave kernel
   int32_t bf0;         // first function of the task's molecule (pairs use global indices)
@@ -74,8 +74,8 @@ struct EriArgs {
   int32_t ntasks;
   int32_t mode;        // MI_MODE_*
   int32_t nbf;         // J/K: matrix dimension
-  double tau2;         // FP32 VRR when Qmax_ab * Qmax_cd < tau2 (0: never)
-  double cut2;         // skip when Qmax_ab * Qmax_cd < cut2 (0: never)
+  double tau2;         // FP32 VRR when Vmax_ab * Vmax_cd < tau2 (0: never)
+  double cut2;         // skip when Qmax_ab * Qmax_cd < cut2 (0: never; the output is pre-zeroed)
   const double* D;     // J/K: density, nbf x nbf row-major
   double* Jacc;        // J/K: unsymmetrised accumulators (J = Jacc + Jacc^T)
   double* Kacc;
@@ -196,12 +196,12 @@ __device__ __forceinline__ void stage_boys_slice(double* lds, const double* __re
 // assigned to tasks by blockStart (tasks sorted by it).  A task pairs two
 // contiguous pair lists; same==1 means the triangular space p <= q.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int find_task(const EriArgs& a) {
+// Task holding launch-flat quartet index q (tasks sorted by blockStart).
+__device__ __forceinline__ int find_task(const EriArgs& a, int64_t q) {
   int lo = 0, hi = a.ntasks - 1;
-  const int b = blockIdx.x;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
-    if (a.tasks[mid].blockStart <= b) lo = mid; else hi = mid - 1;
+    if (a.tasks[mid].blockStart <= q) lo = mid; else hi = mid - 1;
   }
   return lo;
 }
@@ -228,12 +228,14 @@ __device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, Quar
   qc.M = T.M;
 }
 
-// Lane-group decode for the register kernels: G lanes per quartet.
+// Lane-group decode for the register kernels: G lanes per quartet (G is the
+// same for every task of a launch); lane groups walk the launch's quartets
+// in order across task boundaries, so blocks stay full whatever the task sizes.
 __device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc) {
-  const TaskDesc T = a.tasks[find_task(a)];
-  const int G = T.G;
-  const int per_block = blockDim.x / G;
-  const int64_t qi = (int64_t)(blockIdx.x - T.blockStart) * per_block + threadIdx.x / G;
+  const int G = a.tasks[0].G;
+  const int64_t gq = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const TaskDesc T = a.tasks[find_task(a, gq)];
+  const int64_t qi = gq - T.blockStart;
   qc.G = G;
   qc.g = threadIdx.x & (G - 1);
   if (qi >= T.nq) return false;

@@ -189,7 +189,7 @@ void free_plan_buffers(mi_plan* p) {
   for (auto& e : p->ev_join)
     if (e) hipEventDestroy(e);
   void* bufs[] = {p->d_shell_xyz, p->d_shell_prim_off, p->d_prim_exp, p->d_prim_cn, p->d_bf_scale, p->d_topo,
-                  p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch};
+                  p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch,   p->d_bounds};
   for (void* b : bufs) {
     if (!b) continue;
     if (p->alloc_stream)
@@ -240,6 +240,14 @@ int mi_create(int device, mi_ctx** out) {
       c->nstreams = k + 1;
       break;
     }
+  if (hipStreamCreateWithFlags(&c->bstream, hipStreamNonBlocking) != hipSuccess) {
+    for (auto& st : c->aux)
+      if (st) hipStreamDestroy(st);
+    hipFree(c->d_boys);
+    mi::free_wtables(c->wtables);
+    delete c;
+    return MI_EDEVICE;
+  }
   hipDeviceProp_t prop;
   c->lds_limit = 65536;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
@@ -259,6 +267,7 @@ void mi_destroy(mi_ctx* ctx) {
   hipSetDevice(ctx->device);
   for (auto& st : ctx->aux)
     if (st) hipStreamDestroy(st);
+  if (ctx->bstream) hipStreamDestroy(ctx->bstream);
   hipFree(ctx->d_boys);
   mi::free_wtables(ctx->wtables);
   delete ctx;
@@ -309,14 +318,15 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
   // One launch per (class, variant); within a launch, tasks with the most
   // work per block come first so the longest blocks start earliest.
   for (int c = 0; c < mi::nclass(); ++c) {
-    // vk 0: register kernel; 1-3: wave kernel with 1, 2, 4 wavefronts;
-    // 4-7: global-table kernel with 1, 2, 4, 8 wavefronts
-    for (int vk = 0; vk < 8; ++vk) {
-      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : 2);
-      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (1 << (vk - 4)));
+    // vk 0-6: register kernel with G = 1, 2, ..., 64 lanes per quartet;
+    // 7-9: wave kernel with 1, 2, 4 wavefronts; 10-13: global-table kernel
+    // with 1, 2, 4, 8 wavefronts
+    for (int vk = 0; vk < 14; ++vk) {
+      const int var = vk < 7 ? 1 : (vk < 10 ? 0 : 2);
+      const int nw = vk < 7 ? (1 << vk) : (vk < 10 ? (1 << (vk - 7)) : (1 << (vk - 10)));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
-        if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
+        if (t.variant == var && t.G == nw) ts.push_back(t);
       if (ts.empty()) continue;
       Launch L{};
       L.cls = c;
@@ -327,13 +337,12 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
       L.ntasks = (int)ts.size();
       int64_t blocks = 0;
       int64_t qs = 0;
-      for (TaskDesc& t : ts) {
-        // register kernel: first block of the task; wave kernel: first quartet
-        t.blockStart = (int32_t)(reg ? blocks : qs);
-        blocks += reg ? (t.nq * t.G + 255) / 256 : 0;
+      for (TaskDesc& t : ts) {  // first launch-flat quartet of the task
+        t.blockStart = (int32_t)qs;
         qs += t.nq;
         all_tasks.push_back(t);
       }
+      if (reg) blocks = (qs * nw + 255) / 256;
       if (var == 0) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
       if (var == 2) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wg_kernel_blocks_per_cu(c, nw));
       if (blocks > INT32_MAX || qs > INT32_MAX) {
@@ -372,21 +381,60 @@ int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
       (rc = upload<PrimPair>(P, &P->d_pp, nullptr, (size_t)P->npp)) ||
       (rc = upload(P, &P->d_pp_pair, h.pp_pair.data(), h.pp_pair.size())) ||
       (rc = upload(P, &P->d_tasks, h.tasks.data(), h.tasks.size())) ||
-      (P->scratch_doubles && (rc = upload<double>(P, &P->d_scratch, nullptr, P->scratch_doubles))))
+      (P->scratch_doubles && (rc = upload<double>(P, &P->d_scratch, nullptr, P->scratch_doubles))) ||
+      (!h.bounds.empty() && (rc = upload(P, &P->d_bounds, h.bounds.data(), h.bounds.size()))))
     return rc;
   return MI_OK;
 }
 
-int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, bool defer_upload,
-                  mi_plan** out) {
+// Schwarz bounds of every shell pair, in the canonical pair order
+// (molecule, pair class, contraction bucket, pair index): a bounds-only plan
+// (diagonal quartets) runs on the context's bounds stream with stream-ordered
+// memory and is read back.  Q = max |(ab|ab)|, V = max |(e0|e0)| (VRR outputs).
+static int compute_pair_bounds(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts,
+                               std::vector<double>& Q, std::vector<double>& V) {
+  mi_plan* B = nullptr;
+  int rc = mi_plan_build(ctx, bases, nmol, opts, MI_BUILD_BOUNDS_ONLY | MI_BUILD_DEFER_UPLOAD, &B);
+  if (rc) return rc;
+  hipStream_t st = ctx->bstream;
+  rc = mi_plan_upload(B, st);
+  if (rc == MI_OK) rc = mi_plan_run(B, nullptr, 0, st, nullptr, nullptr);
+  std::vector<PairGeom> g(B->npair);
+  if (rc == MI_OK) {
+    hipError_t e = hipMemcpyAsync(g.data(), B->d_geom, sizeof(PairGeom) * g.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "Schwarz bounds readback: %s", hipGetErrorString(e));
+  }
+  if (rc == MI_OK) {
+    Q.resize(g.size());
+    V.resize(g.size());
+    for (size_t r = 0; r < g.size(); ++r) {
+      Q[r] = g[r].Q;
+      V[r] = g[r].V;
+    }
+  }
+  mi_plan_destroy(B);
+  return rc;
+}
+
+int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, int flags, mi_plan** out) {
   if (!ctx || !out || !bases || nmol <= 0) return set_err(ctx, MI_EINVAL, "bad plan arguments");
   *out = nullptr;
   int rc;
   for (int m = 0; m < nmol; ++m)
     if ((rc = validate(ctx, &bases[m]))) return rc;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const bool bounds_only = (flags & MI_BUILD_BOUNDS_ONLY) != 0;
+  const double tau2 = (opts && opts->precision == MI_PREC_MIXED) ? opts->tau * opts->tau : 0.0;
+  const double cut2 = (opts && opts->schwarz_cut > 0) ? opts->schwarz_cut * opts->schwarz_cut : 0.0;
+  // Bounds first (a device pass) when the plan screens or mixes precision:
+  // pairs are then ordered by bound and screened quartets never enter a task.
+  std::vector<double> kQ, kV;
+  if (!bounds_only && !(flags & MI_BUILD_NO_BOUNDS) && (tau2 > 0.0 || cut2 > 0.0))
+    if ((rc = compute_pair_bounds(ctx, bases, nmol, opts, kQ, kV))) return rc;
   mi_plan* P = new mi_plan();
   P->ctx = ctx;
+  P->bounds_only = bounds_only;
   P->order = opts ? opts->order : MI_ORDER_CHRR;
   if (P->order != MI_ORDER_CHRR && P->order != MI_ORDER_IINDEX) {
     delete P;
@@ -430,6 +478,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   // Shell pairs (I >= J within a molecule), canonical shell order L_A >= L_B.
   struct PairKey {
     int mol, pc, bucket, pidx, A, B, K;
+    double Q, V;
   };
   std::vector<PairKey> keys;
   for (int m = 0; m < nmol; ++m)
@@ -439,7 +488,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         if (L[J] > L[I]) std::swap(A, B);
         const int K = np[A] * np[B];
         const int i = I - mol_sh0[m], j = J - mol_sh0[m];
-        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K});
+        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0});
       }
   std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
     if (x.mol != y.mol) return x.mol < y.mol;
@@ -447,6 +496,24 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     if (x.bucket != y.bucket) return x.bucket < y.bucket;
     return x.pidx < y.pidx;
   });
+  const bool has_bounds = !kQ.empty();
+  if (has_bounds) {
+    if (kQ.size() != keys.size()) {
+      delete P;
+      return set_err(ctx, MI_EDEVICE, "Schwarz bounds: %zu pairs, plan has %zu", kQ.size(), keys.size());
+    }
+    for (size_t r = 0; r < keys.size(); ++r) {
+      keys[r].Q = kQ[r];
+      keys[r].V = kV[r];
+    }
+    // within each (molecule, class, bucket) list: largest bound first
+    std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
+      if (x.mol != y.mol) return x.mol < y.mol;
+      if (x.pc != y.pc) return x.pc < y.pc;
+      if (x.bucket != y.bucket) return x.bucket < y.bucket;
+      return x.Q > y.Q;
+    });
+  }
   const int npair = (int)keys.size();
   std::vector<PairTopo> topo(npair);
   std::vector<int32_t> pp_pair;
@@ -486,54 +553,91 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     while (x < lists.size() && lists[x].mol < (int)m) ++x;
     mol_list0[m] = (int)x;
   }
-  // Tasks grouped by quartet class.
+  // Tasks grouped by quartet class.  With Schwarz bounds and a cut, a list
+  // pair becomes rectangles (bra rows x leading ket columns) that leave the
+  // screened quartets out: in a list ordered by bound, the ket columns kept
+  // for bra row i are a prefix of length n_i, non-increasing in i.  Rows are
+  // grouped while n_i stays within 80% of the group's first (the few extra
+  // quartets are screened in the kernels).
   std::vector<std::vector<TaskDesc>> per_class(mi::nclass());
-  std::vector<int64_t> cls_shell(mi::nclass(), 0), cls_prim(mi::nclass(), 0);
   int64_t nq_shell = 0, nq_prim = 0;
-  for (int m = 0; m < nmol; ++m)
+  std::vector<int64_t> Kpre(npair + 1, 0);  // prefix sums of primitive pairs in record order
+  for (int r = 0; r < npair; ++r) Kpre[r + 1] = Kpre[r] + keys[r].K;
+  const bool screen_tasks = has_bounds && cut2 > 0.0;
+  for (int m = 0; m < nmol && !bounds_only; ++m)
   for (int x = mol_list0[m]; x < mol_list0[m + 1]; ++x)
     for (int y = x; y < mol_list0[m + 1]; ++y) {
       const List* bra = &lists[x];
       const List* ket = &lists[y];
       if (ket->pc > bra->pc) std::swap(bra, ket);
-      TaskDesc t{};
-      t.braStart = bra->start;
-      t.nBra = bra->n;
-      t.ketStart = ket->start;
-      t.nKet = ket->n;
-      t.same = (x == y);
-      t.cls = quartet_class(bra->pc, ket->pc);
-      t.nq = t.same ? (int64_t)bra->n * (bra->n + 1) / 2 : (int64_t)bra->n * ket->n;
-      t.bf0 = mol_bf0[m];
-      t.obase = P->mol_off[m];
-      {
-        const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
-        t.M = nbm * (nbm + 1) / 2;
-      }
       const double kavg = ((double)bra->Ksum / bra->n) * ((double)ket->Ksum / ket->n);
       const int bb = keys[bra->start].bucket, kb = keys[ket->start].bucket;
-      if ((rc = choose_kernel(ctx, t, bb, kb, kavg))) {
+      // one task: bra records [b0, b0+nb) x ket records [k0, k0+nk), or the
+      // triangle of [b0, b0+nb) with itself (same)
+      auto emit = [&](int b0, int nb, int k0, int nk, int same) -> int {
+        TaskDesc t{};
+        t.braStart = b0;
+        t.nBra = nb;
+        t.ketStart = k0;
+        t.nKet = nk;
+        t.same = same;
+        t.cls = quartet_class(bra->pc, ket->pc);
+        t.nq = same ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
+        t.bf0 = mol_bf0[m];
+        t.obase = P->mol_off[m];
+        const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
+        t.M = nbm * (nbm + 1) / 2;
+        int e = choose_kernel(ctx, t, bb, kb, kavg);
+        if (e) return e;
+        const int64_t sk = Kpre[b0 + nb] - Kpre[b0];
+        if (same) {
+          // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
+          int64_t sk2 = 0;
+          for (int r = b0; r < b0 + nb; ++r) sk2 += (int64_t)keys[r].K * keys[r].K;
+          t.nprim = (sk * sk + sk2) / 2;
+        } else {
+          t.nprim = sk * (Kpre[k0 + nk] - Kpre[k0]);
+        }
+        nq_shell += t.nq;
+        nq_prim += t.nprim;
+        per_class[t.cls].push_back(t);
+        return MI_OK;
+      };
+      if (!screen_tasks) {
+        rc = emit(bra->start, bra->n, ket->start, ket->n, x == y);
+      } else {
+        // n_i: ket columns with Q_i Q_j >= cut2 (kets ordered by Q, largest first)
+        auto kept = [&](int i) -> int {
+          const double qi = keys[bra->start + i].Q;
+          if (!(qi > 0.0)) return 0;
+          const double need = cut2 / qi;
+          int lo = 0, hi = ket->n;  // first j with Q_j < need
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (keys[ket->start + mid].Q >= need) lo = mid + 1; else hi = mid;
+          }
+          return lo;
+        };
+        int row = 0;
+        rc = MI_OK;
+        if (x == y) {
+          // rows whose kept columns include all j <= i form a triangle
+          while (row < bra->n && kept(row) > row) ++row;
+          if (row > 0) rc = emit(bra->start, row, bra->start, row, 1);
+        }
+        while (rc == MI_OK && row < bra->n) {
+          const int n0 = kept(row);
+          if (n0 == 0) break;  // every later row keeps fewer
+          int end = row + 1;
+          while (end < bra->n && end - row < 256 && kept(end) * 5 >= n0 * 4) ++end;
+          rc = emit(bra->start + row, end - row, ket->start, n0, 0);
+          row = end;
+        }
+      }
+      if (rc) {
         delete P;
         return rc;
       }
-      int64_t tprim;
-      if (t.same) {
-        // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
-        int64_t sk = 0, sk2 = 0;
-        for (int r = bra->start; r < bra->start + bra->n; ++r) {
-          sk += keys[r].K;
-          sk2 += (int64_t)keys[r].K * keys[r].K;
-        }
-        tprim = (sk * sk + sk2) / 2;
-      } else {
-        tprim = (int64_t)bra->Ksum * ket->Ksum;
-      }
-      t.nprim = tprim;
-      nq_shell += t.nq;
-      nq_prim += tprim;
-      cls_shell[t.cls] += t.nq;
-      cls_prim[t.cls] += tprim;
-      per_class[t.cls].push_back(t);
     }
   P->nq_shell = nq_shell;
   P->nq_prim = nq_prim;
@@ -543,10 +647,15 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     return rc;
   }
   // Schwarz pass: diagonal quartets (p|p) of every shell pair, needed for
-  // FP32-VRR selection (MI_PREC_MIXED) and for screening (schwarz_cut > 0).
-  P->tau2 = (opts && opts->precision == MI_PREC_MIXED) ? opts->tau * opts->tau : 0.0;
-  P->cut2 = (opts && opts->schwarz_cut > 0) ? opts->schwarz_cut * opts->schwarz_cut : 0.0;
-  P->need_schwarz = P->tau2 > 0.0 || P->cut2 > 0.0;
+  // FP32-VRR selection (MI_PREC_MIXED) and for screening (schwarz_cut > 0)
+  // when the bounds were not computed with the plan (or for them).
+  P->tau2 = tau2;
+  P->cut2 = cut2;
+  P->need_schwarz = bounds_only || ((tau2 > 0.0 || cut2 > 0.0) && !has_bounds);
+  if (has_bounds) {
+    P->h.bounds.resize(npair);
+    for (int r = 0; r < npair; ++r) P->h.bounds[r] = make_double2(keys[r].Q, keys[r].V);
+  }
   if (P->need_schwarz) {
     std::vector<std::vector<TaskDesc>> diag(mi::nclass());
     for (int m = 0; m < nmol; ++m)
@@ -631,9 +740,9 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     P->device_bytes = nb(h.xyz.size(), 8) + nb(h.poff.size(), 4) + nb(h.pexp.size(), 8) + nb(h.pcn.size(), 8) +
                       nb(h.bfs.size(), 8) + nb(h.topo.size(), sizeof(PairTopo)) + nb(npair, sizeof(PairGeom)) +
                       nb(P->npp, sizeof(PrimPair)) + nb(h.pp_pair.size(), 4) + nb(h.tasks.size(), sizeof(TaskDesc)) +
-                      (P->scratch_doubles ? 8 * P->scratch_doubles : 0);
+                      (P->scratch_doubles ? 8 * P->scratch_doubles : 0) + 16 * h.bounds.size();
   }
-  if (!defer_upload && (rc = mi_plan_upload(P, nullptr))) {
+  if (!(flags & MI_BUILD_DEFER_UPLOAD) && (rc = mi_plan_upload(P, nullptr))) {
     free_plan_buffers(P);
     delete P;
     return rc;
@@ -655,11 +764,11 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
 extern "C" {
 
 int mi_plan_create(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, mi_plan** out) {
-  return mi_plan_build(ctx, b, 1, opts, false, out);
+  return mi_plan_build(ctx, b, 1, opts, 0, out);
 }
 
 int mi_batch_plan_create(mi_ctx* ctx, const mi_basis* bases, int32_t nmol, const mi_opts* opts, mi_plan** out) {
-  return mi_plan_build(ctx, bases, nmol, opts, false, out);
+  return mi_plan_build(ctx, bases, nmol, opts, 0, out);
 }
 
 int64_t mi_plan_out_len(const mi_plan* P) { return P ? P->n_unique : -1; }
@@ -695,7 +804,7 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
 int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev, const JKArgs* jk) {
   if (!P) return MI_EINVAL;
   mi_ctx* ctx = P->ctx;
-  if (!jk) {
+  if (!jk && !P->bounds_only) {
     if (!out) return set_err(ctx, MI_EINVAL, "output pointer is NULL");
     if (out_len != P->n_unique)
       return set_err(ctx, MI_ESIZE, "output length %lld, basis needs %lld", (long long)out_len,
@@ -704,7 +813,7 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
   hipStream_t s = (hipStream_t)stream;
   if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], s));
   HIP_TRY(ctx, mi::launch_pair_prims(P->npp, P->d_pp_pair, P->d_topo, P->d_shell_xyz, P->d_shell_prim_off,
-                                     P->d_prim_exp, P->d_prim_cn, P->d_pp, P->d_geom, s));
+                                     P->d_prim_exp, P->d_prim_cn, P->d_pp, P->d_geom, P->d_bounds, s));
   EriArgs a{};
   a.topo = P->d_topo;
   a.geom = P->d_geom;
@@ -733,6 +842,9 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
     const int64_t nn = (int64_t)P->nbf * P->nbf;
     HIP_TRY(ctx, mi::launch_zero(jk->J, nn, s));
     HIP_TRY(ctx, mi::launch_zero(jk->K, nn, s));
+  } else if (P->cut2 > 0.0 && !P->bounds_only) {
+    // screened quartets are never written: their integrals are zero
+    HIP_TRY(ctx, mi::launch_zero(out, P->n_unique, s));
   }
   // Class launches are independent (disjoint output slots): fork them over the
   // plan's stream lanes and join back into the caller's stream.
@@ -864,7 +976,8 @@ int mi_jk(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, const double* D, 
 
 int mi_plan_pair_bounds(const mi_plan* P, double* Q, double* V, int32_t* mol) {
   if (!P || !Q || !V || !mol) return MI_EINVAL;
-  if (!P->need_schwarz) return set_err(P->ctx, MI_EINVAL, "plan has no Schwarz pass (plain FP64 without screening)");
+  if (!P->need_schwarz && !P->d_bounds)
+    return set_err(P->ctx, MI_EINVAL, "plan has no Schwarz bounds (plain FP64 without screening)");
   std::vector<PairGeom> g(P->npair);
   std::vector<PairTopo> t(P->npair);
   HIP_TRY(P->ctx, hipMemcpy(g.data(), P->d_geom, sizeof(PairGeom) * g.size(), hipMemcpyDeviceToHost));

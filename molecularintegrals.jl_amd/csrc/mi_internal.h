@@ -30,6 +30,7 @@ struct mi_ctx {
   int ncu = 256;
   int nstreams = MI_MAX_STREAMS;              // class launches spread over this many streams
   hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
+  hipStream_t bstream = nullptr;              // Schwarz bounds passes at plan build
   std::vector<TuneEntry> tune;                // [class][bra bucket][ket bucket]
 };
 
@@ -78,12 +79,15 @@ struct mi_plan {
   // and retiring a plan never waits for work on other streams.  The host
   // copies of uploaded data live as long as the plan.
   hipStream_t alloc_stream = nullptr;
+  bool bounds_only = false;         // Schwarz pass only (mi_plan_build with MI_BUILD_BOUNDS_ONLY)
+  double2* d_bounds = nullptr;      // per pair (Q, V) computed when the plan was built, or null
   size_t device_bytes = 0;  // bytes of device memory the plan holds (or needs, before upload)
   struct HostData {         // host images of the uploaded buffers
     std::vector<double> xyz, pexp, pcn, bfs;
     std::vector<int32_t> poff, pp_pair;
     std::vector<PairTopo> topo;
     std::vector<TaskDesc> tasks;
+    std::vector<double2> bounds;
   } h;
 };
 
@@ -96,12 +100,21 @@ struct JKArgs {
 // Error text into the context; returns code.
 int mi_set_err(mi_ctx* c, int code, const char* fmt, ...);
 
-// Plan over nmol molecules.  defer_upload: compute everything on the host
-// (P->device_bytes is then what mi_plan_upload will allocate) and leave the
-// device buffers to mi_plan_upload; alloc_stream != nullptr there selects
-// stream-ordered allocation (hipMallocAsync / hipFreeAsync) on that stream.
-int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, bool defer_upload,
-                  mi_plan** out);
+// Plan over nmol molecules.  Flags:
+//   MI_BUILD_DEFER_UPLOAD  compute everything on the host (P->device_bytes is
+//                          then what mi_plan_upload will allocate) and leave
+//                          the device buffers to mi_plan_upload, whose
+//                          alloc_stream != nullptr selects stream-ordered
+//                          allocation (hipMallocAsync / hipFreeAsync);
+//   MI_BUILD_NO_BOUNDS     do not run the Schwarz bounds pass at build time
+//                          (screening and FP32 selection then happen in the
+//                          kernels on bounds from a per-execute pass);
+//   MI_BUILD_BOUNDS_ONLY   a plan of the diagonal quartets alone, whose
+//                          execute fills the per-pair bounds.
+#define MI_BUILD_DEFER_UPLOAD 1
+#define MI_BUILD_NO_BOUNDS 2
+#define MI_BUILD_BOUNDS_ONLY 4
+int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, int flags, mi_plan** out);
 int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream);
 // One execute on stream (ev: per-launch timing events or nullptr; jk: the
 // Coulomb/exchange arrays or nullptr for the unique array).

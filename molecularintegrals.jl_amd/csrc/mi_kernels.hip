@@ -29,7 +29,8 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
                                                          const int32_t* __restrict__ shell_prim_off,
                                                          const double* __restrict__ prim_exp,
                                                          const double* __restrict__ prim_cn,
-                                                         PrimPair* __restrict__ pp, PairGeom* __restrict__ geom) {
+                                                         PrimPair* __restrict__ pp, PairGeom* __restrict__ geom,
+                                                         const double2* __restrict__ bounds) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npp) return;
   const int p = pp_pair[r];
@@ -61,7 +62,10 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
   pp[r] = o;
   if (local == 0) {
     PairGeom g;
-    g.ABx = ABx; g.ABy = ABy; g.ABz = ABz; g.Q = 0.0; g.V = 0.0; g.pad = 0.0;
+    g.ABx = ABx; g.ABy = ABy; g.ABz = ABz; g.pad = 0.0;
+    // Schwarz bounds computed when the plan was built, or zero for a Schwarz pass to fill
+    g.Q = bounds ? bounds[p].x : 0.0;
+    g.V = bounds ? bounds[p].y : 0.0;
     geom[p] = g;
   }
 }
@@ -166,7 +170,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     qc.G = 1;
     qc.g = 0;
     const int qmode = quartet_mode(args, qc);  // the wave kernel keeps FP64 VRR; 2 = screened
-    if (qmode == 2 && args.mode != MI_MODE_UNIQUE) continue;  // uniform per block
+    if (qmode == 2) continue;                   // uniform per block (output pre-zeroed)
     const PairTopo P = args.topo[qc.bra];
     const PairTopo Q = args.topo[qc.ket];
     if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
@@ -176,7 +180,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     double acc[W_ACC_MAX];
 #pragma unroll
     for (int k = 0; k < W_ACC_MAX; ++k) acc[k] = 0.0;
-    const int K = qmode == 2 ? 0 : P.npp * Q.npp;
+    const int K = P.npp * Q.npp;
     for (int t = wv; t < K; t += nw) {
       const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
       const PrimPair bp = args.pp[P.ppoff + u];
@@ -308,19 +312,9 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
     qc.G = 1;
     qc.g = 0;
     const int qmode = quartet_mode(args, qc);  // FP64 VRR throughout; 2 = screened (uniform per block)
+    if (qmode == 2) continue;                   // output pre-zeroed
     const PairTopo P = args.topo[qc.bra];
     const PairTopo Q = args.topo[qc.ket];
-    if (qmode == 2) {
-      if (args.mode == MI_MODE_UNIQUE) {
-        const ScatterCtx sc = make_scatter(args, qc, P, Q);
-        for (int x = tid; x < D.nout; x += nthr) {
-          const uint2_t o = gO[x];
-          scatter_elem(args, sc, qc.bra, sc.fa0 + (o.y & 15), sc.fb0 + ((o.y >> 4) & 15),
-                       sc.fc0 + ((o.y >> 8) & 15), sc.fd0 + ((o.y >> 12) & 15), 0.0);
-        }
-      }
-      continue;
-    }
     if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
     if (tid < nb) scl[W_SCL + tid] = args.bf_scale[P.bfB + tid];
     if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
@@ -467,11 +461,11 @@ void free_wtables(void* handle) {
 
 hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const PairTopo* topo, const double* shell_xyz,
                              const int32_t* shell_prim_off, const double* prim_exp, const double* prim_cn,
-                             PrimPair* pp, PairGeom* geom, hipStream_t stream) {
+                             PrimPair* pp, PairGeom* geom, const double2* bounds, hipStream_t stream) {
   if (npp == 0) return hipSuccess;
   dim3 grid((npp + 255) / 256), block(256);
   hipLaunchKernelGGL(pair_prims_kernel, grid, block, 0, stream, npp, pp_pair, topo, shell_xyz, shell_prim_off,
-                     prim_exp, prim_cn, pp, geom);
+                     prim_exp, prim_cn, pp, geom, bounds);
   return hipGetLastError();
 }
 

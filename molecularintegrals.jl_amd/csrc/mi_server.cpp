@@ -88,7 +88,7 @@ int serr(mi_server* s, int code, const std::string& msg) {
 // the merged-output share.
 int request_bytes(mi_server* s, Request& r) {
   mi_plan* P = nullptr;
-  int rc = mi_plan_build(s->ctx, &r.b, 1, &r.o, true, &P);
+  int rc = mi_plan_build(s->ctx, &r.b, 1, &r.o, MI_BUILD_DEFER_UPLOAD | MI_BUILD_NO_BOUNDS, &P);
   if (rc) return serr(s, rc, s->ctx->err);
   r.n_unique = P->n_unique;
   r.bytes = P->device_bytes + (r.kind == MI_REQ_ERI ? sizeof(double) * (size_t)P->n_unique : 0);
@@ -139,7 +139,7 @@ int launch(mi_server* s, const std::vector<int64_t>& ids, int slot) {
   Batch b;
   b.slot = slot;
   b.reqs = ids;
-  int rc = mi_plan_build(s->ctx, bases.data(), (int)bases.size(), &r0.o, true, &b.plan);
+  int rc = mi_plan_build(s->ctx, bases.data(), (int)bases.size(), &r0.o, MI_BUILD_DEFER_UPLOAD, &b.plan);
   if (rc == MI_OK) rc = mi_plan_upload(b.plan, st);
   if (rc == MI_OK && r0.kind == MI_REQ_ERI) {
     const int64_t len = b.plan->n_unique;
