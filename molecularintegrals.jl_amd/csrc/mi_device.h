@@ -47,7 +47,8 @@ struct TaskDesc {
   int32_t braStart, nBra, ketStart, nKet;
   int32_t same;        // bra list == ket list: triangular quartet space
   int32_t G;           // lanes per quartet (power of two, <= 64)
-  int32_t blockStart;  // first shell quartet of this task in its launch (launch-flat quartet index)
+  int32_t blockStart;  // register kernels: first wavefront of the task in its launch;
+                       // wave kernels: first shell quartet (launch-flat index)
   int32_t cls;         // quartet class id
   int32_t variant;     // 1 register kernel, 0 LDS wave kernel
   int32_t bf0;         // first function of the task's molecule (pairs use global indices)
@@ -206,6 +207,23 @@ __device__ __forceinline__ int find_task(const EriArgs& a, int64_t q) {
   return lo;
 }
 
+// The same, starting from a task ti known to start at or before q: gallop
+// forward, then bisect (cost logarithmic in the distance, not in ntasks).
+__device__ __forceinline__ int advance_task(const EriArgs& a, int ti, int64_t q) {
+  int lo = ti, hi = ti + 1, step = 1;
+  while (hi < a.ntasks && a.tasks[hi].blockStart <= q) {
+    lo = hi;
+    step <<= 1;
+    hi = lo + step;
+  }
+  if (hi > a.ntasks) hi = a.ntasks;
+  while (hi - lo > 1) {  // tasks[lo] <= q < tasks[hi]
+    const int mid = (lo + hi) >> 1;
+    if (a.tasks[mid].blockStart <= q) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, QuartetCtx& qc) {
   if (T.same == 2) {  // Schwarz pass: the diagonal quartets (p|p)
     qc.bra = qc.ket = T.braStart + (int)qi;
@@ -228,16 +246,19 @@ __device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, Quar
   qc.M = T.M;
 }
 
-// Lane-group decode for the register kernels: G lanes per quartet (G is the
-// same for every task of a launch); lane groups walk the launch's quartets
-// in order across task boundaries, so blocks stay full whatever the task sizes.
+// Lane-group decode for the register kernels: G lanes per quartet.  Each task
+// of a launch owns ceil(nq G / 64) whole wavefronts (blockStart = its first
+// wavefront), so a wavefront never spans two tasks, lane groups stay aligned
+// whatever each task's G, and blocks stay full across task boundaries.
 __device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc) {
-  const int G = a.tasks[0].G;
-  const int64_t gq = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  const TaskDesc T = a.tasks[find_task(a, gq)];
-  const int64_t qi = gq - T.blockStart;
+  const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x >> 6);  // block-uniform: scalar search
+  const int64_t w = w0 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const TaskDesc T = a.tasks[advance_task(a, find_task(a, w0), w)];
+  const int G = T.G;
+  const int64_t qi = ((w - T.blockStart) * 64 + lane) / G;
   qc.G = G;
-  qc.g = threadIdx.x & (G - 1);
+  qc.g = lane & (G - 1);
   if (qi >= T.nq) return false;
   decode_index(T, qi, qc);
   return true;

@@ -318,15 +318,14 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
   // One launch per (class, variant); within a launch, tasks with the most
   // work per block come first so the longest blocks start earliest.
   for (int c = 0; c < mi::nclass(); ++c) {
-    // vk 0-6: register kernel with G = 1, 2, ..., 64 lanes per quartet;
-    // 7-9: wave kernel with 1, 2, 4 wavefronts; 10-13: global-table kernel
-    // with 1, 2, 4, 8 wavefronts
-    for (int vk = 0; vk < 14; ++vk) {
-      const int var = vk < 7 ? 1 : (vk < 10 ? 0 : 2);
-      const int nw = vk < 7 ? (1 << vk) : (vk < 10 ? (1 << (vk - 7)) : (1 << (vk - 10)));
+    // vk 0: register kernel (each task its own G); 1-3: wave kernel with
+    // 1, 2, 4 wavefronts; 4-7: global-table kernel with 1, 2, 4, 8 wavefronts
+    for (int vk = 0; vk < 8; ++vk) {
+      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : 2);
+      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (1 << (vk - 4)));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
-        if (t.variant == var && t.G == nw) ts.push_back(t);
+        if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
       if (ts.empty()) continue;
       Launch L{};
       L.cls = c;
@@ -337,15 +336,19 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
       L.ntasks = (int)ts.size();
       int64_t blocks = 0;
       int64_t qs = 0;
-      for (TaskDesc& t : ts) {  // first launch-flat quartet of the task
-        t.blockStart = (int32_t)qs;
+      int64_t waves = 0;
+      for (TaskDesc& t : ts) {
+        // register kernel: first wavefront of the task (a task takes
+        // ceil(nq G / 64) whole wavefronts); wave kernels: first quartet
+        t.blockStart = (int32_t)(reg ? waves : qs);
+        waves += reg ? (t.nq * t.G + 63) / 64 : 0;
         qs += t.nq;
         all_tasks.push_back(t);
       }
-      if (reg) blocks = (qs * nw + 255) / 256;
+      if (reg) blocks = (waves + 3) / 4;
       if (var == 0) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
       if (var == 2) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wg_kernel_blocks_per_cu(c, nw));
-      if (blocks > INT32_MAX || qs > INT32_MAX) {
+      if (blocks > INT32_MAX || qs > INT32_MAX || waves > INT32_MAX) {
         return set_err(ctx, MI_ENOMEM, "class %d: %lld quartets exceed one launch", c, (long long)qs);
       }
       L.nblocks = (int)blocks;

@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
   }
   int ti = 0;
   for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
-    while (ti + 1 < args.ntasks && args.tasks[ti + 1].blockStart <= q) ++ti;
+    ti = advance_task(args, ti, q);
     const TaskDesc T = args.tasks[ti];
     QuartetCtx qc;
     decode_index(T, q - T.blockStart, qc);
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
   const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
   int ti = 0;
   for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
-    while (ti + 1 < args.ntasks && args.tasks[ti + 1].blockStart <= q) ++ti;
+    ti = advance_task(args, ti, q);
     const TaskDesc T = args.tasks[ti];
     QuartetCtx qc;
     decode_index(T, q - T.blockStart, qc);

@@ -152,8 +152,7 @@ class Plan:
 
     def launches(self) -> list:
         """Per class launch: ((la,lb,lc,ld), shell quartets, primitive quartets,
-        blocks, kernel), kernel = "R<G>" (generated register kernel, G lanes per
-        quartet), "W<nw>" (LDS
+        blocks, kernel), kernel = "R" (generated register kernel), "W<nw>" (LDS
         wave kernel, nw wavefronts per quartet) or "G<nw>" (global-table kernel,
         nw wavefronts per block)."""
         out = []
@@ -163,7 +162,7 @@ class Plan:
             n3 = (ctypes.c_int64 * 3)()
             k2 = (ctypes.c_int32 * 2)()
             self.ctx.check(self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3, k2))
-            kern = ("R" if k2[0] == 1 else "W" if k2[0] == 0 else "G") + str(k2[1])
+            kern = "R" if k2[0] == 1 else ("W" if k2[0] == 0 else "G") + str(k2[1])
             out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), kern))
         return out
 
