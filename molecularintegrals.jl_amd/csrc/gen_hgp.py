@@ -298,14 +298,37 @@ def w_tables(c):
     return desc, vitems, accsrc, hitems, outs
 
 
+# Per-primitive coefficient block of the LDS wave kernel (doubles, W_CF of
+# them per wavefront): the VRR items index it, so one formula serves both
+# builds without selects.
+CF_PA, CF_WP, CF_QC, CF_WQ = 0, 3, 6, 9      # + direction
+CF_ETA_S, CF_ZETA_S = 12, 13                 # eta/(zeta+eta), zeta/(zeta+eta)
+CF_N_OO2Z, CF_N_OO2E, CF_N_OO2S = 16, 24, 32  # + n: n/(2 zeta), n/(2 eta), n/(2(zeta+eta)), n = 0..7
+
+
+def vrr_item_w(s1, s2, s3, dr, n1, n2, kind, h2, h3):
+    """LDS-kernel VRR item (4 words): value = cf[A] v[s1] + cf[B] v[s1+1]
+    + cf[C] v[s3+1] + cf[D] (v[s2] - cf[E] v[s2+1]), slots as byte offsets.
+      w0 = 8 s1 | A << 20 | B << 26,  w1 = 8 s2 | C << 20 | D << 26,  w2 = 8 s3 | E << 20
+    a-build: A = PA_i, B = WP_i, C = 0, D = n1/(2 zeta), E = eta/s
+    c-build: A = QC_i, B = WQ_i, C = n1/(2 s), D = n2/(2 eta), E = zeta/s"""
+    assert max(s1, s2, s3) < (1 << 17) and n1 < 8 and n2 < 8
+    if kind == 0:
+        A, B, C, D, E = CF_PA + dr, CF_WP + dr, CF_N_OO2S, CF_N_OO2Z + (n1 if h2 else 0), CF_ETA_S
+    else:
+        A, B = CF_QC + dr, CF_WQ + dr
+        C, D, E = CF_N_OO2S + (n1 if h3 else 0), CF_N_OO2E + (n2 if h2 else 0), CF_ZETA_S
+    return [8 * s1 | A << 20 | B << 26, 8 * s2 | C << 20 | D << 26, 8 * s3 | E << 20, 0]
+
+
 def pack16(vi, hi, ou, am):
-    """LDS-kernel blob: uint2 items with 16-bit slot fields.
-      VRR  x = s1 | s2 << 16, y = s3 | dir << 16 | n1 << 18 | n2 << 21 | kind << 24 | has2 << 25 | has3 << 26
+    """LDS-kernel blob: VRR items (vrr_item_w, 4 words), then uint2 HRR items
+    with 16-bit slot fields, out words and acc sources:
       HRR  x = s1 | s2 << 16, y = dir | ket << 2
       out  slot | ma << 16 | mb << 20 | mc << 24 | md << 28;  acc as is."""
     words = []
-    for s1, s2, s3, dr, n1, n2, kind, h2, h3 in vi:
-        words += [s1 | (s2 << 16), s3 | (dr << 16) | (n1 << 18) | (n2 << 21) | (kind << 24) | (h2 << 25) | (h3 << 26)]
+    for it in vi:
+        words += vrr_item_w(*it)
     for s1, s2, dr, ket in hi:
         words += [s1 | (s2 << 16), dr | (ket << 2)]
     words += [o | (ma << 16) | (mb << 20) | (mc << 24) | (md << 28) for o, ma, mb, mc, md in ou]

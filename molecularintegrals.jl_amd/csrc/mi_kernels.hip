@@ -80,6 +80,7 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
 // ---------------------------------------------------------------------------
 #define W_ACC_MAX 16   // contracted values per lane: nacc <= 1024 (classes up to (dd|dd))
 #define W_SCL 16       // LDS stride of the per-shell scale rows
+#define W_CF 48        // per-wavefront primitive coefficient block (indices: gen_hgp.py CF_*)
 
 struct WTablesDev {
   const uint32_t* blob;  // kWBlob on the device
@@ -135,29 +136,31 @@ __host__ __device__ __forceinline__ int w_desc_doubles(const WClassDesc& D) { re
 // contracted (e0|f0) in registers; the shares are summed in wave order (a
 // fixed order: results do not depend on scheduling), then all waves run the
 // HRR levels and the scatter.
-// LDS: [op tables][hrr: nhrr][scales: 4*W_SCL][vrr: nw * nvrr]
+// LDS: [op tables][hrr: nhrr][scales: 4*W_SCL][coefficients: nw * W_CF][vrr: nw * nvrr]
+template <int ACC>
 __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W,
                                                     int64_t nq_launch) {
   extern __shared__ double lds[];
   const int nv = D.vlev[D.nvlev], nh = D.hlev[D.nhlev];
-  uint2_t* sV = reinterpret_cast<uint2_t*>(lds);
-  uint2_t* sH = sV + nv;
+  const uint4* sV = reinterpret_cast<const uint4*>(lds);
+  uint2_t* sH = reinterpret_cast<uint2_t*>(lds) + 2 * nv;
   uint32_t* sO = reinterpret_cast<uint32_t*>(sH + nh);
   uint32_t* sA = sO + D.nout;
   double* hrr = lds + w_desc_doubles(D);
   double* scl = hrr + D.nhrr;
-  double* vrr_all = scl + 4 * W_SCL;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int nw = nthr >> 6, wv = tid >> 6, lane = tid & 63;
+  double* cf = scl + 4 * W_SCL + wv * W_CF;
+  double* vrr_all = scl + 4 * W_SCL + nw * W_CF;
   double* vrr = vrr_all + wv * D.nvrr;
   stage_chunks(reinterpret_cast<uint4*>(lds), reinterpret_cast<const uint4*>(W.blob) + D.boff, D.bwords / 4);
   __syncthreads();
   const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
   const int mtop = D.mmax - 1;
   const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
-  int asrc[W_ACC_MAX];
+  int asrc[ACC];
 #pragma unroll
-  for (int k = 0; k < W_ACC_MAX; ++k) {
+  for (int k = 0; k < ACC; ++k) {
     const int x = k * 64 + lane;
     asrc[k] = x < D.nacc ? (int)(sA[x] & 0xFFFF) : 0;  // bit 31 marks a diagonal (e0|e0)
   }
@@ -177,9 +180,9 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     if (tid < nb) scl[W_SCL + tid] = args.bf_scale[P.bfB + tid];
     if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
     if (tid < nd) scl[3 * W_SCL + tid] = args.bf_scale[Q.bfB + tid];
-    double acc[W_ACC_MAX];
+    double acc[ACC];
 #pragma unroll
-    for (int k = 0; k < W_ACC_MAX; ++k) acc[k] = 0.0;
+    for (int k = 0; k < ACC; ++k) acc[k] = 0.0;
     const int K = P.npp * Q.npp;
     for (int t = wv; t < K; t += nw) {
       const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
@@ -193,41 +196,50 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
       // Boys: lane m (< vt0) evaluates the top order and recurses down to m.
       if (lane < D.vt0) vrr[lane] = pref * boys_lane(btab, T2, mtop, lane);
       __syncwarp();
-      const double oo2z = bp.oo2z, oo2e = kp.oo2z, oo2s = 0.5 * inv_s;
-      const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
-      const double WPx = -eta_s * PQx, WPy = -eta_s * PQy, WPz = -eta_s * PQz;
-      const double WQx = zeta_s * PQx, WQy = zeta_s * PQy, WQz = zeta_s * PQz;
+      // the primitive quartet's coefficient block (indices: gen_hgp.py CF_*)
+      if (lane < 40) {
+        const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
+        double c = 0.0;
+        if (lane < 3) c = sel3(lane, bp.PAx, bp.PAy, bp.PAz);
+        else if (lane < 6) c = -eta_s * sel3(lane - 3, PQx, PQy, PQz);
+        else if (lane < 9) c = sel3(lane - 6, kp.PAx, kp.PAy, kp.PAz);
+        else if (lane < 12) c = zeta_s * sel3(lane - 9, PQx, PQy, PQz);
+        else if (lane == 12) c = eta_s;
+        else if (lane == 13) c = zeta_s;
+        else if (lane >= 16) {
+          const int grp = (lane - 16) >> 3;
+          const double n = (double)((lane - 16) & 7);
+          c = n * (grp == 0 ? bp.oo2z : (grp == 1 ? kp.oo2z : 0.5 * inv_s));
+        }
+        cf[lane] = c;
+      }
+      __syncwarp();
+      const char* vb = reinterpret_cast<const char*>(vrr);
       for (int lv = 0; lv < D.nvlev; ++lv) {
         const int i1 = D.vlev[lv + 1];
         for (int it = D.vlev[lv] + lane; it < i1; it += 64) {
-          const uint2_t o = sV[it];
-          const int s1 = o.x & 0xFFFF, s2 = o.x >> 16, s3 = o.y & 0xFFFF;
-          const int dir = (o.y >> 16) & 3;
-          const double n1 = (double)((o.y >> 18) & 7), n2 = (double)((o.y >> 21) & 7);
-          const bool cb = (o.y >> 24) & 1;
-          // a-build: PA [s1] + WP [s1+1] + n1/(2 zeta)([s2] - eta/s [s2+1])
-          // c-build: QC [s1] + WQ [s1+1] + n1/(2s) [s3+1] + n2/(2 eta)([s2] - zeta/s [s2+1])
-          const double X1 = cb ? sel3(dir, kp.PAx, kp.PAy, kp.PAz) : sel3(dir, bp.PAx, bp.PAy, bp.PAz);
-          const double X2 = cb ? sel3(dir, WQx, WQy, WQz) : sel3(dir, WPx, WPy, WPz);
-          const double c3 = cb ? n1 * oo2s : 0.0;
-          const double c2 = cb ? n2 * oo2e : n1 * oo2z;
-          const double r2 = cb ? zeta_s : eta_s;
-          double val = X1 * vrr[s1] + X2 * vrr[s1 + 1];
-          val += c3 * vrr[s3 + 1];
-          val += c2 * (vrr[s2] - r2 * vrr[s2 + 1]);
+          const uint4 o = sV[it];
+          const double v1 = *reinterpret_cast<const double*>(vb + (o.x & 0xFFFFF));
+          const double v1p = *reinterpret_cast<const double*>(vb + (o.x & 0xFFFFF) + 8);
+          const double v2 = *reinterpret_cast<const double*>(vb + (o.y & 0xFFFFF));
+          const double v2p = *reinterpret_cast<const double*>(vb + (o.y & 0xFFFFF) + 8);
+          const double v3p = *reinterpret_cast<const double*>(vb + (o.z & 0xFFFFF) + 8);
+          double val = cf[(o.x >> 20) & 63] * v1 + cf[o.x >> 26] * v1p;
+          val += cf[(o.y >> 20) & 63] * v3p;
+          val += cf[o.y >> 26] * (v2 - cf[o.z >> 20] * v2p);
           vrr[D.vt0 + it] = val;
         }
         __syncwarp();
       }
 #pragma unroll
-      for (int k = 0; k < W_ACC_MAX; ++k)
+      for (int k = 0; k < ACC; ++k)
         if (k * 64 + lane < D.nacc) acc[k] += vrr[asrc[k]];
       __syncwarp();
     }
     // per-wave shares -> own VRR area, then summed in wave order into hrr[0..nacc)
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < W_ACC_MAX; ++k)
+    for (int k = 0; k < ACC; ++k)
       if (k * 64 + lane < D.nacc) vrr[k * 64 + lane] = acc[k];
     __syncthreads();
     for (int x = tid; x < D.nacc; x += nthr) {
@@ -477,12 +489,21 @@ hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStr
 
 size_t wclass_lds_bytes(int c, int nw) {
   const WClassDesc& D = kWClassDesc[c];
-  return sizeof(double) * (size_t)(w_desc_doubles(D) + D.nhrr + 4 * W_SCL + (size_t)nw * D.nvrr);
+  return sizeof(double) * (size_t)(w_desc_doubles(D) + D.nhrr + 4 * W_SCL + (size_t)nw * (W_CF + D.nvrr));
+}
+
+// Instantiation holding the class's contracted block: ACC values per lane.
+static const void* w_kernel_fn(int c) {
+  const int nacc = kWClassDesc[c].nacc;
+  if (nacc <= 64 * 2) return (const void*)hgp_w_kernel<2>;
+  if (nacc <= 64 * 4) return (const void*)hgp_w_kernel<4>;
+  if (nacc <= 64 * 8) return (const void*)hgp_w_kernel<8>;
+  return (const void*)hgp_w_kernel<16>;
 }
 
 int wave_kernel_blocks_per_cu(int c, int nw) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)hgp_w_kernel, 64 * nw,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, w_kernel_fn(c), 64 * nw,
                                                    wclass_lds_bytes(c, nw)) != hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;
@@ -500,8 +521,8 @@ hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, in
   WTablesDev W{t->blob};
   const WClassDesc D = kWClassDesc[c];
   const size_t lds = wclass_lds_bytes(c, nw);
-  hipLaunchKernelGGL(hgp_w_kernel, dim3(nblocks), dim3(64 * nw), lds, stream, args, D, W, nq_launch);
-  return hipGetLastError();
+  void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<WClassDesc*>(&D), &W, &nq_launch};
+  return hipLaunchKernel(w_kernel_fn(c), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
 size_t wgclass_lds_bytes(int c) { return sizeof(double) * (size_t)(4 * W_SCL + kWClassDesc[c].nvrr); }
@@ -544,8 +565,12 @@ hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream) {
 }
 
 hipError_t set_wave_kernel_lds_limit(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void*)hgp_w_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
+  const void* wk[] = {(const void*)hgp_w_kernel<2>, (const void*)hgp_w_kernel<4>, (const void*)hgp_w_kernel<8>,
+                      (const void*)hgp_w_kernel<16>};
+  for (const void* f : wk) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+  }
   return hipFuncSetAttribute((const void*)hgp_wg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
