@@ -2,14 +2,18 @@
 
 For every quartet class it picks, on the GPU it runs on:
   * the kernel variant: generated register kernel (lane group per shell
-    quartet) or the LDS wave kernel (1, 2 or 4 wavefronts per shell quartet), and
+    quartet), the LDS wave kernel (1, 2 or 4 wavefronts per shell quartet) or
+    the global-table kernel (1, 2, 4 or 8 wavefronts per block), and
   * for the register kernel, the lanes per quartet G as a function of the
     contraction depth: G = pow2floor(K_bra * K_ket / tau), clamped to [1, 64],
-    with tau chosen per class; for the wave kernel, the wavefront count.
+    with tau chosen per class; for the wave kernels, the wavefront count.
 Method: each candidate setting is applied to all classes at once, one
-execute is profiled with HIP events between launches (median of ``reps``),
-and each class keeps its fastest candidate.  The result is written as
-hgp_tuning.txt rows "class bra_bucket ket_bucket variant G", read by mi_create.
+execute per workload is profiled with HIP events between launches (median
+of ``reps``), and each class keeps the candidate with the smallest sum over
+workloads of its time relative to that workload's best candidate (so the
+small, latency-bound workload and the large batched one count alike).  The
+result is written as hgp_tuning.txt rows "class bra_bucket ket_bucket variant
+G", read by mi_create.
 
 Usage (on the GPU):  python -m mi_amd.autotune [--out PATH] [--reps N]
 """
@@ -37,15 +41,20 @@ def pow2floor(x):
     return g
 
 
-def rows_for(cls_choice, has_reg, w_ok):
-    """cls_choice[c] = (variant, tau) for the register kernel or (0, nw) for
-    the wave kernel with nw wavefronts per quartet."""
+def fallback(c, w_ok, g_ok):
+    """A wave-kernel setting that exists for class c."""
+    w = [n for n in (1, 2, 4) if w_ok[c].get(n)]
+    return (0, max(w)) if w else (2, 4)
+
+
+def rows_for(cls_choice, has_reg, w_ok, g_ok):
+    """cls_choice[c] = (1, tau) for the register kernel, (0, nw) for the LDS
+    wave kernel, (2, nw) for the global-table kernel."""
     rows = []
     for c, (var, tau) in cls_choice.items():
-        if var == 1 and not has_reg[c]:
-            var, tau = 0, max([n for n in (1, 2, 4) if w_ok[c].get(n)] or [1])
-        if var == 0 and not w_ok[c].get(int(tau), False):
-            tau = max([n for n in (1, 2, 4) if w_ok[c].get(n)] or [1])
+        if (var == 1 and not has_reg[c]) or (var == 0 and not w_ok[c].get(int(tau), False)) or \
+                (var == 2 and not g_ok[c].get(int(tau), False)):
+            var, tau = fallback(c, w_ok, g_ok)
         for bb in range(NBUCKET):
             for kb in range(NBUCKET):
                 G = pow2floor(max(1.0, BUCKET_K[bb] * BUCKET_K[kb] / tau)) if var == 1 else int(tau)
@@ -53,10 +62,12 @@ def rows_for(cls_choice, has_reg, w_ok):
     return rows
 
 
-def class_times(ctx, fb, rows, reps):
+def class_times(ctx, wl, rows, reps):
+    """Per-class time (ms) of one execute of workload wl: a FlatBasis or a
+    list of them (a batched plan)."""
     import torch
     ctx.set_tuning(rows)
-    plan = ctx.plan(fb, order=1)
+    plan = ctx.batch_plan(wl) if isinstance(wl, list) else ctx.plan(wl, order=1)
     out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
     sptr = torch.cuda.current_stream().cuda_stream
     launches = plan.launches()
@@ -75,46 +86,63 @@ def autotune(workloads, reps=7, verbose=True):
     ctx = Context(0)
     ctx.set_streams(1)
     # which variants each class has: probe the library by setting a row
-    has_reg, w_ok = {}, {}
+    has_reg, w_ok, g_ok = {}, {}, {}
     for k in range(len(classes)):
         try:
             ctx.set_tuning([(k, 0, 0, 1, 1)])
             has_reg[k] = True
         except Exception:
             has_reg[k] = False
-        w_ok[k] = {}
-        for nw in (1, 2, 4):
-            try:
-                ctx.set_tuning([(k, 0, 0, 0, nw)])
-                w_ok[k][nw] = True
-            except Exception:
-                w_ok[k][nw] = False
-    cands = [(0, 1), (0, 2), (0, 4)] + [(1, t) for t in TAUS]
-    total = {k: {} for k in range(len(classes))}
+        w_ok[k], g_ok[k] = {}, {}
+        for nw in (1, 2, 4, 8):
+            for var, ok in ((0, w_ok[k]), (2, g_ok[k])):
+                try:
+                    ctx.set_tuning([(k, 0, 0, var, nw)])
+                    ok[nw] = True
+                except Exception:
+                    ok[nw] = False
+    cands = [(0, 1), (0, 2), (0, 4), (2, 1), (2, 2), (2, 4), (2, 8)] + [(1, t) for t in TAUS]
+    # times[w][k][cand]
+    times = [{k: {} for k in range(len(classes))} for _ in workloads]
     for var, tau in cands:
         choice = {k: (var, tau) for k in range(len(classes))}
-        rows = rows_for(choice, has_reg, w_ok)
-        for fb in workloads:
-            per = class_times(ctx, fb, rows, reps)
+        rows = rows_for(choice, has_reg, w_ok, g_ok)
+        for wi, wl in enumerate(workloads):
+            per = class_times(ctx, wl, rows, reps)
             for cls, t in per.items():
                 k = cid[cls]
-                if (var == 1 and not has_reg[k]) or (var == 0 and not w_ok[k].get(tau)):
+                if (var == 1 and not has_reg[k]) or (var == 0 and not w_ok[k].get(tau)) or \
+                        (var == 2 and not g_ok[k].get(tau)):
                     continue
-                total[k][(var, tau)] = total[k].get((var, tau), 0.0) + t
+                times[wi][k][(var, tau)] = t
+    total = {k: {} for k in range(len(classes))}
+    for k in range(len(classes)):
+        for wi in range(len(workloads)):
+            tk = times[wi][k]
+            if not tk:
+                continue
+            t0 = min(tk.values())
+            for cand, t in tk.items():
+                total[k][cand] = total[k].get(cand, 0.0) + t / t0
+        # a candidate must have run on every workload that has the class
+        nw_have = sum(1 for wi in range(len(workloads)) if times[wi][k])
+        total[k] = {c: v for c, v in total[k].items()
+                    if sum(1 for wi in range(len(workloads)) if c in times[wi][k]) == nw_have}
     best = {}
     for k in range(len(classes)):
         if not total[k]:
-            best[k] = (1 if has_reg[k] else 0, 4)
+            best[k] = (1, 4) if has_reg[k] else fallback(k, w_ok, g_ok)
             continue
         (var, tau), t = min(total[k].items(), key=lambda kv: kv[1])
         best[k] = (var, tau)
         if verbose:
-            alt = ", ".join(f"{'W' + str(tt) if v == 0 else 'R/' + str(tt if tt < 1e6 else 'inf')}:{x * 1e3:.1f}"
+            name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
+            alt = ", ".join(f"{name(v, tt)}:{x:.2f}"
                             for (v, tt), x in sorted(total[k].items(), key=lambda kv: kv[1])[:4])
-            print(f"[autotune] {''.join('spdf'[x] for x in classes[k])}: "
-                  f"{'W nw=' + str(tau) if var == 0 else 'R tau=' + str(tau)}  ({alt} us)", flush=True)
+            print(f"[autotune] {''.join('spdf'[x] for x in classes[k])}: {name(var, tau)}  "
+                  f"(score = sum of time / best per workload: {alt})", flush=True)
     ctx.close()
-    return rows_for(best, has_reg, w_ok), best
+    return rows_for(best, has_reg, w_ok, g_ok), best
 
 
 def main():
@@ -124,13 +152,18 @@ def main():
     ap.add_argument("--reps", type=int, default=7)
     args = ap.parse_args()
     os.environ["MI_HGP_TUNING"] = "none"  # tune from the heuristic baseline
+    # the 60-function molecule (latency-bound launches), a batch of 48
+    # molecules of 3-24 atoms (throughput), and a cluster-recipe molecule
+    # with f shells
     wls = [configs.ethane_split_valence(0).flat()]
-    rng_mols = configs.batch_molecules(1, nmol=2)
-    wls += [m.flat() for m in rng_mols]
+    wls.append([m.flat() for m in configs.batch_molecules(1, nmol=48)])
+    rng = np.random.Generator(np.random.PCG64(9))
+    wls.append(configs.random_molecule(rng, 8, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat())
     rows, best = autotune(wls, args.reps)
     with open(args.out, "w") as f:
         f.write("# HGP ERI kernel choice per (class, bra bucket, ket bucket): variant (1 register kernel, "
-                "0 LDS wave kernel) and lanes per quartet G.\n# Written by mi_amd/autotune.py on the GPU it "
+                "0 LDS wave kernel, 2 global-table kernel) and G (lanes per quartet, or wavefronts).\n"
+                "# Written by mi_amd/autotune.py on the GPU it "
                 "describes; columns: class bra_bucket ket_bucket variant G\n")
         for r in rows:
             f.write(" ".join(str(int(x)) for x in r) + "\n")
