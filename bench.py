@@ -220,13 +220,14 @@ def leg_serving(nreq, rate, seed=0):
                      "peak_device_gb": r.stats["peak_bytes"] / 1e9}
         log(f"[bench] serving {name}: p50 {r.p(0.5) * 1e3:.1f} ms p99 {r.p(0.99) * 1e3:.1f} ms "
             f"{r.throughput:.3e} unique ERIs/s, makespan {r.makespan_s:.2f} s")
-    err = 0.0
+    err = {"eri": 0.0, "jk": 0.0}
     for q in reqs:
         a = [t.cpu().numpy() for t in res["plain_fp64"].outputs[q.rid]]
         b = [t.cpu().numpy() for t in res["mixed"].outputs[q.rid]]
         k = 0 if q.kind == "eri" else 1
-        err = max(err, max(float(np.abs(x - y).max()) for x, y in zip(a[k:], b[k:])))
-    out["mixed"]["max_abs_err_vs_fp64"] = err
+        err[q.kind] = max(err[q.kind], max(float(np.abs(x - y).max()) for x, y in zip(a[k:], b[k:])))
+    out["mixed"]["max_abs_err_eri_vs_fp64"] = err["eri"]
+    out["mixed"]["max_abs_err_jk_vs_fp64"] = err["jk"]
     del res
     return out
 
