@@ -97,7 +97,9 @@ int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
  * Affects plans created afterwards. */
 int mi_ctx_set_tuning(mi_ctx *ctx, const int32_t *rows, int32_t nrows);
 /* Number of streams class launches are spread over (1 = caller's stream only;
- * at most 4).  Affects plans created afterwards. */
+ * at most 16; default 4, HIP's default number of hardware queues -- more
+ * streams overlap only if the process runs with GPU_MAX_HW_QUEUES raised to
+ * match).  Affects plans created afterwards. */
 int mi_ctx_set_streams(mi_ctx *ctx, int32_t nstreams);
 
 /* One-shot drop-in: host basis in, host array out (caller-owned, out_len

@@ -299,16 +299,19 @@ __device__ __forceinline__ ScatterCtx make_scatter(const EriArgs& a, const Quart
 // Unique slot of the block element (fa fb|fc fd) or -1 when the element is a
 // duplicate inside this shell quartet.  Slot algebra: src/Utils.jl:14-19 and
 // the enumeration order of src/Basis.jl:203.
+// Pair indices in 32 bits (molecules hold at most 65535 functions, so
+// i(i-1) < 2^32 and M + 1 < 2^32), slot products as 32x32->64 multiplies.
 __device__ __forceinline__ int64_t unique_slot(const ScatterCtx& s, int fa, int fb, int fc, int fd) {
   if (s.braSame && fa < fb) return -1;
   if (s.ketSame && fc < fd) return -1;
-  const int64_t i = (fa > fb ? fa : fb) + 1, j = (fa > fb ? fb : fa) + 1;
-  const int64_t k = (fc > fd ? fc : fd) + 1, l = (fc > fd ? fd : fc) + 1;
-  const int64_t ij = i * (i - 1) / 2 + j, kl = k * (k - 1) / 2 + l;
+  const uint32_t i = (uint32_t)(fa > fb ? fa : fb) + 1u, j = (uint32_t)(fa > fb ? fb : fa) + 1u;
+  const uint32_t k = (uint32_t)(fc > fd ? fc : fd) + 1u, l = (uint32_t)(fc > fd ? fd : fc) + 1u;
+  const uint32_t ij = ((i * (i - 1u)) >> 1) + j, kl = ((k * (k - 1u)) >> 1) + l;
   if (s.quartetSame && ij > kl) return -1;
-  const int64_t lo = ij < kl ? ij : kl, hi = ij < kl ? kl : ij;
-  if (s.order == 0) return hi * (hi - 1) / 2 + lo - 1;
-  return (lo - 1) * (s.M + 1) - lo * (lo - 1) / 2 + hi - lo;
+  const uint32_t lo = ij < kl ? ij : kl, hi = ij < kl ? kl : ij;
+  if (s.order == 0) return (int64_t)(((uint64_t)hi * (uint64_t)(hi - 1u)) >> 1) + (int64_t)lo - 1;
+  return (int64_t)((uint64_t)(lo - 1u) * (uint64_t)(s.M + 1)) - (int64_t)(((uint64_t)lo * (uint64_t)(lo - 1u)) >> 1) +
+         (int64_t)(hi - lo);
 }
 
 // 0: FP64 VRR, 1: FP32 VRR, 2: screened.  Screening uses the Schwarz bound of

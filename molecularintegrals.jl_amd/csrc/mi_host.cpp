@@ -161,6 +161,7 @@ int validate(mi_ctx* ctx, const mi_basis* b) {
     nbf += ncart(L);
   }
   if (nbf != b->nbf) return set_err(ctx, MI_EINVAL, "nbf=%d but the shells hold %d functions", b->nbf, nbf);
+  if (nbf > 65535) return set_err(ctx, MI_EUNSUPPORTED, "nbf=%d: at most 65535 functions per molecule", nbf);
   return MI_OK;
 }
 

@@ -11,7 +11,7 @@
 #include "mi_kernels.h"
 
 #define MI_NBUCKET 5
-#define MI_MAX_STREAMS 4
+#define MI_MAX_STREAMS 16
 
 // Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
 // generated register kernel with G lanes per quartet, 0 = LDS wave kernel with
@@ -28,7 +28,7 @@ struct mi_ctx {
   std::string err;
   size_t lds_limit = 0;
   int ncu = 256;
-  int nstreams = MI_MAX_STREAMS;              // class launches spread over this many streams
+  int nstreams = 4;                           // class launches spread over this many streams
   hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
   hipStream_t bstream = nullptr;              // Schwarz bounds passes at plan build
   std::vector<TuneEntry> tune;                // [class][bra bucket][ket bucket]
