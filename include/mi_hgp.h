@@ -88,13 +88,15 @@ const char *mi_last_error(const mi_ctx *ctx);
 int32_t mi_lmax(void);                   /* largest supported shell L      */
 int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
 
-/* Kernel selection (no counterpart in the reference).  rows: nrows x 5 int32
+/* Kernel selection (no counterpart in the reference).  rows: nrows x 6 int32
  * (quartet class, bra contraction bucket, ket contraction bucket, variant
- * [1 generated register kernel, 0 LDS wave kernel], G = lanes per quartet for
- * the register kernel or wavefronts per quartet (1, 2, 4) for the wave kernel).
- * Replaces the whole table; unlisted entries use the built-in heuristic.
- * mi_create loads hgp_tuning.txt beside the library (MI_HGP_TUNING overrides).
- * Affects plans created afterwards. */
+ * [1 generated register kernel, 0 LDS wave kernel, 2 global-table kernel],
+ * G = lanes per quartet for the register kernel or wavefronts per quartet /
+ * block for the wave kernels, regime [0: class launches of fewer than 65536
+ * shell quartets, 1: larger ones, -1: both]).  Replaces the whole table
+ * (or nothing, on an invalid row); unlisted entries use the built-in
+ * heuristic.  mi_create loads hgp_tuning.txt beside the library
+ * (MI_HGP_TUNING overrides).  Affects plans created afterwards. */
 int mi_ctx_set_tuning(mi_ctx *ctx, const int32_t *rows, int32_t nrows);
 /* Number of streams class launches are spread over (1 = caller's stream only;
  * at most 16; default 4, HIP's default number of hardware queues -- more

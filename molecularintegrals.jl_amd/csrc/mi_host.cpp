@@ -100,11 +100,32 @@ int pow2floor(double x) {
   return g;
 }
 
-size_t tune_index(int cls, int bb, int kb) { return ((size_t)cls * MI_NBUCKET + bb) * MI_NBUCKET + kb; }
+size_t tune_index(int regime, int cls, int bb, int kb) {
+  return (((size_t)regime * mi::nclass() + cls) * MI_NBUCKET + bb) * MI_NBUCKET + kb;
+}
 
-// Tuning file: lines "class bra_bucket ket_bucket variant G" ('#' comments),
-// written by mi_amd/autotune.py.  Default location: hgp_tuning.txt next to
-// the library; MI_HGP_TUNING overrides, MI_HGP_TUNING=none disables.
+// Check one tuning row; regime -1 applies it to both regimes.
+bool tuning_row_ok(const mi_ctx* c, int cls, int bb, int kb, int var, int G, int regime) {
+  if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET) return false;
+  if (regime < -1 || regime >= MI_NREGIME) return false;
+  if (G < 1 || G > 64 || (G & (G - 1))) return false;
+  if (var < 0 || var > 2) return false;
+  if (var == 1 && !mi::has_register_kernel(cls)) return false;
+  if (var == 0 && !mi::wclass_fits_nw(cls, G, c->lds_limit)) return false;
+  if (var == 2 && !mi::wgclass_fits_nw(cls, G, c->lds_limit)) return false;
+  return true;
+}
+
+void set_tuning_row(mi_ctx* c, int cls, int bb, int kb, int var, int G, int regime) {
+  for (int r = 0; r < MI_NREGIME; ++r)
+    if (regime < 0 || regime == r) c->tune[tune_index(r, cls, bb, kb)] = {var, G};
+}
+
+// Tuning file: lines "class bra_bucket ket_bucket variant G [regime]" ('#'
+// comments; regime 0 = class launches of fewer than MI_SMALL_LAUNCH shell
+// quartets, 1 = larger ones, absent = both), written by mi_amd/autotune.py.
+// Default location: hgp_tuning.txt next to the library; MI_HGP_TUNING
+// overrides, MI_HGP_TUNING=none disables.
 void load_tuning_file(mi_ctx* c) {
   std::string path;
   const char* env = std::getenv("MI_HGP_TUNING");
@@ -124,15 +145,11 @@ void load_tuning_file(mi_ctx* c) {
   while (f && std::getline(f, line)) {
     if (line.empty() || line[0] == '#') continue;
     std::istringstream is(line);
-    int cls, bb, kb, var, G;
+    int cls, bb, kb, var, G, regime = -1;
     if (!(is >> cls >> bb >> kb >> var >> G)) continue;
-    if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET) continue;
-    if (var == 1 && !mi::has_register_kernel(cls)) continue;
-    if (G < 1 || G > 64 || (G & (G - 1))) continue;
-    if (var == 0 && !mi::wclass_fits_nw(cls, G, c->lds_limit)) continue;
-    if (var == 2 && !mi::wgclass_fits_nw(cls, G, c->lds_limit)) continue;
-    if (var < 0 || var > 2) continue;
-    c->tune[tune_index(cls, bb, kb)] = {var, G};
+    if (!(is >> regime)) regime = -1;
+    if (!tuning_row_ok(c, cls, bb, kb, var, G, regime)) continue;
+    set_tuning_row(c, cls, bb, kb, var, G, regime);
   }
 }
 
@@ -257,7 +274,7 @@ int mi_create(int device, mi_ctx** out) {
   }
   c->lds_limit = std::min<size_t>(c->lds_limit, 160 * 1024);  // CDNA4: 160 KiB LDS per CU
   if (c->lds_limit > 65536 && mi::set_wave_kernel_lds_limit(c->lds_limit) != hipSuccess) c->lds_limit = 65536;
-  c->tune.assign((size_t)mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
+  c->tune.assign((size_t)MI_NREGIME * mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
   load_tuning_file(c);
   *out = c;
   return MI_OK;
@@ -284,8 +301,8 @@ void mi_destroy(mi_ctx* ctx) {
 // register kernel when one exists with lanes ~ primitive quartets / 4, else the
 // wave kernel with as many wavefronts (<= 4) as the LDS budget allows, else the
 // global-table kernel with 4 wavefronts per block.
-static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg) {
-  const TuneEntry& te = ctx->tune[tune_index(t.cls, bb, kb)];
+static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, int regime) {
+  const TuneEntry& te = ctx->tune[tune_index(regime, t.cls, bb, kb)];
   int variant, G;
   if (te.variant >= 0) {
     variant = te.variant;
@@ -564,6 +581,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   // grouped while n_i stays within 80% of the group's first (the few extra
   // quartets are screened in the kernels).
   std::vector<std::vector<TaskDesc>> per_class(mi::nclass());
+  std::vector<std::vector<double>> kavgs(mi::nclass());
+  std::vector<int64_t> cls_nq(mi::nclass(), 0);
   int64_t nq_shell = 0, nq_prim = 0;
   std::vector<int64_t> Kpre(npair + 1, 0);  // prefix sums of primitive pairs in record order
   for (int r = 0; r < npair; ++r) Kpre[r + 1] = Kpre[r] + keys[r].K;
@@ -591,8 +610,9 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         t.obase = P->mol_off[m];
         const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
         t.M = nbm * (nbm + 1) / 2;
-        int e = choose_kernel(ctx, t, bb, kb, kavg);
-        if (e) return e;
+        t.G = bb;  // kernel chosen below, once the class's quartet count is known
+        t.variant = kb;
+        kavgs[t.cls].push_back(kavg);
         const int64_t sk = Kpre[b0 + nb] - Kpre[b0];
         if (same) {
           // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
@@ -605,6 +625,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         nq_shell += t.nq;
         nq_prim += t.nprim;
         per_class[t.cls].push_back(t);
+        cls_nq[t.cls] += t.nq;
         return MI_OK;
       };
       if (!screen_tasks) {
@@ -643,6 +664,16 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         return rc;
       }
     }
+  // kernel per task: the tuning regime follows the size of the class launch
+  for (int c = 0; c < mi::nclass(); ++c)
+    for (size_t k = 0; k < per_class[c].size(); ++k) {
+      TaskDesc& t = per_class[c][k];
+      const int bb = t.G, kb = t.variant;
+      if ((rc = choose_kernel(ctx, t, bb, kb, kavgs[c][k], cls_nq[c] < MI_SMALL_LAUNCH ? 0 : 1))) {
+        delete P;
+        return rc;
+      }
+    }
   P->nq_shell = nq_shell;
   P->nq_prim = nq_prim;
   std::vector<TaskDesc> all_tasks;
@@ -676,7 +707,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
         t.M = nbm * (nbm + 1) / 2;
         const int bk = keys[li.start].bucket;
-        if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n)))) {
+        if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n), 0))) {
           delete P;
           return rc;
         }
@@ -884,18 +915,16 @@ int mi_plan_execute(mi_plan* P, double* out, int64_t out_len, void* stream) {
 
 int mi_ctx_set_tuning(mi_ctx* ctx, const int32_t* rows, int32_t nrows) {
   if (!ctx || (nrows > 0 && !rows)) return MI_EINVAL;
-  ctx->tune.assign((size_t)mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
+  std::vector<TuneEntry> keep = ctx->tune;
+  ctx->tune.assign((size_t)MI_NREGIME * mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
   for (int r = 0; r < nrows; ++r) {
-    const int32_t* q = rows + 5 * r;
-    const int cls = q[0], bb = q[1], kb = q[2], var = q[3], G = q[4];
-    if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET)
-      return set_err(ctx, MI_EINVAL, "tuning row %d: bad class/bucket", r);
-    if (G < 1 || G > 64 || (G & (G - 1))) return set_err(ctx, MI_EINVAL, "tuning row %d: G=%d", r, G);
-    if ((var < 0 || var > 2) || (var == 1 && !mi::has_register_kernel(cls)) ||
-        (var == 0 && !mi::wclass_fits_nw(cls, G, ctx->lds_limit)) ||
-        (var == 2 && !mi::wgclass_fits_nw(cls, G, ctx->lds_limit)))
-      return set_err(ctx, MI_EINVAL, "tuning row %d: variant %d (G=%d) unavailable for class %d", r, var, G, cls);
-    ctx->tune[tune_index(cls, bb, kb)] = {var, G};
+    const int32_t* q = rows + 6 * r;
+    if (!tuning_row_ok(ctx, q[0], q[1], q[2], q[3], q[4], q[5])) {
+      ctx->tune = keep;
+      return set_err(ctx, MI_EINVAL, "tuning row %d (class %d buckets %d,%d variant %d G %d regime %d) unavailable", r,
+                     q[0], q[1], q[2], q[3], q[4], q[5]);
+    }
+    set_tuning_row(ctx, q[0], q[1], q[2], q[3], q[4], q[5]);
   }
   return MI_OK;
 }

@@ -11,6 +11,8 @@
 #include "mi_kernels.h"
 
 #define MI_NBUCKET 5
+#define MI_NREGIME 2             // tuning regimes: small and large class launches
+#define MI_SMALL_LAUNCH 65536    // shell quartets below which a class launch is "small" (latency-bound)
 #define MI_MAX_STREAMS 16
 
 // Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
@@ -31,7 +33,7 @@ struct mi_ctx {
   int nstreams = 4;                           // class launches spread over this many streams
   hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
   hipStream_t bstream = nullptr;              // Schwarz bounds passes at plan build
-  std::vector<TuneEntry> tune;                // [class][bra bucket][ket bucket]
+  std::vector<TuneEntry> tune;                // [regime][class][bra bucket][ket bucket]
 };
 
 struct Launch {

@@ -7,13 +7,15 @@ For every quartet class it picks, on the GPU it runs on:
   * for the register kernel, the lanes per quartet G as a function of the
     contraction depth: G = pow2floor(K_bra * K_ket / tau), clamped to [1, 64],
     with tau chosen per class; for the wave kernels, the wavefront count.
+Two regimes are tuned apart: small class launches (fewer than 65536 shell
+quartets, latency-bound: more lanes per quartet pay) and large ones
+(throughput-bound); a plan picks the regime per class from its quartet count.
 Method: each candidate setting is applied to all classes at once, one
 execute per workload is profiled with HIP events between launches (median
-of ``reps``), and each class keeps the candidate with the smallest sum over
-workloads of its time relative to that workload's best candidate (so the
-small, latency-bound workload and the large batched one count alike).  The
-result is written as hgp_tuning.txt rows "class bra_bucket ket_bucket variant
-G", read by mi_create.
+of ``reps``), and each class keeps, per regime, the candidate with the
+smallest sum over that regime's workloads of its time relative to the
+workload's best candidate.  The result is written as hgp_tuning.txt rows
+"class bra_bucket ket_bucket variant G regime", read by mi_create.
 
 Usage (on the GPU):  python -m mi_amd.autotune [--out PATH] [--reps N]
 """
@@ -80,7 +82,9 @@ def class_times(ctx, wl, rows, reps):
     return per
 
 
-def autotune(workloads, reps=7, verbose=True):
+def autotune(workloads, regimes, reps=7, verbose=True):
+    """workloads: FlatBasis or lists of them (batched plans); regimes[i]: the
+    tuning regime (0 small class launches, 1 large) workload i stands for."""
     classes = quartet_classes()
     cid = {c: k for k, c in enumerate(classes)}
     ctx = Context(0)
@@ -115,34 +119,49 @@ def autotune(workloads, reps=7, verbose=True):
                         (var == 2 and not g_ok[k].get(tau)):
                     continue
                 times[wi][k][(var, tau)] = t
-    total = {k: {} for k in range(len(classes))}
-    for k in range(len(classes)):
-        for wi in range(len(workloads)):
-            tk = times[wi][k]
-            if not tk:
-                continue
-            t0 = min(tk.values())
-            for cand, t in tk.items():
-                total[k][cand] = total[k].get(cand, 0.0) + t / t0
-        # a candidate must have run on every workload that has the class
-        nw_have = sum(1 for wi in range(len(workloads)) if times[wi][k])
-        total[k] = {c: v for c, v in total[k].items()
-                    if sum(1 for wi in range(len(workloads)) if c in times[wi][k]) == nw_have}
-    best = {}
-    for k in range(len(classes)):
-        if not total[k]:
-            best[k] = (1, 4) if has_reg[k] else fallback(k, w_ok, g_ok)
-            continue
-        (var, tau), t = min(total[k].items(), key=lambda kv: kv[1])
-        best[k] = (var, tau)
-        if verbose:
-            name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
-            alt = ", ".join(f"{name(v, tt)}:{x:.2f}"
-                            for (v, tt), x in sorted(total[k].items(), key=lambda kv: kv[1])[:4])
-            print(f"[autotune] {''.join('spdf'[x] for x in classes[k])}: {name(var, tau)}  "
-                  f"(score = sum of time / best per workload: {alt})", flush=True)
+    def pick(wis):
+        """Best candidate per class over the workloads wis (score: sum of
+        time / best time per workload; a candidate must have run on every
+        workload that has the class)."""
+        best, score = {}, {}
+        for k in range(len(classes)):
+            tot = {}
+            have = [wi for wi in wis if times[wi][k]]
+            for wi in have:
+                t0 = min(times[wi][k].values())
+                for cand, t in times[wi][k].items():
+                    tot[cand] = tot.get(cand, 0.0) + t / t0
+            tot = {c: v for c, v in tot.items() if all(c in times[wi][k] for wi in have)}
+            if tot:
+                best[k] = min(tot.items(), key=lambda kv: kv[1])[0]
+                score[k] = tot
+        return best, score
+
+    name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
+    rows = []
+    picks = []
+    for regime in (0, 1):
+        wis = [wi for wi, r in enumerate(regimes) if r == regime]
+        picks.append(pick(wis))
+    for regime in (0, 1):
+        best, score = picks[regime]
+        other = picks[1 - regime][0]
+        choice = {}
+        for k in range(len(classes)):
+            if k in best:
+                choice[k] = best[k]
+            elif k in other:  # class absent from this regime's workloads
+                choice[k] = other[k]
+            else:
+                choice[k] = (1, 4) if has_reg[k] else fallback(k, w_ok, g_ok)
+            if verbose and k in score:
+                alt = ", ".join(f"{name(v, tt)}:{x:.2f}"
+                                for (v, tt), x in sorted(score[k].items(), key=lambda kv: kv[1])[:4])
+                print(f"[autotune] regime {regime} {''.join('spdf'[x] for x in classes[k])}: "
+                      f"{name(*choice[k])}  ({alt})", flush=True)
+        rows += [r + (regime,) for r in rows_for(choice, has_reg, w_ok, g_ok)]
     ctx.close()
-    return rows_for(best, has_reg, w_ok, g_ok), best
+    return rows
 
 
 def main():
@@ -152,19 +171,21 @@ def main():
     ap.add_argument("--reps", type=int, default=7)
     args = ap.parse_args()
     os.environ["MI_HGP_TUNING"] = "none"  # tune from the heuristic baseline
-    # the 60-function molecule (latency-bound launches), a batch of 48
-    # molecules of 3-24 atoms (throughput), and a cluster-recipe molecule
-    # with f shells
-    wls = [configs.ethane_split_valence(0).flat()]
-    wls.append([m.flat() for m in configs.batch_molecules(1, nmol=48)])
+    # small class launches (latency-bound): the 60-function molecule and a
+    # 4-atom cluster-recipe molecule with f shells; large ones (throughput): a
+    # batch of 48 molecules of 3-24 atoms and a 16-atom cluster-recipe molecule
     rng = np.random.Generator(np.random.PCG64(9))
-    wls.append(configs.random_molecule(rng, 8, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat())
-    rows, best = autotune(wls, args.reps)
+    wls = [configs.ethane_split_valence(0).flat(),
+           configs.random_molecule(rng, 4, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat(),
+           [m.flat() for m in configs.batch_molecules(1, nmol=48)],
+           configs.random_molecule(rng, 16, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat()]
+    rows = autotune(wls, [0, 0, 1, 1], args.reps)
     with open(args.out, "w") as f:
-        f.write("# HGP ERI kernel choice per (class, bra bucket, ket bucket): variant (1 register kernel, "
-                "0 LDS wave kernel, 2 global-table kernel) and G (lanes per quartet, or wavefronts).\n"
-                "# Written by mi_amd/autotune.py on the GPU it "
-                "describes; columns: class bra_bucket ket_bucket variant G\n")
+        f.write("# HGP ERI kernel choice per (class, bra bucket, ket bucket, regime): variant (1 register "
+                "kernel, 0 LDS wave kernel, 2 global-table kernel) and G (lanes per quartet, or wavefronts);\n"
+                "# regime 0 = class launches of fewer than 65536 shell quartets, 1 = larger ones.\n"
+                "# Written by mi_amd/autotune.py on the GPU it describes; columns: class bra_bucket ket_bucket "
+                "variant G regime\n")
         for r in rows:
             f.write(" ".join(str(int(x)) for x in r) + "\n")
     print(f"[autotune] wrote {args.out} ({len(rows)} rows)", flush=True)

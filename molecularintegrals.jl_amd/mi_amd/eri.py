@@ -73,8 +73,10 @@ class Context:
         return J, K
 
     def set_tuning(self, rows):
-        """rows: iterable of (class, bra_bucket, ket_bucket, variant, G)."""
-        arr = np.ascontiguousarray(np.asarray(list(rows), dtype=np.int32).reshape(-1, 5))
+        """rows: iterable of (class, bra_bucket, ket_bucket, variant, G[, regime]);
+        regime 0 = small class launches, 1 = large, -1 (default) = both."""
+        rows = [tuple(r) + ((-1,) if len(r) == 5 else ()) for r in rows]
+        arr = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1, 6))
         self.check(self.L.mi_ctx_set_tuning(self.h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                             arr.shape[0]))
 
