@@ -167,6 +167,12 @@ int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *
 /* One execute with HIP events between launches on `stream`; synchronises and
  * writes nlaunch+1 durations in ms (ms[0] = shell-pair kernel). */
 int mi_plan_profile(mi_plan *plan, double *out_device, int64_t out_len, void *stream, float *ms);
+/* One profiled execute (output written as by mi_plan_execute), then the
+ * class launches are reassigned to stream lanes by their measured durations
+ * (longest first, each to the least-loaded lane) instead of the built-in cost
+ * estimate.  Results do not change; a captured graph is dropped (capture
+ * again).  For small, latency-bound plans executed many times. */
+int mi_plan_balance(mi_plan *plan, double *out_device, int64_t out_len, void *stream);
 /* The same for one Coulomb/exchange execute (device arrays as in
  * mi_plan_jk_execute); ms[0] includes the Schwarz pass, ms[1] the zeroing
  * of J and K. */

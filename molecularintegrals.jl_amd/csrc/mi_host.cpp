@@ -387,6 +387,21 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
   return MI_OK;
 }
 
+// Issue order and stream lanes: launches by cost, longest first, each to the
+// least-loaded lane (LPT).
+static void assign_lanes(mi_plan* P) {
+  std::stable_sort(P->launches.begin(), P->launches.end(),
+                   [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
+  std::vector<double> load(P->nlanes, 0.0);
+  for (Launch& L : P->launches) {
+    int best = 0;
+    for (int k = 1; k < P->nlanes; ++k)
+      if (load[k] < load[best]) best = k;
+    L.lane = best;
+    load[best] += L.cost;
+  }
+}
+
 int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
   int rc;
   const mi_plan::HostData& h = P->h;
@@ -729,8 +744,6 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     L.cost = (double)L.nq_prim * (D.nvrr + D.nacc) + (double)L.nq_shell * (D.nhrr + D.nout) +
              2.0e4 * (D.nvlev + D.nhlev + 1);
   }
-  std::stable_sort(P->launches.begin(), P->launches.end(),
-                   [](const Launch& x, const Launch& y) { return x.cost > y.cost; });
   // Scratch of the global-table launches: the main launches may run
   // concurrently (stream lanes), so each gets its own slice; the Schwarz pass
   // runs before them on the caller's stream and reuses the same memory.
@@ -749,16 +762,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     P->scratch_doubles = std::max(off, soff);
   }
   P->nlanes = std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
-  {
-    std::vector<double> load(P->nlanes, 0.0);
-    for (Launch& L : P->launches) {
-      int best = 0;
-      for (int k = 1; k < P->nlanes; ++k)
-        if (load[k] < load[best]) best = k;
-      L.lane = best;
-      load[best] += L.cost;
-    }
-  }
+  assign_lanes(P);
   // Host images of the device buffers; uploaded now, or by the caller of a
   // deferred build (the serving engine admits a plan by its device bytes).
   {
@@ -1061,6 +1065,20 @@ static int profile_run(mi_plan* P, double* out, int64_t out_len, void* stream, f
 
 int mi_plan_profile(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms) {
   return profile_run(P, out, out_len, stream, ms, nullptr);
+}
+
+int mi_plan_balance(mi_plan* P, double* out, int64_t out_len, void* stream) {
+  if (!P) return MI_EINVAL;
+  std::vector<float> ms(P->launches.size() + 1);
+  int rc = profile_run(P, out, out_len, stream, ms.data(), nullptr);
+  if (rc) return rc;
+  for (size_t i = 0; i < P->launches.size(); ++i) P->launches[i].cost = std::max(ms[i + 1], 1e-6f);
+  assign_lanes(P);
+  if (P->gexec) {  // the captured graph has the old lanes
+    hipGraphExecDestroy(P->gexec);
+    P->gexec = nullptr;
+  }
+  return MI_OK;
 }
 
 int mi_plan_jk_profile(mi_plan* P, const double* D, double* J, double* K, void* stream, float* ms) {

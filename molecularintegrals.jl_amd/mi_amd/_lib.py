@@ -20,7 +20,7 @@ MI_PREC_FP64, MI_PREC_MIXED = 0, 1
 # Every symbol include/mi_hgp.h declares (checked by the CPU test suite).
 EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique", "mi_eri_unique",
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
-            "mi_plan_launch_info", "mi_plan_profile", "mi_plan_jk_profile", "mi_ctx_set_tuning", "mi_ctx_set_streams",
+            "mi_plan_launch_info", "mi_plan_profile", "mi_plan_jk_profile", "mi_plan_balance", "mi_ctx_set_tuning", "mi_ctx_set_streams",
             "mi_batch_plan_create", "mi_plan_out_len", "mi_plan_mol_offsets", "mi_plan_graph_capture",
             "mi_plan_graph_launch", "mi_jk", "mi_plan_jk_execute", "mi_plan_pair_bounds",
             "mi_server_create", "mi_server_destroy", "mi_server_last_error", "mi_server_submit", "mi_server_step",
@@ -112,6 +112,8 @@ def load():
     L.mi_plan_profile.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.POINTER(ctypes.c_float)]
     L.mi_plan_jk_profile.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_float)]
     L.mi_plan_jk_profile.restype = ctypes.c_int
+    L.mi_plan_balance.argtypes = [vp, vp, ctypes.c_int64, vp]
+    L.mi_plan_balance.restype = ctypes.c_int
     L.mi_plan_profile.restype = ctypes.c_int
     L.mi_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
     L.mi_ctx_set_tuning.restype = ctypes.c_int

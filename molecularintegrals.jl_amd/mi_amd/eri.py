@@ -176,6 +176,12 @@ class Plan:
                                                   ctypes.c_void_p(stream), ms))
         return [float(x) for x in ms]
 
+    def balance(self, out_ptr: int, stream: int = 0):
+        """One profiled execute, then stream lanes reassigned by measured
+        launch times (mi_plan_balance)."""
+        self.ctx.check(self.ctx.L.mi_plan_balance(self.h, ctypes.c_void_p(out_ptr), self.n_unique,
+                                                  ctypes.c_void_p(stream)))
+
     def jk_profile(self, D_ptr: int, J_ptr: int, K_ptr: int, stream: int = 0) -> list:
         """Per-launch durations (ms) of one Coulomb/exchange execute."""
         n = self.ctx.L.mi_plan_nlaunch(self.h) + 1
