@@ -183,29 +183,48 @@ def emit_rkernel(c):
     w(f"  double sA[{ncart(la)}], sB[{ncart(lb)}], sC[{ncart(lc)}], sD[{ncart(ld)}];")
     w(f"  load_scales<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, P, Q, sA, sB, sC, sD);")
     w("  const ScatterCtx sc = make_scatter(args, qc, P, Q);")
-    # Coulomb/exchange: contract the scaled block in registers, a few atomics per quartet
-    vals = []
+    # scaled block values, element e = ((ma nb + mb) nc + mc) nd + md
+    nout = 0
     for ma, a in enumerate(shell_aos(la)):
         for mb, b in enumerate(shell_aos(lb)):
             for mc, cc in enumerate(shell_aos(lc)):
                 for md, d in enumerate(shell_aos(ld)):
-                    vals.append(f"h{a}_{b}_{cc}_{d} * (sA[{ma}] * sB[{mb}]) * (sC[{mc}] * sD[{md}])")
+                    w(f"  const double o{nout} = h{a}_{b}_{cc}_{d} * (sA[{ma}] * sB[{mb}]) * (sC[{mc}] * sD[{md}]);")
+                    nout += 1
+    # Coulomb/exchange: contract the scaled block in registers, a few atomics per quartet
     w("  if (args.mode == MI_MODE_JK) {")
-    w(f"    const double ov[{len(vals)}] = {{")
-    for k in range(0, len(vals), 2):
-        w("      " + ", ".join(vals[k:k + 2]) + ",")
+    w(f"    const double ov[{nout}] = {{")
+    for k in range(0, nout, 8):
+        w("      " + ", ".join(f"o{e}" for e in range(k, min(nout, k + 8))) + ",")
     w("    };")
     w(f"    jk_block_regs<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, sc, ov);")
     w("    return;")
     w("  }")
-    idx = 0
-    for ma, a in enumerate(shell_aos(la)):
-        for mb, b in enumerate(shell_aos(lb)):
-            for mc, cc in enumerate(shell_aos(lc)):
-                for md, d in enumerate(shell_aos(ld)):
-                    w(f"  scatter_one(args, sc, {idx}, {ma}, {mb}, {mc}, {md}, "
-                      f"h{a}_{b}_{cc}_{d} * (sA[{ma}] * sB[{mb}]) * (sC[{mc}] * sD[{md}]));")
-                    idx += 1
+    # Scatter: the G lanes of the quartet hold the same block; lane g of W =
+    # min(G, 8) takes elements e = e0 + g (value picked by lane, no divergent
+    # per-element code), so a wavefront issues nout / W slot computations.
+    tmpl = f"<{ncart(lb)}, {ncart(lc)}, {ncart(ld)}>"
+    w("  const int gl = qc.g;")
+    w("  switch (qc.G) {  // uniform per wavefront (a wavefront holds one task)")
+    for W in (1, 2, 4, 8):
+        w(f"    {'default' if W == 8 else 'case ' + str(W)}: {{")
+        if W > 1:
+            w(f"      if (gl >= {W}) break;")
+        for e0 in range(0, nout, W):
+            k = min(W, nout - e0)
+            if W == 1:
+                w(f"      scatter_block_elem{tmpl}(args, sc, {e0}, o{e0});")
+                continue
+            sel = f"o{e0 + k - 1}"
+            for j in range(k - 2, -1, -1):
+                sel = f"gl == {j} ? o{e0 + j} : {sel}"
+            if k < W:
+                w(f"      if (gl < {k}) scatter_block_elem{tmpl}(args, sc, {e0} + gl, {sel});")
+            else:
+                w(f"      scatter_block_elem{tmpl}(args, sc, {e0} + gl, {sel});")
+        w("      break;")
+        w("    }")
+    w("  }")
     w("}")
     return "\n".join(L), g, h
 

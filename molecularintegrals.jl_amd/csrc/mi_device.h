@@ -526,6 +526,13 @@ __device__ __forceinline__ void jk_block_lds(const EriArgs& a, const ScatterCtx&
   }
 }
 
+// Element e of a block with NB, NC, ND functions on shells B, C, D.
+template <int NB, int NC, int ND>
+__device__ __forceinline__ void scatter_block_elem(const EriArgs& a, const ScatterCtx& s, int e, double v) {
+  const int md = e % ND, mc = (e / ND) % NC, mb = (e / (ND * NC)) % NB, ma = e / (ND * NC * NB);
+  scatter_elem(a, s, s.bra, s.fa0 + ma, s.fb0 + mb, s.fc0 + mc, s.fd0 + md, v);
+}
+
 __device__ __forceinline__ void scatter_one(const EriArgs& a, const ScatterCtx& s, int idx, int ma, int mb, int mc,
                                             int md, double val) {
   if ((idx & s.gmask) != s.g) return;
