@@ -300,6 +300,9 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
 // are persistent over the launch's quartets like the wave kernel.
 // LDS: [scales: 4*W_SCL][vrr: nvrr]
 // ---------------------------------------------------------------------------
+// HL: the contracted block and HRR levels also fit in LDS (after the VRR
+// area) -- then no global scratch is used.
+template <bool HL>
 __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const WClassDesc D,
                                                      const uint32_t* __restrict__ gblob,
                                                      double* __restrict__ scratch, int64_t nq_launch) {
@@ -310,7 +313,7 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
   const uint4* gH = reinterpret_cast<const uint4*>(gblob + D.gh);
   const uint2_t* gO = reinterpret_cast<const uint2_t*>(gblob + D.go);
   const uint2_t* gA = reinterpret_cast<const uint2_t*>(gblob + D.ga);
-  double* hrr = scratch + (size_t)blockIdx.x * D.nhrr;
+  double* hrr = HL ? vrr + D.nvrr : scratch + (size_t)blockIdx.x * D.nhrr;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
   const int mtop = D.mmax - 1;
@@ -525,7 +528,23 @@ hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, in
   return hipLaunchKernel(w_kernel_fn(c), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
-size_t wgclass_lds_bytes(int c) { return sizeof(double) * (size_t)(4 * W_SCL + kWClassDesc[c].nvrr); }
+// The global-table kernel keeps its HRR area in LDS too when the whole
+// block's LDS stays within this (several blocks per CU remain resident).
+#define WG_HRR_LDS_MAX (64 * 1024)
+
+static bool wg_hrr_in_lds(int c) {
+  const WClassDesc& D = kWClassDesc[c];
+  return sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + D.nhrr) <= WG_HRR_LDS_MAX;
+}
+
+static const void* wg_kernel_fn(int c) {
+  return wg_hrr_in_lds(c) ? (const void*)hgp_wg_kernel<true> : (const void*)hgp_wg_kernel<false>;
+}
+
+size_t wgclass_lds_bytes(int c) {
+  const WClassDesc& D = kWClassDesc[c];
+  return sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + (wg_hrr_in_lds(c) ? D.nhrr : 0));
+}
 
 bool wgclass_fits_nw(int c, int nw, size_t lds_limit) {
   return (nw == 1 || nw == 2 || nw == 4 || nw == 8) && wgclass_lds_bytes(c) <= lds_limit;
@@ -533,21 +552,21 @@ bool wgclass_fits_nw(int c, int nw, size_t lds_limit) {
 
 int wg_kernel_blocks_per_cu(int c, int nw) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)hgp_wg_kernel, 64 * nw,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, wg_kernel_fn(c), 64 * nw,
                                                    wgclass_lds_bytes(c)) != hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;
 }
 
-size_t wg_scratch_doubles(int c, int nblocks) { return (size_t)nblocks * kWClassDesc[c].nhrr; }
+size_t wg_scratch_doubles(int c, int nblocks) { return wg_hrr_in_lds(c) ? 0 : (size_t)nblocks * kWClassDesc[c].nhrr; }
 
 hipError_t launch_class_wave_global(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch,
                                     void* wtables, double* scratch, hipStream_t stream) {
   const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
   const WClassDesc D = kWClassDesc[c];
-  hipLaunchKernelGGL(hgp_wg_kernel, dim3(nblocks), dim3(64 * nw), wgclass_lds_bytes(c), stream, args, D, t->gblob,
-                     scratch, nq_launch);
-  return hipGetLastError();
+  const uint32_t* gb = t->gblob;
+  void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<WClassDesc*>(&D), &gb, &scratch, &nq_launch};
+  return hipLaunchKernel(wg_kernel_fn(c), dim3(nblocks), dim3(64 * nw), kargs, wgclass_lds_bytes(c), stream);
 }
 
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream) {
@@ -571,7 +590,10 @@ hipError_t set_wave_kernel_lds_limit(size_t bytes) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
   }
-  return hipFuncSetAttribute((const void*)hgp_wg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = hipFuncSetAttribute((const void*)hgp_wg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)hgp_wg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 }  // namespace mi
