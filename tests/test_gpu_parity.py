@@ -294,13 +294,16 @@ def test_batch_plan_matches_each_molecule(ctx):
         plan.close()
 
 
-def test_graph_replay_equals_eager(ctx):
+@pytest.mark.parametrize("form", ["fp64", "mixed_screened"])
+def test_graph_replay_equals_eager(ctx, form):
     """The captured execute, replayed several times with eager work and
-    synchronisations between replays, reproduces the eager output bit for bit."""
+    synchronisations between replays, reproduces the eager output bit for bit
+    (the screened form includes the output zero-fill kernel in the graph)."""
     import torch
     from mi_amd import configs
     mols = configs.batch_molecules(6, nmol=4, natoms_lo=3, natoms_hi=7)
-    plan = ctx.batch_plan([m.flat() for m in mols])
+    kw = {} if form == "fp64" else dict(precision=1, tau=1e-8, schwarz_cut=1e-10)
+    plan = ctx.batch_plan([m.flat() for m in mols], **kw)
     s = torch.cuda.current_stream()
     eager = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
     plan.execute(eager.data_ptr(), s.cuda_stream)
@@ -454,3 +457,44 @@ def test_balance_keeps_results(ctx):
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), ref)
     plan.close()
+
+
+def test_batched_screened_equals_single(ctx):
+    """Screening and FP32 selection work per molecule: a molecule's slice of a
+    screened mixed batch equals its own screened mixed plan bit for bit, and
+    stays within the cut of the FP64 result."""
+    import torch
+    from mi_amd import configs
+    mols = configs.batch_molecules(12, nmol=5, natoms_lo=4, natoms_hi=9)
+    fbs = [m.flat() for m in mols]
+    kw = dict(precision=1, tau=1e-8, schwarz_cut=1e-11)
+    plan = ctx.batch_plan(fbs, **kw)
+    out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+    plan.execute(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    off = plan.mol_offsets()
+    got = out.cpu().numpy()
+    for m, fb in enumerate(fbs):
+        alone = ctx.eri_unique(fb, **kw)
+        assert np.array_equal(got[off[m]:off[m + 1]], alone), f"molecule {m}"
+        ref = ctx.eri_unique(fb)
+        assert np.abs(alone - ref).max() <= 1e-11 + 1e-10 * 1e-3
+    plan.close()
+
+
+def test_single_shell_and_size_limits(ctx):
+    """One s shell (one integral), one f shell alone (all (ff|ff) of one
+    centre), and a basis beyond 65535 functions per molecule (rejected with
+    MI_EUNSUPPORTED before any device work)."""
+    from mi_amd import flat_from_shells
+    from mi_amd._lib import MI_EUNSUPPORTED, MiError
+    fb = flat_from_shells([((0.1, 0.2, 0.3), 0, [1.3], [1.0])])
+    got = ctx.eri_unique(fb)
+    ref = O.eri_unique_hgp(fb)
+    assert got.shape == (1,) and abs(got[0] - ref[0]) <= 1e-13
+    fb = flat_from_shells([((0.0, 0.0, 0.0), 3, [0.8, 0.3], [0.6, 0.5])])
+    assert_eri_close(ctx.eri_unique(fb), O.eri_unique_hgp(fb), "one f shell", fb, O.ORDER_CHRR)
+    big = flat_from_shells([((0.0, 0.0, 0.1 * k), 3, [1.0], [1.0]) for k in range(6554)])
+    assert big.nbf == 65540
+    with pytest.raises(MiError) as e:
+        ctx.plan(big)
+    assert e.value.code == MI_EUNSUPPORTED
