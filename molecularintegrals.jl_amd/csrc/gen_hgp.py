@@ -127,16 +127,24 @@ def emit_rkernel(c):
     for (e, f) in g.targets:
         w(f"  double acc{e}_{f} = 0.0;")
     w(f"  const int nu = P.npp, nv = Q.npp, ntot = nu * nv;")
+    w(f"  const double Ax = args.shell_xyz[3 * P.shA], Ay = args.shell_xyz[3 * P.shA + 1], "
+      f"Az = args.shell_xyz[3 * P.shA + 2];")
+    w(f"  const double Cx = args.shell_xyz[3 * Q.shA], Cy = args.shell_xyz[3 * Q.shA + 1], "
+      f"Cz = args.shell_xyz[3 * Q.shA + 2];")
     w(f"  const int G = qc.G;")
     w(f"  int u = qc.g / Q.npp, v = qc.g - (qc.g / Q.npp) * Q.npp;")
     w(f"  const int du = G / Q.npp, dv = G - du * Q.npp;")
+    # primitive pairs of the next iteration are loaded while this one computes
+    w(f"  PrimPair bpn, kpn;")
+    w(f"  if (qc.g < ntot) {{ bpn = args.pp[P.ppoff + u]; kpn = args.pp[Q.ppoff + v]; }}")
     w(f"  for (int t = qc.g; t < ntot; t += G) {{")
-    w(f"    const PrimPair bp = args.pp[P.ppoff + u];")
-    w(f"    const PrimPair kp = args.pp[Q.ppoff + v];")
+    w(f"    const PrimPair bp = bpn, kp = kpn;")
     w(f"    v += dv; u += du; if (v >= nv) {{ v -= nv; ++u; }}")
+    w(f"    if (t + G < ntot) {{ bpn = args.pp[P.ppoff + u]; kpn = args.pp[Q.ppoff + v]; }}")
     w(f"    const double zeta = bp.zeta, eta = kp.zeta;")
     w(f"    const double inv_s = 1.0 / (zeta + eta);")
-    w(f"    const double PQx = bp.Px - kp.Px, PQy = bp.Py - kp.Py, PQz = bp.Pz - kp.Pz;")
+    w(f"    const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),")
+    w(f"                 PQz = (Az + bp.PAz) - (Cz + kp.PAz);")
     w(f"    const double T = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);")
     w(f"    const double pref = bp.K * kp.K * sqrt(inv_s);")
     w(f"    double F[{mmax}];")

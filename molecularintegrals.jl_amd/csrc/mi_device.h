@@ -37,10 +37,12 @@ struct PairGeom {
   double pad;
 };
 
-struct __attribute__((aligned(16))) PrimPair {
-  double Px, Py, Pz, zeta;
-  double PAx, PAy, PAz, K;   // K = sqrt(2) pi^{5/4} c'_a c'_b exp(-ab|AB|^2/zeta) / zeta
-  double oo2z, pad0, pad1, pad2;
+// One primitive pair, one 64-byte line: P itself is not stored, kernels form
+// P = A + PA with the pair's first centre A exactly as the pair kernel does.
+struct __attribute__((aligned(64))) PrimPair {
+  double PAx, PAy, PAz, zeta;
+  double K;                  // K = sqrt(2) pi^{5/4} c'_a c'_b exp(-ab|AB|^2/zeta) / zeta
+  double oo2z, pad0, pad1;
 };
 
 struct TaskDesc {
@@ -67,10 +69,12 @@ struct EriArgs {
   const PairTopo* topo;
   PairGeom* geom;
   const PrimPair* pp;
+  const double* shell_xyz;  // shell centres (P = A + PA of a primitive pair)
   const double* bf_scale;
   const double* boys;
   double* out;
   const TaskDesc* tasks;
+  const int32_t* block_task;  // register kernels: task of each block's first wavefront
   int32_t order;       // 0 iindex, 1 chrr
   int32_t ntasks;
   int32_t mode;        // MI_MODE_*
@@ -251,10 +255,12 @@ __device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, Quar
 // wavefront), so a wavefront never spans two tasks, lane groups stay aligned
 // whatever each task's G, and blocks stay full across task boundaries.
 __device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc) {
-  const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x >> 6);  // block-uniform: scalar search
+  const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x >> 6);
   const int64_t w = w0 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const TaskDesc T = a.tasks[advance_task(a, find_task(a, w0), w)];
+  // the task of the block's first wavefront (host table, else a block-uniform search)
+  const int t0 = a.block_task ? a.block_task[blockIdx.x] : find_task(a, w0);
+  const TaskDesc T = a.tasks[advance_task(a, t0, w)];
   const int G = T.G;
   const int64_t qi = ((w - T.blockStart) * 64 + lane) / G;
   qc.G = G;

@@ -207,7 +207,8 @@ void free_plan_buffers(mi_plan* p) {
   for (auto& e : p->ev_join)
     if (e) hipEventDestroy(e);
   void* bufs[] = {p->d_shell_xyz, p->d_shell_prim_off, p->d_prim_exp, p->d_prim_cn, p->d_bf_scale, p->d_topo,
-                  p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch,   p->d_bounds};
+                  p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch,   p->d_bounds,
+                  p->d_block_task};
   for (void* b : bufs) {
     if (!b) continue;
     if (p->alloc_stream)
@@ -363,7 +364,18 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
         qs += t.nq;
         all_tasks.push_back(t);
       }
-      if (reg) blocks = (waves + 3) / 4;
+      if (reg) {
+        // block -> task of its first wavefront (one load in the kernel
+        // instead of a search)
+        blocks = (waves + 3) / 4;
+        L.bt_off = (int64_t)P->h.block_task.size();
+        int ti = 0;
+        for (int64_t b = 0; b < blocks; ++b) {
+          const int64_t w0 = 4 * b;
+          while (ti + 1 < L.ntasks && all_tasks[L.task_off + ti + 1].blockStart <= w0) ++ti;
+          P->h.block_task.push_back(ti);
+        }
+      }
       if (var == 0) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
       if (var == 2) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wg_kernel_blocks_per_cu(c, nw));
       if (blocks > INT32_MAX || qs > INT32_MAX || waves > INT32_MAX) {
@@ -418,7 +430,8 @@ int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
       (rc = upload(P, &P->d_pp_pair, h.pp_pair.data(), h.pp_pair.size())) ||
       (rc = upload(P, &P->d_tasks, h.tasks.data(), h.tasks.size())) ||
       (P->scratch_doubles && (rc = upload<double>(P, &P->d_scratch, nullptr, P->scratch_doubles))) ||
-      (!h.bounds.empty() && (rc = upload(P, &P->d_bounds, h.bounds.data(), h.bounds.size()))))
+      (!h.bounds.empty() && (rc = upload(P, &P->d_bounds, h.bounds.data(), h.bounds.size()))) ||
+      (!h.block_task.empty() && (rc = upload(P, &P->d_block_task, h.block_task.data(), h.block_task.size()))))
     return rc;
   return MI_OK;
 }
@@ -778,6 +791,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     auto nb = [](size_t n, size_t sz) { return sz * std::max<size_t>(n, 1); };
     P->device_bytes = nb(h.xyz.size(), 8) + nb(h.poff.size(), 4) + nb(h.pexp.size(), 8) + nb(h.pcn.size(), 8) +
                       nb(h.bfs.size(), 8) + nb(h.topo.size(), sizeof(PairTopo)) + nb(npair, sizeof(PairGeom)) +
+                      nb(h.block_task.size(), 4) +
                       nb(P->npp, sizeof(PrimPair)) + nb(h.pp_pair.size(), 4) + nb(h.tasks.size(), sizeof(TaskDesc)) +
                       (P->scratch_doubles ? 8 * P->scratch_doubles : 0) + 16 * h.bounds.size();
   }
@@ -826,6 +840,7 @@ int mi_plan_mol_offsets(const mi_plan* P, int64_t* offsets) {
 static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P, hipStream_t ls) {
   a.tasks = P->d_tasks + L.task_off;
   a.ntasks = L.ntasks;
+  a.block_task = L.var == 1 ? P->d_block_task + L.bt_off : nullptr;
   if (L.var == 1)
     HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
   else if (L.var == 0)
@@ -857,6 +872,7 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
   a.topo = P->d_topo;
   a.geom = P->d_geom;
   a.pp = P->d_pp;
+  a.shell_xyz = P->d_shell_xyz;
   a.bf_scale = P->d_bf_scale;
   a.boys = ctx->d_boys;
   a.out = out;

@@ -41,6 +41,7 @@ struct Launch {
   int var;  // 1 register kernel, 0 LDS wave kernel, 2 global-table kernel
   int nw;   // wave kernels: wavefronts per block
   size_t scratch_off;  // global-table kernel: offset of its slice of the plan scratch (doubles)
+  int64_t bt_off;      // register kernel: offset of its block -> task table in the plan's
   int task_off, ntasks, nblocks;
   int64_t nq_shell, nq_prim;
   double cost;  // relative device time estimate (longest launches go first)
@@ -83,6 +84,7 @@ struct mi_plan {
   hipStream_t alloc_stream = nullptr;
   bool bounds_only = false;         // Schwarz pass only (mi_plan_build with MI_BUILD_BOUNDS_ONLY)
   double2* d_bounds = nullptr;      // per pair (Q, V) computed when the plan was built, or null
+  int32_t* d_block_task = nullptr;  // register launches: task of each block's first wavefront
   size_t device_bytes = 0;  // bytes of device memory the plan holds (or needs, before upload)
   struct HostData {         // host images of the uploaded buffers
     std::vector<double> xyz, pexp, pcn, bfs;
@@ -90,6 +92,7 @@ struct mi_plan {
     std::vector<PairTopo> topo;
     std::vector<TaskDesc> tasks;
     std::vector<double2> bounds;
+    std::vector<int32_t> block_task;
   } h;
 };
 

@@ -44,21 +44,20 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
   const double zeta = a + b, ooz = 1.0 / zeta;
   const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
   const double rab2 = ABx * ABx + ABy * ABy + ABz * ABz;
-  // PA = P - A = -(b/zeta) AB and P = A + PA: exactly A for one-centre pairs,
-  // so one-centre quartets keep PA = QC = W - P = 0 without rounding residue
-  // (the textbook (aA + bB)/zeta leaves ~1e-16|A| that tight-exponent
-  // prefactors amplify past 1e-12).
+  // PA = P - A = -(b/zeta) AB and P = A + PA (formed by the kernels): exactly
+  // A for one-centre pairs, so one-centre quartets keep PA = QC = W - P = 0
+  // without rounding residue (the textbook (aA + bB)/zeta leaves ~1e-16|A|
+  // that tight-exponent prefactors amplify past 1e-12).
   const double bz = b * ooz;
   const double PAx = -bz * ABx, PAy = -bz * ABy, PAz = -bz * ABz;
-  const double Px = Ax + PAx, Py = Ay + PAy, Pz = Az + PAz;
   // 2 pi^{5/2} / (zeta eta sqrt(zeta+eta)) exp(...) split as K_ab K_cd / sqrt(zeta+eta)
   const double kfac = 5.9149671727956129;  // sqrt(2) * pi^{5/4}
   PrimPair o;
-  o.Px = Px; o.Py = Py; o.Pz = Pz; o.zeta = zeta;
+  o.zeta = zeta;
   o.PAx = PAx; o.PAy = PAy; o.PAz = PAz;
   o.K = kfac * prim_cn[pa] * prim_cn[pb] * exp(-a * b * rab2 * ooz) * ooz;
   o.oo2z = 0.5 * ooz;
-  o.pad0 = o.pad1 = o.pad2 = 0.0;
+  o.pad0 = o.pad1 = 0.0;
   pp[r] = o;
   if (local == 0) {
     PairGeom g;
@@ -176,6 +175,8 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     if (qmode == 2) continue;                   // uniform per block (output pre-zeroed)
     const PairTopo P = args.topo[qc.bra];
     const PairTopo Q = args.topo[qc.ket];
+    const double Ax = args.shell_xyz[3 * P.shA], Ay = args.shell_xyz[3 * P.shA + 1], Az = args.shell_xyz[3 * P.shA + 2];
+    const double Cx = args.shell_xyz[3 * Q.shA], Cy = args.shell_xyz[3 * Q.shA + 1], Cz = args.shell_xyz[3 * Q.shA + 2];
     if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
     if (tid < nb) scl[W_SCL + tid] = args.bf_scale[P.bfB + tid];
     if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
@@ -190,7 +191,8 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
       const double inv_s = 1.0 / (zeta + eta);
-      const double PQx = bp.Px - kp.Px, PQy = bp.Py - kp.Py, PQz = bp.Pz - kp.Pz;
+      const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),
+                   PQz = (Az + bp.PAz) - (Cz + kp.PAz);
       const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
       const double pref = bp.K * kp.K * sqrt(inv_s);
       // Boys: lane m (< vt0) evaluates the top order and recurses down to m.
@@ -330,6 +332,8 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
     if (qmode == 2) continue;                   // output pre-zeroed
     const PairTopo P = args.topo[qc.bra];
     const PairTopo Q = args.topo[qc.ket];
+    const double Ax = args.shell_xyz[3 * P.shA], Ay = args.shell_xyz[3 * P.shA + 1], Az = args.shell_xyz[3 * P.shA + 2];
+    const double Cx = args.shell_xyz[3 * Q.shA], Cy = args.shell_xyz[3 * Q.shA + 1], Cz = args.shell_xyz[3 * Q.shA + 2];
     if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
     if (tid < nb) scl[W_SCL + tid] = args.bf_scale[P.bfB + tid];
     if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
@@ -341,7 +345,8 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
       const double inv_s = 1.0 / (zeta + eta);
-      const double PQx = bp.Px - kp.Px, PQy = bp.Py - kp.Py, PQz = bp.Pz - kp.Pz;
+      const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),
+                   PQz = (Az + bp.PAz) - (Cz + kp.PAz);
       const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
       const double pref = bp.K * kp.K * sqrt(inv_s);
       if (tid < D.vt0) vrr[tid] = pref * boys_lane(btab, T2, mtop, tid);
