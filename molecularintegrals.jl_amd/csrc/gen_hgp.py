@@ -134,13 +134,10 @@ def emit_rkernel(c):
     w(f"  const int G = qc.G;")
     w(f"  int u = qc.g / Q.npp, v = qc.g - (qc.g / Q.npp) * Q.npp;")
     w(f"  const int du = G / Q.npp, dv = G - du * Q.npp;")
-    # primitive pairs of the next iteration are loaded while this one computes
-    w(f"  PrimPair bpn, kpn;")
-    w(f"  if (qc.g < ntot) {{ bpn = args.pp[P.ppoff + u]; kpn = args.pp[Q.ppoff + v]; }}")
     w(f"  for (int t = qc.g; t < ntot; t += G) {{")
-    w(f"    const PrimPair bp = bpn, kp = kpn;")
+    w(f"    const PrimPair bp = args.pp[P.ppoff + u];")
+    w(f"    const PrimPair kp = args.pp[Q.ppoff + v];")
     w(f"    v += dv; u += du; if (v >= nv) {{ v -= nv; ++u; }}")
-    w(f"    if (t + G < ntot) {{ bpn = args.pp[P.ppoff + u]; kpn = args.pp[Q.ppoff + v]; }}")
     w(f"    const double zeta = bp.zeta, eta = kp.zeta;")
     w(f"    const double inv_s = 1.0 / (zeta + eta);")
     w(f"    const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),")
