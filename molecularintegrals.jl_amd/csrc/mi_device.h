@@ -26,7 +26,7 @@ struct PairTopo {
   int32_t shA, shB;   // canonical shells (L_A >= L_B)
   int32_t bfA, bfB;   // first function of each shell (0-based)
   int32_t ppoff, npp; // primitive-pair range [ppoff, ppoff+npp)
-  int32_t nprimB;     // primitives of shell B (row length of the pair's prim block)
+  int32_t unused;
   int32_t pidx;       // canonical pair index tri(I,J) of the original shells
 };
 

@@ -208,7 +208,7 @@ void free_plan_buffers(mi_plan* p) {
     if (e) hipEventDestroy(e);
   void* bufs[] = {p->d_shell_xyz, p->d_shell_prim_off, p->d_prim_exp, p->d_prim_cn, p->d_bf_scale, p->d_topo,
                   p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch,   p->d_bounds,
-                  p->d_block_task};
+                  p->d_block_task, p->d_pp_ab};
   for (void* b : bufs) {
     if (!b) continue;
     if (p->alloc_stream)
@@ -428,6 +428,7 @@ int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
       (rc = upload<PairGeom>(P, &P->d_geom, nullptr, (size_t)P->npair)) ||
       (rc = upload<PrimPair>(P, &P->d_pp, nullptr, (size_t)P->npp)) ||
       (rc = upload(P, &P->d_pp_pair, h.pp_pair.data(), h.pp_pair.size())) ||
+      (rc = upload(P, &P->d_pp_ab, h.pp_ab.data(), h.pp_ab.size())) ||
       (rc = upload(P, &P->d_tasks, h.tasks.data(), h.tasks.size())) ||
       (P->scratch_doubles && (rc = upload<double>(P, &P->d_scratch, nullptr, P->scratch_doubles))) ||
       (!h.bounds.empty() && (rc = upload(P, &P->d_bounds, h.bounds.data(), h.bounds.size()))) ||
@@ -525,19 +526,43 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     np[s2] = poff[s2 + 1] - poff[s2];
   }
   // Shell pairs (I >= J within a molecule), canonical shell order L_A >= L_B.
+  // Primitive pairs whose prefactor |K_ab| is below MI_PRIM_PAIR_CUT times the
+  // largest of their shell pair are dropped (22% of them are exactly zero --
+  // exp(-ab|AB|^2/zeta) underflows -- and the rest contribute below FP64
+  // resolution); the kept (ia, ib) of a pair are listed in pp_ab.
   struct PairKey {
     int mol, pc, bucket, pidx, A, B, K;
     double Q, V;
+    int64_t ab0;  // first kept (ia, ib) code in ab_codes
   };
   std::vector<PairKey> keys;
+  std::vector<int32_t> ab_codes;
+  std::vector<double> kk;
   for (int m = 0; m < nmol; ++m)
     for (int I = mol_sh0[m]; I < mol_sh0[m + 1]; ++I)
       for (int J = mol_sh0[m]; J <= I; ++J) {
         int A = I, B = J;
         if (L[J] > L[I]) std::swap(A, B);
-        const int K = np[A] * np[B];
+        double r2 = 0.0;
+        for (int k = 0; k < 3; ++k) r2 += (xyz[3 * A + k] - xyz[3 * B + k]) * (xyz[3 * A + k] - xyz[3 * B + k]);
+        kk.assign((size_t)np[A] * np[B], 0.0);
+        double kmax = 0.0;
+        for (int ia = 0; ia < np[A]; ++ia)
+          for (int ib = 0; ib < np[B]; ++ib) {
+            const double a = pexp[poff[A] + ia], b = pexp[poff[B] + ib];
+            const double v = std::fabs(pcn[poff[A] + ia] * pcn[poff[B] + ib] * std::exp(-a * b / (a + b) * r2) / (a + b));
+            kk[(size_t)ia * np[B] + ib] = v;
+            kmax = std::max(kmax, v);
+          }
+        const int64_t ab0 = (int64_t)ab_codes.size();
+        for (int ia = 0; ia < np[A]; ++ia)
+          for (int ib = 0; ib < np[B]; ++ib) {
+            const double v = kk[(size_t)ia * np[B] + ib];
+            if (v > MI_PRIM_PAIR_CUT * kmax || v == kmax) ab_codes.push_back(ia | (ib << 16));
+          }
+        const int K = (int)((int64_t)ab_codes.size() - ab0);
         const int i = I - mol_sh0[m], j = J - mol_sh0[m];
-        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0});
+        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0, ab0});
       }
   std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
     if (x.mol != y.mol) return x.mol < y.mol;
@@ -565,12 +590,15 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   }
   const int npair = (int)keys.size();
   std::vector<PairTopo> topo(npair);
-  std::vector<int32_t> pp_pair;
+  std::vector<int32_t> pp_pair, pp_ab;
   int64_t ppoff = 0;
   for (int r = 0; r < npair; ++r) {
     const PairKey& k = keys[r];
-    topo[r] = {k.A, k.B, bfoff[k.A], bfoff[k.B], (int32_t)ppoff, k.K, np[k.B], k.pidx};
-    for (int t = 0; t < k.K; ++t) pp_pair.push_back(r);
+    topo[r] = {k.A, k.B, bfoff[k.A], bfoff[k.B], (int32_t)ppoff, k.K, 0, k.pidx};
+    for (int t = 0; t < k.K; ++t) {
+      pp_pair.push_back(r);
+      pp_ab.push_back(ab_codes[k.ab0 + t]);
+    }
     ppoff += k.K;
   }
   if (ppoff > INT32_MAX) {
@@ -787,12 +815,14 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     h.bfs = std::move(bfs);
     h.topo = std::move(topo);
     h.pp_pair = std::move(pp_pair);
+    h.pp_ab = std::move(pp_ab);
     h.tasks = std::move(all_tasks);
     auto nb = [](size_t n, size_t sz) { return sz * std::max<size_t>(n, 1); };
     P->device_bytes = nb(h.xyz.size(), 8) + nb(h.poff.size(), 4) + nb(h.pexp.size(), 8) + nb(h.pcn.size(), 8) +
                       nb(h.bfs.size(), 8) + nb(h.topo.size(), sizeof(PairTopo)) + nb(npair, sizeof(PairGeom)) +
                       nb(h.block_task.size(), 4) +
-                      nb(P->npp, sizeof(PrimPair)) + nb(h.pp_pair.size(), 4) + nb(h.tasks.size(), sizeof(TaskDesc)) +
+                      nb(P->npp, sizeof(PrimPair)) + nb(h.pp_pair.size(), 4) + nb(h.pp_ab.size(), 4) +
+                      nb(h.tasks.size(), sizeof(TaskDesc)) +
                       (P->scratch_doubles ? 8 * P->scratch_doubles : 0) + 16 * h.bounds.size();
   }
   if (!(flags & MI_BUILD_DEFER_UPLOAD) && (rc = mi_plan_upload(P, nullptr))) {
@@ -866,7 +896,7 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
   }
   hipStream_t s = (hipStream_t)stream;
   if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], s));
-  HIP_TRY(ctx, mi::launch_pair_prims(P->npp, P->d_pp_pair, P->d_topo, P->d_shell_xyz, P->d_shell_prim_off,
+  HIP_TRY(ctx, mi::launch_pair_prims(P->npp, P->d_pp_pair, P->d_pp_ab, P->d_topo, P->d_shell_xyz, P->d_shell_prim_off,
                                      P->d_prim_exp, P->d_prim_cn, P->d_pp, P->d_geom, P->d_bounds, s));
   EriArgs a{};
   a.topo = P->d_topo;

@@ -13,6 +13,7 @@
 #define MI_NBUCKET 5
 #define MI_NREGIME 2             // tuning regimes: small and large class launches
 #define MI_SMALL_LAUNCH 65536    // shell quartets below which a class launch is "small" (latency-bound)
+#define MI_PRIM_PAIR_CUT 1e-30   // primitive pairs with |K| below this times their shell pair's largest are dropped
 #define MI_MAX_STREAMS 16
 
 // Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
@@ -65,6 +66,7 @@ struct mi_plan {
   PairGeom* d_geom = nullptr;
   PrimPair* d_pp = nullptr;
   int32_t* d_pp_pair = nullptr;
+  int32_t* d_pp_ab = nullptr;       // (ia | ib << 16) of each kept primitive pair
   TaskDesc* d_tasks = nullptr;
   double* d_scratch = nullptr;     // HRR work areas of the global-table launches
   size_t scratch_doubles = 0;
@@ -88,7 +90,7 @@ struct mi_plan {
   size_t device_bytes = 0;  // bytes of device memory the plan holds (or needs, before upload)
   struct HostData {         // host images of the uploaded buffers
     std::vector<double> xyz, pexp, pcn, bfs;
-    std::vector<int32_t> poff, pp_pair;
+    std::vector<int32_t> poff, pp_pair, pp_ab;
     std::vector<PairTopo> topo;
     std::vector<TaskDesc> tasks;
     std::vector<double2> bounds;

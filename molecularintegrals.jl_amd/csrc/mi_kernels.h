@@ -14,7 +14,8 @@ bool has_register_kernel(int c);
 hipError_t upload_wtables(void** handle);
 void free_wtables(void* handle);
 
-hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const PairTopo* topo, const double* shell_xyz,
+hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const int32_t* pp_ab, const PairTopo* topo,
+                             const double* shell_xyz,
                              const int32_t* shell_prim_off, const double* prim_exp, const double* prim_cn,
                              PrimPair* pp, PairGeom* geom, const double2* bounds, hipStream_t stream);
 hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStream_t stream);

@@ -24,6 +24,7 @@
 // src/HGP.jl:139-155, hoisted out of the quartet loop).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t* __restrict__ pp_pair,
+                                                         const int32_t* __restrict__ pp_ab,
                                                          const PairTopo* __restrict__ topo,
                                                          const double* __restrict__ shell_xyz,
                                                          const int32_t* __restrict__ shell_prim_off,
@@ -36,7 +37,7 @@ __global__ void __launch_bounds__(256) pair_prims_kernel(int npp, const int32_t*
   const int p = pp_pair[r];
   const PairTopo t = topo[p];
   const int local = r - t.ppoff;
-  const int ia = local / t.nprimB, ib = local - ia * t.nprimB;
+  const int ia = pp_ab[r] & 0xFFFF, ib = pp_ab[r] >> 16;
   const int pa = shell_prim_off[t.shA] + ia, pb = shell_prim_off[t.shB] + ib;
   const double a = prim_exp[pa], b = prim_exp[pb];
   const double Ax = shell_xyz[3 * t.shA], Ay = shell_xyz[3 * t.shA + 1], Az = shell_xyz[3 * t.shA + 2];
@@ -479,12 +480,13 @@ void free_wtables(void* handle) {
   delete t;
 }
 
-hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const PairTopo* topo, const double* shell_xyz,
+hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const int32_t* pp_ab, const PairTopo* topo,
+                             const double* shell_xyz,
                              const int32_t* shell_prim_off, const double* prim_exp, const double* prim_cn,
                              PrimPair* pp, PairGeom* geom, const double2* bounds, hipStream_t stream) {
   if (npp == 0) return hipSuccess;
   dim3 grid((npp + 255) / 256), block(256);
-  hipLaunchKernelGGL(pair_prims_kernel, grid, block, 0, stream, npp, pp_pair, topo, shell_xyz, shell_prim_off,
+  hipLaunchKernelGGL(pair_prims_kernel, grid, block, 0, stream, npp, pp_pair, pp_ab, topo, shell_xyz, shell_prim_off,
                      prim_exp, prim_cn, pp, geom, bounds);
   return hipGetLastError();
 }
