@@ -169,7 +169,7 @@ def leg_jk(seed=0):
     (no screening) vs screening at 1e-12 vs mixed (FP32 VRR below 1e-8) with
     screening; errors of J and K against the plain FP64 GPU result."""
     import torch
-    from mi_amd import Context, configs
+    from mi_amd import Context, _lib, configs
 
     wl, D = configs.cluster_40(seed)
     fb = wl.flat()
@@ -184,7 +184,7 @@ def leg_jk(seed=0):
     res, mats = {}, {}
     for name, kw, reps in (("plain_fp64", {}, 1), ("fp64_screened", dict(schwarz_cut=1e-12), 2),
                            ("mixed_screened", dict(precision=1, tau=1e-8, schwarz_cut=1e-12), 2)):
-        plan = ctx.plan(fb, order=0, **kw)
+        plan = ctx.plan(fb, order=_lib.MI_ORDER_JK, **kw)
         ms = timed(lambda: plan.jk_execute(dD.data_ptr(), J.data_ptr(), K.data_ptr(), sp), reps, st)
         mats[name] = (J.cpu().numpy(), K.cpu().numpy())
         res[name] = {"ms": ms, "value": nu / (ms * 1e-3)}

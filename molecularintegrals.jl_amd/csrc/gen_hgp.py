@@ -119,14 +119,21 @@ def emit_rkernel(c):
     w(f"  __shared__ double sboys[BOYS_NROW * 8];")
     w(f"  stage_boys_slice(sboys, args.boys, {mmax - 1});")
     w(f"  QuartetCtx qc;")
-    w(f"  if (!decode_quartet(args, qc)) return;")
+    # Lanes without a quartet (past the task's end, or screened) return, except
+    # in the J/K build: there every lane joins the wavefront's run reduction
+    # (jk_block_regs) with a zero block.
+    w(f"  bool live = decode_quartet(args, qc);")
+    w(f"  if (!live && args.mode != MI_MODE_JK) return;")
     w(f"  const PairTopo P = args.topo[qc.bra];")
     w(f"  const PairTopo Q = args.topo[qc.ket];")
-    w(f"  const int qmode = quartet_mode(args, qc);  // 0 FP64 VRR, 1 FP32 VRR, 2 screened")
-    w("  if (qmode == 2) return;  // screened: output pre-zeroed")
+    w(f"  const int qmode = live ? quartet_mode(args, qc) : 2;  // 0 FP64 VRR, 1 FP32 VRR, 2 screened")
+    w("  if (qmode == 2) {  // screened: output pre-zeroed")
+    w("    if (args.mode != MI_MODE_JK) return;")
+    w("    live = false;")
+    w("  }")
     for (e, f) in g.targets:
         w(f"  double acc{e}_{f} = 0.0;")
-    w(f"  const int nu = P.npp, nv = Q.npp, ntot = nu * nv;")
+    w(f"  const int nu = P.npp, nv = Q.npp, ntot = live ? nu * nv : 0;")
     w(f"  const double Ax = args.shell_xyz[3 * P.shA], Ay = args.shell_xyz[3 * P.shA + 1], "
       f"Az = args.shell_xyz[3 * P.shA + 2];")
     w(f"  const double Cx = args.shell_xyz[3 * Q.shA], Cy = args.shell_xyz[3 * Q.shA + 1], "
@@ -202,7 +209,7 @@ def emit_rkernel(c):
     for k in range(0, nout, 8):
         w("      " + ", ".join(f"o{e}" for e in range(k, min(nout, k + 8))) + ",")
     w("    };")
-    w(f"    jk_block_regs<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, sc, ov);")
+    w(f"    jk_block_regs<{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}>(args, sc, live, ov);")
     w("    return;")
     w("  }")
     # Scatter: the G lanes of the quartet hold the same block; lane g of W =

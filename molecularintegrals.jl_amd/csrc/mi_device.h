@@ -64,6 +64,9 @@ struct TaskDesc {
 #define MI_MODE_UNIQUE 0   // write the unique (ij|kl) array
 #define MI_MODE_SCHWARZ 1  // diagonal (ab|ab): atomic max of |value| into PairGeom.Q
 #define MI_MODE_JK 2       // accumulate Coulomb/exchange against a density
+#ifndef MI_JK_RUN
+#define MI_JK_RUN 32       // J/K, wave kernels: longest run of consecutive quartets per block (< 2: off)
+#endif
 
 struct EriArgs {
   const PairTopo* topo;
@@ -84,6 +87,8 @@ struct EriArgs {
   const double* D;     // J/K: density, nbf x nbf row-major
   double* Jacc;        // J/K: unsymmetrised accumulators (J = Jacc + Jacc^T)
   double* Kacc;
+  int32_t jk_run;      // J/K, wave kernels: quartets per contiguous run (0: one atomic per contribution)
+  unsigned long long* run_ctr;  // J/K runs: next run of this launch (zero before the launch)
 };
 
 struct QuartetCtx {
@@ -265,9 +270,11 @@ __device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc)
   const int64_t qi = ((w - T.blockStart) * 64 + lane) / G;
   qc.G = G;
   qc.g = lane & (G - 1);
-  if (qi >= T.nq) return false;
-  decode_index(T, qi, qc);
-  return true;
+  // a lane past the task's last quartet gets the task's first quartet (valid
+  // indices; the caller drops it or, in the J/K build, contributes nothing)
+  const bool live = qi < T.nq;
+  decode_index(T, live ? qi : 0, qc);
+  return live;
 }
 
 template <int NA, int NB, int NC, int ND>
@@ -395,34 +402,86 @@ __device__ __forceinline__ double jk_factor(const ScatterCtx& s) {
   return (s.braSame ? 0.5 : 1.0) * (s.ketSame ? 0.5 : 1.0) * (s.quartetSame ? 0.5 : 1.0);
 }
 
+// Segments of a wavefront's quartet slots (G lanes each, slot = lane / G):
+// maximal runs of consecutive slots with equal key.  start: first slot of
+// this slot's run; tail: last slot of its run; merge: some run is longer than
+// one slot (wavefront-uniform).
+struct WaveSeg {
+  int slot, nslot, G, start;
+  bool tail, merge;
+};
+
+__device__ __forceinline__ WaveSeg wave_segments(int gmask, bool live, uint32_t key) {
+  WaveSeg r;
+  r.G = gmask + 1;
+  r.slot = (threadIdx.x & 63) / r.G;
+  r.nslot = 64 / r.G;
+  if (!live) key = 0xFFFF0000u | (uint32_t)r.slot;  // never equal to a live key (offsets < 65535)
+  const uint32_t prev = __shfl_up(key, r.G, 64), next = __shfl_down(key, r.G, 64);
+  const bool head = r.slot == 0 || prev != key;
+  r.tail = r.slot == r.nslot - 1 || next != key;
+  r.merge = __any(!head);
+  int st = head ? r.slot : 0;
+  if (r.merge)
+    for (int d = 1; d < r.nslot; d <<= 1) {
+      const int o = __shfl_up(st, d * r.G, 64);
+      if (r.slot >= d) st = max(st, o);
+    }
+  r.start = st;
+  return r;
+}
+
+// Segmented inclusive sum over the slots of the run (lanes of equal g): the
+// run's tail slot ends with the run's total.  Every lane of the wavefront calls it.
+__device__ __forceinline__ double wave_segment_sum(const WaveSeg& r, double x) {
+  if (!r.merge) return x;
+  for (int d = 1; d < r.nslot; d <<= 1) {
+    const double o = __shfl_up(x, d * r.G, 64);
+    if (r.slot - d >= r.start) x += o;
+  }
+  return x;
+}
+
 // Register-resident block (generated kernels): outputs t = 0..T-1 of the six
-// contractions are split over the G lanes of the quartet (t mod G).
+// contractions are split over the G lanes of the quartet (t mod G).  The
+// J_ab, K_ac and K_bc targets repeat over consecutive quartets of a task row
+// (same bra pair, and the same ket shell C in plans laid out for J/K), so
+// those contributions are first summed over each run of equal targets in the
+// wavefront, and only the run's last quartet issues the atomic.  Every lane of
+// the wavefront calls this; live = false marks a lane without a quartet (its
+// block is zero and it issues no atomic).
 template <int NA, int NB, int NC, int ND>
-__device__ __forceinline__ void jk_block_regs(const EriArgs& a, const ScatterCtx& s,
+__device__ __forceinline__ void jk_block_regs(const EriArgs& a, const ScatterCtx& s, bool live,
                                               const double (&v)[NA * NB * NC * ND]) {
   const double f = jk_factor(s);
   const int64_t n = a.nbf;
   const double* __restrict__ D = a.D;
   const int64_t ra = s.fa0 * n, rb = s.fb0 * n, rc = s.fc0 * n;
+  const WaveSeg rab = wave_segments(s.gmask, live, (uint32_t)s.fa0 << 16 | (uint32_t)s.fb0);
+  const WaveSeg rac = wave_segments(s.gmask, live, (uint32_t)s.fa0 << 16 | (uint32_t)s.fc0);
+  const WaveSeg rbc = wave_segments(s.gmask, live, (uint32_t)s.fb0 << 16 | (uint32_t)s.fc0);
 #define V_(i, j, k, l) v[(((i) * NB + (j)) * NC + (k)) * ND + (l)]
   constexpr int T0 = NA * NB, T1 = T0 + NC * ND, T2 = T1 + NA * NC, T3 = T2 + NA * ND, T4 = T3 + NB * NC;
 #pragma unroll
   for (int i = 0; i < NA; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (((i * NB + j) & s.gmask) == s.g) {
-        double acc = 0.0;
+    for (int j = 0; j < NB; ++j) {
+      const bool own = ((i * NB + j) & s.gmask) == s.g;
+      double acc = 0.0;
+      if (own) {
 #pragma unroll
         for (int k = 0; k < NC; ++k)
 #pragma unroll
           for (int l = 0; l < ND; ++l) acc += V_(i, j, k, l) * D[rc + k * n + s.fd0 + l];
-        atomicAdd(&a.Jacc[ra + i * n + s.fb0 + j], 2.0 * f * acc);
       }
+      acc = wave_segment_sum(rab, 2.0 * f * acc);
+      if (own && live && rab.tail) atomicAdd(&a.Jacc[ra + i * n + s.fb0 + j], acc);
+    }
 #pragma unroll
   for (int k = 0; k < NC; ++k)
 #pragma unroll
     for (int l = 0; l < ND; ++l)
-      if (((T0 + k * ND + l) & s.gmask) == s.g) {
+      if (live && ((T0 + k * ND + l) & s.gmask) == s.g) {
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < NA; ++i)
@@ -433,20 +492,23 @@ __device__ __forceinline__ void jk_block_regs(const EriArgs& a, const ScatterCtx
 #pragma unroll
   for (int i = 0; i < NA; ++i)
 #pragma unroll
-    for (int k = 0; k < NC; ++k)
-      if (((T1 + i * NC + k) & s.gmask) == s.g) {
-        double acc = 0.0;
+    for (int k = 0; k < NC; ++k) {
+      const bool own = ((T1 + i * NC + k) & s.gmask) == s.g;
+      double acc = 0.0;
+      if (own) {
 #pragma unroll
         for (int j = 0; j < NB; ++j)
 #pragma unroll
           for (int l = 0; l < ND; ++l) acc += V_(i, j, k, l) * D[rb + j * n + s.fd0 + l];
-        atomicAdd(&a.Kacc[ra + i * n + s.fc0 + k], f * acc);
       }
+      acc = wave_segment_sum(rac, f * acc);
+      if (own && live && rac.tail) atomicAdd(&a.Kacc[ra + i * n + s.fc0 + k], acc);
+    }
 #pragma unroll
   for (int i = 0; i < NA; ++i)
 #pragma unroll
     for (int l = 0; l < ND; ++l)
-      if (((T2 + i * ND + l) & s.gmask) == s.g) {
+      if (live && ((T2 + i * ND + l) & s.gmask) == s.g) {
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
@@ -457,20 +519,23 @@ __device__ __forceinline__ void jk_block_regs(const EriArgs& a, const ScatterCtx
 #pragma unroll
   for (int j = 0; j < NB; ++j)
 #pragma unroll
-    for (int k = 0; k < NC; ++k)
-      if (((T3 + j * NC + k) & s.gmask) == s.g) {
-        double acc = 0.0;
+    for (int k = 0; k < NC; ++k) {
+      const bool own = ((T3 + j * NC + k) & s.gmask) == s.g;
+      double acc = 0.0;
+      if (own) {
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
           for (int l = 0; l < ND; ++l) acc += V_(i, j, k, l) * D[ra + i * n + s.fd0 + l];
-        atomicAdd(&a.Kacc[rb + j * n + s.fc0 + k], f * acc);
       }
+      acc = wave_segment_sum(rbc, f * acc);
+      if (own && live && rbc.tail) atomicAdd(&a.Kacc[rb + j * n + s.fc0 + k], acc);
+    }
 #pragma unroll
   for (int j = 0; j < NB; ++j)
 #pragma unroll
     for (int l = 0; l < ND; ++l)
-      if (((T4 + j * ND + l) & s.gmask) == s.g) {
+      if (live && ((T4 + j * ND + l) & s.gmask) == s.g) {
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < NA; ++i)
@@ -482,14 +547,77 @@ __device__ __forceinline__ void jk_block_regs(const EriArgs& a, const ScatterCtx
 }
 
 // Block staged in LDS (wave kernels): thread tid of nthr takes outputs
-// tid, tid + nthr, ... of the six contractions.
+// t = tid, tid + nthr, ... of the six contractions, ordered J_ab, K_ac, K_bc
+// (the targets shared by consecutive quartets of a task row), then J_cd,
+// K_ad, K_bd.  With racc (LDS, one double per output of the first three
+// groups), the first three groups are summed there over a run of quartets
+// and written when their target changes (jk_run_flush); output t keeps the
+// same thread for every quartet, so racc[t] needs no synchronisation.
+// Next run of jk_run quartets for the block (dynamic: runs differ in cost, so
+// blocks take them from a launch-wide counter until the launch is done).
+__device__ __forceinline__ int64_t next_run(const EriArgs& a, unsigned long long* s_run) {
+  __syncthreads();  // every thread has read the previous run
+  if (threadIdx.x == 0) *s_run = atomicAdd(a.run_ctr, 1ull);
+  __syncthreads();
+  return (int64_t)(*s_run) * a.jk_run;
+}
+
+struct JkRun {
+  uint32_t kab, kac, kbc;  // keys of the targets held in racc (kNoRun: none)
+  int fa0, fb0, fc0;
+};
+constexpr uint32_t kNoRun = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void jk_run_flush(const EriArgs& a, JkRun& r, double* racc, int na, int nb, int nc,
+                                             bool fab, bool fac, bool fbc, int tid, int nthr) {
+  const int64_t n = a.nbf;
+  const int nab = na * nb, nac = na * nc, nbc = nb * nc;
+  fab = fab && r.kab != kNoRun;
+  fac = fac && r.kac != kNoRun;
+  fbc = fbc && r.kbc != kNoRun;
+  if (!(fab || fac || fbc)) return;
+  for (int t = tid; t < nab + nac + nbc; t += nthr) {
+    int u = t;
+    if (u < nab) {
+      if (!fab) continue;
+      const int i = u / nb, j = u - i * nb;
+      atomicAdd(&a.Jacc[(int64_t)(r.fa0 + i) * n + r.fb0 + j], racc[t]);
+    } else if ((u -= nab) < nac) {
+      if (!fac) continue;
+      const int i = u / nc, k = u - i * nc;
+      atomicAdd(&a.Kacc[(int64_t)(r.fa0 + i) * n + r.fc0 + k], racc[t]);
+    } else {
+      if (!fbc) continue;
+      u -= nac;
+      const int j = u / nc, k = u - j * nc;
+      atomicAdd(&a.Kacc[(int64_t)(r.fb0 + j) * n + r.fc0 + k], racc[t]);
+    }
+    racc[t] = 0.0;
+  }
+  if (fab) r.kab = kNoRun;
+  if (fac) r.kac = kNoRun;
+  if (fbc) r.kbc = kNoRun;
+}
+
 __device__ __forceinline__ void jk_block_lds(const EriArgs& a, const ScatterCtx& s, const double* V, int na, int nb,
-                                             int nc, int nd, int tid, int nthr) {
+                                             int nc, int nd, int tid, int nthr, double* racc, JkRun& run) {
   const double f = jk_factor(s);
   const int64_t n = a.nbf;
   const double* __restrict__ D = a.D;
   const int nab = na * nb, ncd = nc * nd, nac = na * nc, nad = na * nd, nbc = nb * nc, nbd = nb * nd;
   const int total = nab + ncd + nac + nad + nbc + nbd;
+  const int nrun = racc ? nab + nac + nbc : 0;
+  if (racc) {
+    const uint32_t kab = (uint32_t)s.fa0 << 16 | (uint32_t)s.fb0, kac = (uint32_t)s.fa0 << 16 | (uint32_t)s.fc0,
+                   kbc = (uint32_t)s.fb0 << 16 | (uint32_t)s.fc0;
+    jk_run_flush(a, run, racc, na, nb, nc, kab != run.kab, kac != run.kac, kbc != run.kbc, tid, nthr);
+    run.kab = kab;
+    run.kac = kac;
+    run.kbc = kbc;
+    run.fa0 = s.fa0;
+    run.fb0 = s.fb0;
+    run.fc0 = s.fc0;
+  }
   for (int t = tid; t < total; t += nthr) {
     int u = t;
     double acc = 0.0, w = f;
@@ -500,35 +628,38 @@ __device__ __forceinline__ void jk_block_lds(const EriArgs& a, const ScatterCtx&
         for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fc0 + k) * n + s.fd0 + l];
       dst = &a.Jacc[(s.fa0 + i) * n + s.fb0 + j];
       w = 2.0 * f;
-    } else if ((u -= nab) < ncd) {
+    } else if ((u -= nab) < nac) {
+      const int i = u / nc, k = u - i * nc;
+      for (int j = 0; j < nb; ++j)
+        for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fb0 + j) * n + s.fd0 + l];
+      dst = &a.Kacc[(s.fa0 + i) * n + s.fc0 + k];
+    } else if ((u -= nac) < nbc) {
+      const int j = u / nc, k = u - j * nc;
+      for (int i = 0; i < na; ++i)
+        for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fa0 + i) * n + s.fd0 + l];
+      dst = &a.Kacc[(s.fb0 + j) * n + s.fc0 + k];
+    } else if ((u -= nbc) < ncd) {
       const int k = u / nd, l = u - k * nd;
       for (int i = 0; i < na; ++i)
         for (int j = 0; j < nb; ++j) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fa0 + i) * n + s.fb0 + j];
       dst = &a.Jacc[(s.fc0 + k) * n + s.fd0 + l];
       w = 2.0 * f;
-    } else if ((u -= ncd) < nac) {
-      const int i = u / nc, k = u - i * nc;
-      for (int j = 0; j < nb; ++j)
-        for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fb0 + j) * n + s.fd0 + l];
-      dst = &a.Kacc[(s.fa0 + i) * n + s.fc0 + k];
-    } else if ((u -= nac) < nad) {
+    } else if ((u -= ncd) < nad) {
       const int i = u / nd, l = u - i * nd;
       for (int j = 0; j < nb; ++j)
         for (int k = 0; k < nc; ++k) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fb0 + j) * n + s.fc0 + k];
       dst = &a.Kacc[(s.fa0 + i) * n + s.fd0 + l];
-    } else if ((u -= nad) < nbc) {
-      const int j = u / nc, k = u - j * nc;
-      for (int i = 0; i < na; ++i)
-        for (int l = 0; l < nd; ++l) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fa0 + i) * n + s.fd0 + l];
-      dst = &a.Kacc[(s.fb0 + j) * n + s.fc0 + k];
     } else {
-      u -= nbc;
+      u -= nad;
       const int j = u / nd, l = u - j * nd;
       for (int i = 0; i < na; ++i)
         for (int k = 0; k < nc; ++k) acc += V[((i * nb + j) * nc + k) * nd + l] * D[(s.fa0 + i) * n + s.fc0 + k];
       dst = &a.Kacc[(s.fb0 + j) * n + s.fd0 + l];
     }
-    atomicAdd(dst, w * acc);
+    if (t < nrun)
+      racc[t] += w * acc;
+    else
+      atomicAdd(dst, w * acc);
   }
 }
 

@@ -208,7 +208,7 @@ void free_plan_buffers(mi_plan* p) {
     if (e) hipEventDestroy(e);
   void* bufs[] = {p->d_shell_xyz, p->d_shell_prim_off, p->d_prim_exp, p->d_prim_cn, p->d_bf_scale, p->d_topo,
                   p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch,   p->d_bounds,
-                  p->d_block_task, p->d_pp_ab};
+                  p->d_block_task, p->d_pp_ab, p->d_run_ctr};
   for (void* b : bufs) {
     if (!b) continue;
     if (p->alloc_stream)
@@ -274,7 +274,10 @@ int mi_create(int device, mi_ctx** out) {
     c->lds_limit = std::max<size_t>(prop.sharedMemPerBlock, prop.maxSharedMemoryPerMultiProcessor);
   }
   c->lds_limit = std::min<size_t>(c->lds_limit, 160 * 1024);  // CDNA4: 160 KiB LDS per CU
-  if (c->lds_limit > 65536 && mi::set_wave_kernel_lds_limit(c->lds_limit) != hipSuccess) c->lds_limit = 65536;
+  if (c->lds_limit > 65536 && mi::set_wave_kernel_lds_limit(c->lds_limit) != hipSuccess) {
+    (void)hipGetLastError();  // not sticky: later launches check hipGetLastError
+    c->lds_limit = 65536;
+  }
   c->tune.assign((size_t)MI_NREGIME * mi::nclass() * MI_NBUCKET * MI_NBUCKET, TuneEntry());
   load_tuning_file(c);
   *out = c;
@@ -377,7 +380,13 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
         }
       }
       if (var == 0) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
-      if (var == 2) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wg_kernel_blocks_per_cu(c, nw));
+      L.tl = false;
+      if (var == 2) {
+        // tables in LDS only where that costs no resident blocks: a launch
+        // smaller than the chip's capacity at the larger LDS footprint
+        blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wg_kernel_blocks_per_cu(c, nw, false));
+        L.tl = mi::wg_tables_fit(c) && blocks <= (int64_t)ctx->ncu * mi::wg_kernel_blocks_per_cu(c, nw, true);
+      }
       if (blocks > INT32_MAX || qs > INT32_MAX || waves > INT32_MAX) {
         return set_err(ctx, MI_ENOMEM, "class %d: %lld quartets exceed one launch", c, (long long)qs);
       }
@@ -432,13 +441,15 @@ int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
       (rc = upload(P, &P->d_tasks, h.tasks.data(), h.tasks.size())) ||
       (P->scratch_doubles && (rc = upload<double>(P, &P->d_scratch, nullptr, P->scratch_doubles))) ||
       (!h.bounds.empty() && (rc = upload(P, &P->d_bounds, h.bounds.data(), h.bounds.size()))) ||
-      (!h.block_task.empty() && (rc = upload(P, &P->d_block_task, h.block_task.data(), h.block_task.size()))))
+      (!h.block_task.empty() && (rc = upload(P, &P->d_block_task, h.block_task.data(), h.block_task.size()))) ||
+      (rc = upload<unsigned long long>(P, &P->d_run_ctr, nullptr, P->launches.size())))
     return rc;
   return MI_OK;
 }
 
 // Schwarz bounds of every shell pair, in the canonical pair order
-// (molecule, pair class, contraction bucket, pair index): a bounds-only plan
+// (molecule, pair class, contraction bucket, [first shell: MI_ORDER_JK,]
+// pair index) of a plan built with the same options: a bounds-only plan
 // (diagonal quartets) runs on the context's bounds stream with stream-ordered
 // memory and is read back.  Q = max |(ab|ab)|, V = max |(e0|e0)| (VRR outputs).
 static int compute_pair_bounds(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts,
@@ -486,7 +497,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   P->ctx = ctx;
   P->bounds_only = bounds_only;
   P->order = opts ? opts->order : MI_ORDER_CHRR;
-  if (P->order != MI_ORDER_CHRR && P->order != MI_ORDER_IINDEX) {
+  if (P->order != MI_ORDER_CHRR && P->order != MI_ORDER_IINDEX && P->order != MI_ORDER_JK) {
     delete P;
     return set_err(ctx, MI_EINVAL, "unknown order %d", P->order);
   }
@@ -564,10 +575,16 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         const int i = I - mol_sh0[m], j = J - mol_sh0[m];
         keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0, ab0});
       }
-  std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
+  // J/K layout: within a (class, bucket) list, pairs of equal first shell A
+  // are contiguous and form their own list, so the quartets of a task share
+  // the shells A (bra) and C (ket) and consecutive quartets of a task row
+  // share their J_ab, K_ac and K_bc targets (summed before the atomics).
+  const bool jk_layout = P->order == MI_ORDER_JK;
+  std::stable_sort(keys.begin(), keys.end(), [jk_layout](const PairKey& x, const PairKey& y) {
     if (x.mol != y.mol) return x.mol < y.mol;
     if (x.pc != y.pc) return x.pc < y.pc;
     if (x.bucket != y.bucket) return x.bucket < y.bucket;
+    if (jk_layout && x.A != y.A) return x.A < y.A;
     return x.pidx < y.pidx;
   });
   const bool has_bounds = !kQ.empty();
@@ -580,11 +597,12 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
       keys[r].Q = kQ[r];
       keys[r].V = kV[r];
     }
-    // within each (molecule, class, bucket) list: largest bound first
-    std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
+    // within each (molecule, class, bucket[, first shell]) list: largest bound first
+    std::stable_sort(keys.begin(), keys.end(), [jk_layout](const PairKey& x, const PairKey& y) {
       if (x.mol != y.mol) return x.mol < y.mol;
       if (x.pc != y.pc) return x.pc < y.pc;
       if (x.bucket != y.bucket) return x.bucket < y.bucket;
+      if (jk_layout && x.A != y.A) return x.A < y.A;
       return x.Q > y.Q;
     });
   }
@@ -609,7 +627,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   P->npp = (int)ppoff;
   P->pair_mol.resize(npair);
   for (int r = 0; r < npair; ++r) P->pair_mol[r] = keys[r].mol;
-  // Contiguous lists of equal (molecule, pc, bucket).
+  // Contiguous lists of equal (molecule, pc, bucket), J/K layout: and first shell.
   struct List {
     int start, n, pc, Ksum, Kmax, mol;
   };
@@ -618,7 +636,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   for (int r = 0; r < npair;) {
     int e = r;
     int Ksum = 0, Kmax = 0;
-    while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket) {
+    while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket &&
+           (!jk_layout || keys[e].A == keys[r].A)) {
       Ksum += keys[e].K;
       Kmax = std::max(Kmax, keys[e].K);
       ++e;
@@ -871,12 +890,13 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
   a.tasks = P->d_tasks + L.task_off;
   a.ntasks = L.ntasks;
   a.block_task = L.var == 1 ? P->d_block_task + L.bt_off : nullptr;
+  a.run_ctr = (a.mode == MI_MODE_JK && P->d_run_ctr) ? P->d_run_ctr + (&L - P->launches.data()) : nullptr;
   if (L.var == 1)
     HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
   else if (L.var == 0)
     HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
   else
-    HIP_TRY(ctx, mi::launch_class_wave_global(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables,
+    HIP_TRY(ctx, mi::launch_class_wave_global(L.cls, L.nw, L.tl, a, L.nblocks, L.nq_shell, ctx->wtables,
                                               P->d_scratch + L.scratch_off, ls));
   return MI_OK;
 }
@@ -889,6 +909,8 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
   if (!P) return MI_EINVAL;
   mi_ctx* ctx = P->ctx;
   if (!jk && !P->bounds_only) {
+    if (P->order == MI_ORDER_JK)
+      return set_err(ctx, MI_EINVAL, "plan laid out for Coulomb/exchange builds (MI_ORDER_JK): no ERI output");
     if (!out) return set_err(ctx, MI_EINVAL, "output pointer is NULL");
     if (out_len != P->n_unique)
       return set_err(ctx, MI_ESIZE, "output length %lld, basis needs %lld", (long long)out_len,
@@ -927,6 +949,10 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
     const int64_t nn = (int64_t)P->nbf * P->nbf;
     HIP_TRY(ctx, mi::launch_zero(jk->J, nn, s));
     HIP_TRY(ctx, mi::launch_zero(jk->K, nn, s));
+    // run counters of the wave-kernel launches (a kernel: graph-capture safe;
+    // the all-zero bit pattern of a double is integer zero)
+    if (P->d_run_ctr && !P->launches.empty())
+      HIP_TRY(ctx, mi::launch_zero(reinterpret_cast<double*>(P->d_run_ctr), (int64_t)P->launches.size(), s));
   } else if (P->cut2 > 0.0 && !P->bounds_only) {
     // screened quartets are never written: their integrals are zero
     HIP_TRY(ctx, mi::launch_zero(out, P->n_unique, s));
@@ -1032,8 +1058,11 @@ int mi_plan_jk_execute(mi_plan* P, const double* D, double* J, double* K, void* 
 int mi_jk(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, const double* D, double* J, double* K) {
   if (!ctx) return MI_EINVAL;
   if (!b || !D || !J || !K) return set_err(ctx, MI_EINVAL, "NULL argument");
+  mi_opts o{};
+  if (opts) o = *opts;
+  o.order = MI_ORDER_JK;
   mi_plan* P = nullptr;
-  int rc = mi_plan_create(ctx, b, opts, &P);
+  int rc = mi_plan_create(ctx, b, &o, &P);
   if (rc) return rc;
   const size_t nn = (size_t)b->nbf * b->nbf;
   double *dD = nullptr, *dJ = nullptr, *dK = nullptr;

@@ -41,6 +41,7 @@ struct Launch {
   int cls;
   int var;  // 1 register kernel, 0 LDS wave kernel, 2 global-table kernel
   int nw;   // wave kernels: wavefronts per block
+  bool tl;  // global-table kernel: op tables staged in LDS
   size_t scratch_off;  // global-table kernel: offset of its slice of the plan scratch (doubles)
   int64_t bt_off;      // register kernel: offset of its block -> task table in the plan's
   int task_off, ntasks, nblocks;
@@ -87,6 +88,7 @@ struct mi_plan {
   bool bounds_only = false;         // Schwarz pass only (mi_plan_build with MI_BUILD_BOUNDS_ONLY)
   double2* d_bounds = nullptr;      // per pair (Q, V) computed when the plan was built, or null
   int32_t* d_block_task = nullptr;  // register launches: task of each block's first wavefront
+  unsigned long long* d_run_ctr = nullptr;  // J/K: run counter of each launch (cleared every J/K execute)
   size_t device_bytes = 0;  // bytes of device memory the plan holds (or needs, before upload)
   struct HostData {         // host images of the uploaded buffers
     std::vector<double> xyz, pexp, pcn, bfs;

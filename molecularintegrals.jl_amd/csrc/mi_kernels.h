@@ -27,12 +27,14 @@ size_t wclass_lds_bytes(int c, int nw);
 bool wclass_fits(int c);
 hipError_t set_wave_kernel_lds_limit(size_t bytes);
 // global-table kernel (variant 2): nw in {1, 2, 4, 8} wavefronts per block,
-// scratch = wg_scratch_doubles(c, nblocks) doubles
-size_t wgclass_lds_bytes(int c);
+// scratch = wg_scratch_doubles(c, nblocks) doubles; tl: op tables staged in
+// LDS (only where wg_tables_fit(c); more LDS per block, fewer blocks per CU)
+bool wg_tables_fit(int c);
+size_t wgclass_lds_bytes(int c, bool tl);
 bool wgclass_fits_nw(int c, int nw, size_t lds_limit);
-int wg_kernel_blocks_per_cu(int c, int nw);
+int wg_kernel_blocks_per_cu(int c, int nw, bool tl);
 size_t wg_scratch_doubles(int c, int nblocks);
-hipError_t launch_class_wave_global(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch,
+hipError_t launch_class_wave_global(int c, int nw, bool tl, const EriArgs& args, int nblocks, int64_t nq_launch,
                                     void* wtables, double* scratch, hipStream_t stream);
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream);
 hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream);

@@ -135,8 +135,12 @@ __host__ __device__ __forceinline__ int w_desc_doubles(const WClassDesc& D) { re
 // VRR area (levels separated by wave-local syncs) and keeps its share of the
 // contracted (e0|f0) in registers; the shares are summed in wave order (a
 // fixed order: results do not depend on scheduling), then all waves run the
-// HRR levels and the scatter.
+// HRR levels and the scatter.  In the J/K build with args.jk_run > 0 a block
+// takes runs of jk_run consecutive quartets instead (runs blockIdx.x,
+// blockIdx.x + gridDim.x, ...), so the J_ab, K_ac and K_bc targets shared by
+// a run are summed in LDS and written once per run or target change.
 // LDS: [op tables][hrr: nhrr][scales: 4*W_SCL][coefficients: nw * W_CF][vrr: nw * nvrr]
+//      [J/K run sums: na nb + na nc + nb nc][run index: 1], J/K build with jk_run > 0
 template <int ACC>
 __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W,
                                                     int64_t nq_launch) {
@@ -164,8 +168,18 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     const int x = k * 64 + lane;
     asrc[k] = x < D.nacc ? (int)(sA[x] & 0xFFFF) : 0;  // bit 31 marks a diagonal (e0|e0)
   }
+  const bool jrun = args.mode == MI_MODE_JK && args.jk_run > 0;
+  double* racc = jrun ? vrr_all + nw * D.nvrr : nullptr;
+  JkRun run{kNoRun, kNoRun, kNoRun, 0, 0, 0};
+  if (jrun)
+    for (int t = tid; t < na * nb + na * nc + nb * nc; t += nthr) racc[t] = 0.0;
+  unsigned long long* s_run = jrun ? reinterpret_cast<unsigned long long*>(racc + na * nb + na * nc + nb * nc) : nullptr;
+  const int64_t CH = jrun ? args.jk_run : 1;
   int ti = 0;
-  for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
+  int64_t c0 = jrun ? next_run(args, s_run) : (int64_t)blockIdx.x;
+  while (c0 < nq_launch) {
+  const int64_t c1 = c0 + CH < nq_launch ? c0 + CH : nq_launch;
+  for (int64_t q = c0; q < c1; ++q) {
     ti = advance_task(args, ti, q);
     const TaskDesc T = args.tasks[ti];
     QuartetCtx qc;
@@ -276,7 +290,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
         vrr_all[x] = hrr[o & 0xFFFF] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
       }
       __syncthreads();
-      jk_block_lds(args, sc, vrr_all, na, nb, nc, nd, tid, nthr);
+      jk_block_lds(args, sc, vrr_all, na, nb, nc, nd, tid, nthr, racc, run);
       __syncthreads();
       continue;
     }
@@ -288,6 +302,9 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
                    hrr[off] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]));
     }
     __syncthreads();
+  }
+  if (jrun) jk_run_flush(args, run, racc, na, nb, nc, true, true, true, tid, nthr);
+  c0 = jrun ? next_run(args, s_run) : c0 + gridDim.x;
   }
 }
 
@@ -304,25 +321,52 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
 // LDS: [scales: 4*W_SCL][vrr: nvrr]
 // ---------------------------------------------------------------------------
 // HL: the contracted block and HRR levels also fit in LDS (after the VRR
-// area) -- then no global scratch is used.
-template <bool HL>
+// area) -- then no global scratch is used.  TL: the class's op tables are
+// staged into LDS too (once per persistent block), read there instead of
+// through L1/L2.
+// J/K build with args.jk_run > 0: runs of quartets as in the wave kernel.
+// LDS: [scales: 4*W_SCL][vrr: nvrr][hrr: nhrr if HL][tables if TL]
+//      [J/K run sums: na nb + na nc + nb nc][run index: 1], J/K build with jk_run > 0
+template <bool HL, bool TL>
 __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const WClassDesc D,
                                                      const uint32_t* __restrict__ gblob,
                                                      double* __restrict__ scratch, int64_t nq_launch) {
   extern __shared__ double lds[];
   double* scl = lds;
   double* vrr = lds + 4 * W_SCL;
-  const uint4* gV = reinterpret_cast<const uint4*>(gblob + D.gv);
-  const uint4* gH = reinterpret_cast<const uint4*>(gblob + D.gh);
-  const uint2_t* gO = reinterpret_cast<const uint2_t*>(gblob + D.go);
-  const uint2_t* gA = reinterpret_cast<const uint2_t*>(gblob + D.ga);
   double* hrr = HL ? vrr + D.nvrr : scratch + (size_t)blockIdx.x * D.nhrr;
   const int tid = threadIdx.x, nthr = blockDim.x;
+  const uint32_t* tab = gblob;  // the class's tables start at word D.gv
+  double* lds_end = vrr + D.nvrr + (HL ? D.nhrr : 0);
+  if (TL) {
+    // 16-byte aligned copy of words [gv, ga + 2 nacc) after the work areas
+    double* t = lds_end;
+    t += (reinterpret_cast<uintptr_t>(t) & 8) ? 1 : 0;
+    const int nwords = (D.ga + 2 * D.nacc - D.gv + 3) & ~3;
+    stage_chunks(reinterpret_cast<uint4*>(t), reinterpret_cast<const uint4*>(gblob + D.gv), nwords / 4);
+    __syncthreads();
+    tab = reinterpret_cast<const uint32_t*>(t) - D.gv;
+    lds_end = t + nwords / 2;
+  }
+  const uint4* gV = reinterpret_cast<const uint4*>(tab + D.gv);
+  const uint4* gH = reinterpret_cast<const uint4*>(tab + D.gh);
+  const uint2_t* gO = reinterpret_cast<const uint2_t*>(tab + D.go);
+  const uint2_t* gA = reinterpret_cast<const uint2_t*>(tab + D.ga);
   const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
   const int mtop = D.mmax - 1;
   const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
+  const bool jrun = args.mode == MI_MODE_JK && args.jk_run > 0;
+  double* racc = jrun ? lds_end : nullptr;
+  JkRun run{kNoRun, kNoRun, kNoRun, 0, 0, 0};
+  if (jrun)
+    for (int t = tid; t < na * nb + na * nc + nb * nc; t += nthr) racc[t] = 0.0;
+  unsigned long long* s_run = jrun ? reinterpret_cast<unsigned long long*>(racc + na * nb + na * nc + nb * nc) : nullptr;
+  const int64_t CH = jrun ? args.jk_run : 1;
   int ti = 0;
-  for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
+  int64_t c0 = jrun ? next_run(args, s_run) : (int64_t)blockIdx.x;
+  while (c0 < nq_launch) {
+  const int64_t c1 = c0 + CH < nq_launch ? c0 + CH : nq_launch;
+  for (int64_t q = c0; q < c1; ++q) {
     ti = advance_task(args, ti, q);
     const TaskDesc T = args.tasks[ti];
     QuartetCtx qc;
@@ -409,7 +453,7 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
         vrr[x] = hrr[o.x] * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
       }
       __syncthreads();
-      jk_block_lds(args, sc, vrr, na, nb, nc, nd, tid, nthr);
+      jk_block_lds(args, sc, vrr, na, nb, nc, nd, tid, nthr, racc, run);
       __syncthreads();
       continue;
     }
@@ -421,12 +465,29 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
     }
     __syncthreads();
   }
+  if (jrun) jk_run_flush(args, run, racc, na, nb, nc, true, true, true, tid, nthr);
+  c0 = jrun ? next_run(args, s_run) : c0 + gridDim.x;
+  }
 }
 
 // Zero a device array (a kernel, not a memset node, so captured graphs replay it safely).
+// 16-byte non-temporal stores (the array is usually far larger than the
+// caches and is next written scattered), one leading and one trailing
+// element written singly when x is not 16-byte aligned or n is odd.
+typedef double zero_d2 __attribute__((ext_vector_type(2)));
 __global__ void __launch_bounds__(256) zero_kernel(double* __restrict__ x, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    x[i] = 0.0;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nthr = (int64_t)gridDim.x * blockDim.x;
+  const int64_t head = (reinterpret_cast<uintptr_t>(x) & 15) ? 1 : 0;
+  if (n <= head) {
+    if (tid == 0 && n > 0) x[0] = 0.0;
+    return;
+  }
+  if (tid == 0 && head) x[0] = 0.0;
+  zero_d2* v = reinterpret_cast<zero_d2*>(x + head);
+  const int64_t nv = (n - head) >> 1;
+  const zero_d2 z = {0.0, 0.0};
+  for (int64_t i = tid; i < nv; i += nthr) __builtin_nontemporal_store(z, v + i);
+  if (tid == 0 && ((n - head) & 1)) x[n - 1] = 0.0;
 }
 
 // J = Jacc + Jacc^T, K = Kacc + Kacc^T, in place (one lane per i <= j).
@@ -525,13 +586,46 @@ bool wclass_fits_nw(int c, int nw, size_t lds_limit) {
   return wclass_fits(c) && nw >= 1 && nw <= 4 && wclass_lds_bytes(c, nw) <= lds_limit;
 }
 
+// J/K build: LDS for the run sums of the wave kernels (one double per J_ab,
+// K_ac, K_bc output, then the block's current run index), used when the
+// block's LDS stays within the limit.
+static size_t jk_run_bytes(int c) {
+  const WClassDesc& D = kWClassDesc[c];
+  const int na = (D.la + 1) * (D.la + 2) / 2, nb = (D.lb + 1) * (D.lb + 2) / 2, nc = (D.lc + 1) * (D.lc + 2) / 2;
+  return sizeof(double) * (size_t)(na * nb + na * nc + nb * nc + 1);
+}
+
+static size_t g_lds_limit = 65536;  // dynamic LDS a wave-kernel block may use (set_wave_kernel_lds_limit)
+
+// Run length: up to MI_JK_RUN quartets, but short enough that the launch
+// holds about four runs per block (small launches stay spread out).
+// The run sums take LDS: the persistent grid is cut to what stays resident
+// with it (the blocks loop over the launch's quartets, any grid is correct).
+static EriArgs with_jk_run(const EriArgs& args, int c, const void* fn, int nw, size_t& lds, int64_t nq_launch,
+                           int& nblocks) {
+  EriArgs a = args;
+  a.jk_run = 0;
+  const int64_t run = std::min<int64_t>(MI_JK_RUN, nq_launch / (4 * (int64_t)std::max(nblocks, 1)));
+  if (a.mode == MI_MODE_JK && a.run_ctr && run >= 2 && lds + jk_run_bytes(c) <= g_lds_limit) {
+    a.jk_run = (int32_t)run;
+    lds += jk_run_bytes(c);
+    int dev = 0, ncu = 0, bpc = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, 64 * nw, lds) == hipSuccess && ncu > 0 && bpc > 0)
+      nblocks = std::min(nblocks, ncu * bpc);
+  }
+  return a;
+}
+
 hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
                              hipStream_t stream) {
   const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
   WTablesDev W{t->blob};
   const WClassDesc D = kWClassDesc[c];
-  const size_t lds = wclass_lds_bytes(c, nw);
-  void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<WClassDesc*>(&D), &W, &nq_launch};
+  size_t lds = wclass_lds_bytes(c, nw);
+  EriArgs a = with_jk_run(args, c, w_kernel_fn(c), nw, lds, nq_launch, nblocks);
+  void* kargs[] = {&a, const_cast<WClassDesc*>(&D), &W, &nq_launch};
   return hipLaunchKernel(w_kernel_fn(c), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
@@ -539,41 +633,56 @@ hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, in
 // block's LDS stays within this (several blocks per CU remain resident).
 #define WG_HRR_LDS_MAX (64 * 1024)
 
+static size_t wg_table_bytes(int c) {
+  const WClassDesc& D = kWClassDesc[c];
+  return 4 * (size_t)((D.ga + 2 * D.nacc - D.gv + 3) & ~3) + 16;  // + alignment slack
+}
+
 static bool wg_hrr_in_lds(int c) {
   const WClassDesc& D = kWClassDesc[c];
   return sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + D.nhrr) <= WG_HRR_LDS_MAX;
 }
 
-static const void* wg_kernel_fn(int c) {
-  return wg_hrr_in_lds(c) ? (const void*)hgp_wg_kernel<true> : (const void*)hgp_wg_kernel<false>;
+bool wg_tables_fit(int c) {
+  const WClassDesc& D = kWClassDesc[c];
+  return wg_hrr_in_lds(c) &&
+         sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + D.nhrr) + wg_table_bytes(c) <= WG_HRR_LDS_MAX;
 }
 
-size_t wgclass_lds_bytes(int c) {
+static const void* wg_kernel_fn(int c, bool tl) {
+  if (tl && wg_tables_fit(c)) return (const void*)hgp_wg_kernel<true, true>;
+  return wg_hrr_in_lds(c) ? (const void*)hgp_wg_kernel<true, false> : (const void*)hgp_wg_kernel<false, false>;
+}
+
+size_t wgclass_lds_bytes(int c, bool tl) {
   const WClassDesc& D = kWClassDesc[c];
-  return sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + (wg_hrr_in_lds(c) ? D.nhrr : 0));
+  return sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + (wg_hrr_in_lds(c) ? D.nhrr : 0)) +
+         (tl && wg_tables_fit(c) ? wg_table_bytes(c) : 0);
 }
 
 bool wgclass_fits_nw(int c, int nw, size_t lds_limit) {
-  return (nw == 1 || nw == 2 || nw == 4 || nw == 8) && wgclass_lds_bytes(c) <= lds_limit;
+  return (nw == 1 || nw == 2 || nw == 4 || nw == 8) && wgclass_lds_bytes(c, false) <= lds_limit;
 }
 
-int wg_kernel_blocks_per_cu(int c, int nw) {
+int wg_kernel_blocks_per_cu(int c, int nw, bool tl) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, wg_kernel_fn(c), 64 * nw,
-                                                   wgclass_lds_bytes(c)) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, wg_kernel_fn(c, tl), 64 * nw,
+                                                   wgclass_lds_bytes(c, tl)) != hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;
 }
 
 size_t wg_scratch_doubles(int c, int nblocks) { return wg_hrr_in_lds(c) ? 0 : (size_t)nblocks * kWClassDesc[c].nhrr; }
 
-hipError_t launch_class_wave_global(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch,
+hipError_t launch_class_wave_global(int c, int nw, bool tl, const EriArgs& args, int nblocks, int64_t nq_launch,
                                     void* wtables, double* scratch, hipStream_t stream) {
   const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
   const WClassDesc D = kWClassDesc[c];
   const uint32_t* gb = t->gblob;
-  void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<WClassDesc*>(&D), &gb, &scratch, &nq_launch};
-  return hipLaunchKernel(wg_kernel_fn(c), dim3(nblocks), dim3(64 * nw), kargs, wgclass_lds_bytes(c), stream);
+  size_t lds = wgclass_lds_bytes(c, tl);
+  EriArgs a = with_jk_run(args, c, wg_kernel_fn(c, tl), nw, lds, nq_launch, nblocks);
+  void* kargs[] = {&a, const_cast<WClassDesc*>(&D), &gb, &scratch, &nq_launch};
+  return hipLaunchKernel(wg_kernel_fn(c, tl), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream) {
@@ -597,10 +706,14 @@ hipError_t set_wave_kernel_lds_limit(size_t bytes) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
   }
-  hipError_t e = hipFuncSetAttribute((const void*)hgp_wg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)hgp_wg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  const void* gk[] = {(const void*)hgp_wg_kernel<false, false>, (const void*)hgp_wg_kernel<true, false>,
+                      (const void*)hgp_wg_kernel<true, true>};
+  for (const void* f : gk) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+  }
+  g_lds_limit = bytes;
+  return hipSuccess;
 }
 
 }  // namespace mi

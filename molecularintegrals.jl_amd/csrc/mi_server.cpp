@@ -282,7 +282,7 @@ int mi_server_submit(mi_server* s, const mi_basis* basis, const mi_opts* opts, i
   r.b.prim_norm = r.pnorm.data();
   r.b.bf_scale = r.scale.data();
   if (opts) r.o = *opts;
-  r.o.order = opts ? opts->order : MI_ORDER_CHRR;
+  r.o.order = kind == MI_REQ_JK ? MI_ORDER_JK : (opts ? opts->order : MI_ORDER_CHRR);
   r.o.stream = nullptr;
   r.D = D;
   r.out = out;

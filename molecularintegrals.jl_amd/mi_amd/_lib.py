@@ -14,7 +14,7 @@ PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("MI_HGP_LIB", os.path.join(PKG_ROOT, "libmi_hgp.so"))
 
 MI_OK, MI_EINVAL, MI_EUNSUPPORTED, MI_EDEVICE, MI_ENOMEM, MI_ESIZE = range(6)
-MI_ORDER_IINDEX, MI_ORDER_CHRR = 0, 1
+MI_ORDER_IINDEX, MI_ORDER_CHRR, MI_ORDER_JK = 0, 1, 2  # MI_ORDER_JK: plan for J/K builds only
 MI_PREC_FP64, MI_PREC_MIXED = 0, 1
 
 # Every symbol include/mi_hgp.h declares (checked by the CPU test suite).

@@ -66,7 +66,7 @@ class Context:
         D = np.ascontiguousarray(D, dtype=np.float64).reshape(n, n)
         J = np.empty((n, n))
         K = np.empty((n, n))
-        o = _lib.opts(_lib.MI_ORDER_IINDEX, precision, tau, schwarz_cut)
+        o = _lib.opts(_lib.MI_ORDER_JK, precision, tau, schwarz_cut)
         dp = ctypes.POINTER(ctypes.c_double)
         self.check(self.L.mi_jk(self.h, ctypes.byref(args.struct), ctypes.byref(o), D.ctypes.data_as(dp),
                                 J.ctypes.data_as(dp), K.ctypes.data_as(dp)))

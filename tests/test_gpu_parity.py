@@ -439,6 +439,68 @@ def test_jk_f_shells_mixed_and_screened(ctx):
     assert np.abs(J1 - J0).max() <= 1e-9 and np.abs(K1 - K0).max() <= 1e-9
 
 
+def _forced_rows(ctx, variant, G):
+    """Tuning rows that put every class able to take (variant, G) on it."""
+    from mi_amd.hgp_tables import quartet_classes
+    rows = []
+    for c in range(len(quartet_classes())):
+        try:
+            ctx.set_tuning([(c, 0, 0, variant, G)])
+        except Exception:
+            continue
+        rows += [(c, bb, kb, variant, G) for bb in range(5) for kb in range(5)]
+    return rows
+
+
+@pytest.mark.parametrize("kernels", ["tuned", "R1", "R8", "W1", "W4", "G1", "G4", "eri_layout"])
+def test_jk_layout_and_kernel_families(kernels):
+    """J/K of a 14-atom split-valence molecule (class launches of thousands of
+    quartets, so the wave kernels sum J_ab/K_ac/K_bc over runs of quartets and
+    the register kernels over runs of wavefront slots) against the oracle's
+    make2JmK algebra on the FP64 unique array: the J/K plan layout
+    (MI_ORDER_JK) with every kernel family forced in turn, and J/K executed on
+    a plan in ERI order.  Also: an ERI execute on a J/K plan is refused."""
+    import torch
+    from mi_amd import Context, _lib, configs
+    wl = configs.random_molecule(np.random.Generator(np.random.PCG64(41)), 14, configs.SPLIT_HEAVY,
+                                 configs.SPLIT_LIGHT)
+    fb = wl.flat()
+    n = fb.nbf
+    D = _density(n, 9)
+    key = ("jk14", n)
+    if key not in _ORACLE_CACHE:
+        eri = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX, nthreads=8)
+        _ORACLE_CACHE[key] = O.jk_from_unique(n, eri, D)
+    Jr, Kr = _ORACLE_CACHE[key]
+    c = Context(0)
+    if kernels not in ("tuned", "eri_layout"):
+        var = {"R": 1, "W": 0, "G": 2}[kernels[0]]
+        c.set_tuning(_forced_rows(c, var, int(kernels[1:])))
+    order = _lib.MI_ORDER_IINDEX if kernels == "eri_layout" else _lib.MI_ORDER_JK
+    plan = c.plan(fb, order=order)
+    print(f"{kernels}: nbf {n}, launches {[(''.join('spdf'[x] for x in L[0]), L[1], L[4]) for L in plan.launches()]}")
+    dD = torch.from_numpy(D).cuda()
+    J = torch.full((n, n), float("nan"), dtype=torch.float64, device="cuda")
+    K = torch.full((n, n), float("nan"), dtype=torch.float64, device="cuda")
+    sp = torch.cuda.current_stream().cuda_stream
+    for rep in range(2):  # the accumulators are cleared by every execute
+        plan.jk_execute(dD.data_ptr(), J.data_ptr(), K.data_ptr(), sp)
+        Jg, Kg = J.cpu().numpy(), K.cpu().numpy()
+        scale = max(1.0, np.abs(Jr).max())
+        print(f"  rep {rep}: max |dJ| {np.abs(Jg - Jr).max():.2e} max |dK| {np.abs(Kg - Kr).max():.2e}")
+        assert np.abs(Jg - Jr).max() <= 1e-11 * scale and np.abs(Kg - Kr).max() <= 1e-11 * scale
+    if order == _lib.MI_ORDER_JK:
+        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+        with pytest.raises(_lib.MiError) as e:
+            plan.execute(out.data_ptr(), sp)
+        assert e.value.code == _lib.MI_EINVAL
+    plan.close()
+    c.close()
+
+
+_ORACLE_CACHE = {}
+
+
 def test_balance_keeps_results(ctx):
     """Reassigning stream lanes from measured launch times changes no bit."""
     import torch
@@ -478,6 +540,33 @@ def test_batched_screened_equals_single(ctx):
         assert np.array_equal(got[off[m]:off[m + 1]], alone), f"molecule {m}"
         ref = ctx.eri_unique(fb)
         assert np.abs(alone - ref).max() <= 1e-11 + 1e-10 * 1e-3
+    plan.close()
+
+
+@pytest.mark.parametrize("shift", [1, 3])
+def test_screened_output_at_odd_offset(ctx, shift):
+    """A screened plan writing into a buffer that starts 8 (or 24) bytes past a
+    16-byte boundary: the pre-zeroing covers exactly [0, n_unique) of it (the
+    guard elements either side keep their NaN) and the values equal those of an
+    aligned output."""
+    import torch
+    from mi_amd import configs
+    rng = np.random.Generator(np.random.PCG64(21))
+    fb = configs.random_molecule(rng, 5, configs.SPLIT_HEAVY, configs.SPLIT_LIGHT).flat()
+    kw = dict(schwarz_cut=1e-10)
+    plan = ctx.plan(fb, **kw)
+    n = plan.n_unique
+    sp = torch.cuda.current_stream().cuda_stream
+    ref = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+    plan.execute(ref.data_ptr(), sp)
+    buf = torch.full((n + shift + 2,), float("nan"), dtype=torch.float64, device="cuda")
+    assert buf.data_ptr() % 16 == 0
+    plan.execute(buf[shift:].data_ptr(), sp)
+    got = buf.cpu().numpy()
+    r = ref.cpu().numpy()
+    assert np.isnan(got[:shift]).all() and np.isnan(got[shift + n:]).all()
+    assert np.array_equal(got[shift:shift + n], r)
+    assert (r == 0.0).sum() > 0  # some quartets were screened (zero-filled)
     plan.close()
 
 
