@@ -40,10 +40,11 @@ extern "C" {
 /* output orders */
 #define MI_ORDER_IINDEX 0  /* all_twoe_ints:      slot = iindex(i,j,k,l) - 1   */
 #define MI_ORDER_CHRR 1    /* all_twoe_ints_chrr: slot = enumeration index - 1 */
-#define MI_ORDER_JK 2      /* no ERI output: a plan laid out for mi_plan_jk_execute
-                              (shell pairs grouped by first shell, so that J/K
-                              contributions to one target are summed before the
-                              atomics); mi_plan_execute on it returns MI_EINVAL */
+#define MI_ORDER_JK 2      /* no ERI output: a plan for mi_plan_jk_execute (without
+                              a Schwarz cut its shell pairs are grouped by first
+                              shell, so that more J/K contributions to one target
+                              are summed before the atomics); mi_plan_execute on
+                              it returns MI_EINVAL */
 
 /* precision modes */
 #define MI_PREC_FP64 0     /* everything in FP64                               */

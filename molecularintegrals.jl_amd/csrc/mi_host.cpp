@@ -448,7 +448,7 @@ int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
 }
 
 // Schwarz bounds of every shell pair, in the canonical pair order
-// (molecule, pair class, contraction bucket, [first shell: MI_ORDER_JK,]
+// (molecule, pair class, contraction bucket, [first shell: J/K layout,]
 // pair index) of a plan built with the same options: a bounds-only plan
 // (diagonal quartets) runs on the context's bounds stream with stream-ordered
 // memory and is read back.  Q = max |(ab|ab)|, V = max |(e0|e0)| (VRR outputs).
@@ -575,11 +575,14 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         const int i = I - mol_sh0[m], j = J - mol_sh0[m];
         keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0, ab0});
       }
-  // J/K layout: within a (class, bucket) list, pairs of equal first shell A
-  // are contiguous and form their own list, so the quartets of a task share
-  // the shells A (bra) and C (ket) and consecutive quartets of a task row
-  // share their J_ab, K_ac and K_bc targets (summed before the atomics).
-  const bool jk_layout = P->order == MI_ORDER_JK;
+  // J/K layout of a plan without a cut: within a (class, bucket) list, pairs
+  // of equal first shell A are contiguous, so along a task row (one bra
+  // pair) the ket shell C changes rarely and the K_ac targets repeat too
+  // (J_ab and K_bc repeat along any row).  A screening plan orders each list
+  // by bound instead (its rectangles leave the screened quartets out);
+  // splitting its lists by first shell as well was measured: about 6% less
+  // J/K time on a 920-function cluster for 20x the plan-build time.
+  const bool jk_layout = P->order == MI_ORDER_JK && cut2 == 0.0;
   std::stable_sort(keys.begin(), keys.end(), [jk_layout](const PairKey& x, const PairKey& y) {
     if (x.mol != y.mol) return x.mol < y.mol;
     if (x.pc != y.pc) return x.pc < y.pc;
@@ -597,14 +600,14 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
       keys[r].Q = kQ[r];
       keys[r].V = kV[r];
     }
-    // within each (molecule, class, bucket[, first shell]) list: largest bound first
-    std::stable_sort(keys.begin(), keys.end(), [jk_layout](const PairKey& x, const PairKey& y) {
-      if (x.mol != y.mol) return x.mol < y.mol;
-      if (x.pc != y.pc) return x.pc < y.pc;
-      if (x.bucket != y.bucket) return x.bucket < y.bucket;
-      if (jk_layout && x.A != y.A) return x.A < y.A;
-      return x.Q > y.Q;
-    });
+    // within each (molecule, class, bucket) list: largest bound first
+    if (!jk_layout)
+      std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
+        if (x.mol != y.mol) return x.mol < y.mol;
+        if (x.pc != y.pc) return x.pc < y.pc;
+        if (x.bucket != y.bucket) return x.bucket < y.bucket;
+        return x.Q > y.Q;
+      });
   }
   const int npair = (int)keys.size();
   std::vector<PairTopo> topo(npair);
@@ -627,22 +630,22 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   P->npp = (int)ppoff;
   P->pair_mol.resize(npair);
   for (int r = 0; r < npair; ++r) P->pair_mol[r] = keys[r].mol;
-  // Contiguous lists of equal (molecule, pc, bucket), J/K layout: and first shell.
+  // Contiguous lists of equal (molecule, pc, bucket).
   struct List {
     int start, n, pc, Ksum, Kmax, mol;
+    double qmax;  // bound of the list's first pair (its largest, once ordered by bound)
   };
   std::vector<List> lists;
   std::vector<int> mol_list0(nmol + 1, 0);
   for (int r = 0; r < npair;) {
     int e = r;
     int Ksum = 0, Kmax = 0;
-    while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket &&
-           (!jk_layout || keys[e].A == keys[r].A)) {
+    while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket) {
       Ksum += keys[e].K;
       Kmax = std::max(Kmax, keys[e].K);
       ++e;
     }
-    lists.push_back({r, e - r, keys[r].pc, Ksum, Kmax, keys[r].mol});
+    lists.push_back({r, e - r, keys[r].pc, Ksum, Kmax, keys[r].mol, keys[r].Q});
     r = e;
   }
   for (size_t x = 0, m = 0; m <= (size_t)nmol; ++m) {
@@ -667,6 +670,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     for (int y = x; y < mol_list0[m + 1]; ++y) {
       const List* bra = &lists[x];
       const List* ket = &lists[y];
+      // lists ordered by bound: nothing of the pair survives the cut
+      if (screen_tasks && bra->qmax * ket->qmax < cut2) continue;
       if (ket->pc > bra->pc) std::swap(bra, ket);
       const double kavg = ((double)bra->Ksum / bra->n) * ((double)ket->Ksum / ket->n);
       const int bb = keys[bra->start].bucket, kb = keys[ket->start].bucket;
