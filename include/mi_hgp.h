@@ -98,7 +98,8 @@ int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
  * [1 generated register kernel, 0 LDS wave kernel, 2 global-table kernel],
  * G = lanes per quartet for the register kernel or wavefronts per quartet /
  * block for the wave kernels, regime [0: class launches of fewer than 65536
- * shell quartets, 1: larger ones, -1: both]).  Replaces the whole table
+ * shell quartets, 1: larger ones, 2: launches of MI_ORDER_JK plans (where a
+ * class has no regime-2 entry they follow 0/1), -1: all]).  Replaces the whole table
  * (or nothing, on an invalid row); unlisted entries use the built-in
  * heuristic.  mi_create loads hgp_tuning.txt beside the library
  * (MI_HGP_TUNING overrides).  Affects plans created afterwards. */

@@ -123,7 +123,8 @@ void set_tuning_row(mi_ctx* c, int cls, int bb, int kb, int var, int G, int regi
 
 // Tuning file: lines "class bra_bucket ket_bucket variant G [regime]" ('#'
 // comments; regime 0 = class launches of fewer than MI_SMALL_LAUNCH shell
-// quartets, 1 = larger ones, absent = both), written by mi_amd/autotune.py.
+// quartets, 1 = larger ones, 2 = launches of J/K plans (MI_ORDER_JK), absent
+// = all), written by mi_amd/autotune.py.
 // Default location: hgp_tuning.txt next to the library; MI_HGP_TUNING
 // overrides, MI_HGP_TUNING=none disables.
 void load_tuning_file(mi_ctx* c) {
@@ -305,8 +306,11 @@ void mi_destroy(mi_ctx* ctx) {
 // register kernel when one exists with lanes ~ primitive quartets / 4, else the
 // wave kernel with as many wavefronts (<= 4) as the LDS budget allows, else the
 // global-table kernel with 4 wavefronts per block.
-static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, int regime) {
-  const TuneEntry& te = ctx->tune[tune_index(regime, t.cls, bb, kb)];
+// regime: 0/1 by the size of the class launch; J/K plans use the J/K regime
+// (2) where it has an entry, else the size regime.
+static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, int regime, bool jk) {
+  const TuneEntry* tj = jk ? &ctx->tune[tune_index(2, t.cls, bb, kb)] : nullptr;
+  const TuneEntry& te = (tj && tj->variant >= 0) ? *tj : ctx->tune[tune_index(regime, t.cls, bb, kb)];
   int variant, G;
   if (te.variant >= 0) {
     variant = te.variant;
@@ -749,7 +753,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     for (size_t k = 0; k < per_class[c].size(); ++k) {
       TaskDesc& t = per_class[c][k];
       const int bb = t.G, kb = t.variant;
-      if ((rc = choose_kernel(ctx, t, bb, kb, kavgs[c][k], cls_nq[c] < MI_SMALL_LAUNCH ? 0 : 1))) {
+      if ((rc = choose_kernel(ctx, t, bb, kb, kavgs[c][k], cls_nq[c] < MI_SMALL_LAUNCH ? 0 : 1,
+                              P->order == MI_ORDER_JK))) {
         delete P;
         return rc;
       }
@@ -787,7 +792,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
         t.M = nbm * (nbm + 1) / 2;
         const int bk = keys[li.start].bucket;
-        if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n), 0))) {
+        if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n), 0, false))) {
           delete P;
           return rc;
         }

@@ -11,7 +11,7 @@
 #include "mi_kernels.h"
 
 #define MI_NBUCKET 5
-#define MI_NREGIME 2             // tuning regimes: small and large class launches
+#define MI_NREGIME 3             // tuning regimes: small and large class launches, J/K plans
 #define MI_SMALL_LAUNCH 65536    // shell quartets below which a class launch is "small" (latency-bound)
 #define MI_PRIM_PAIR_CUT 1e-30   // primitive pairs with |K| below this times their shell pair's largest are dropped
 #define MI_MAX_STREAMS 16

@@ -7,9 +7,11 @@ For every quartet class it picks, on the GPU it runs on:
   * for the register kernel, the lanes per quartet G as a function of the
     contraction depth: G = pow2floor(K_bra * K_ket / tau), clamped to [1, 64],
     with tau chosen per class; for the wave kernels, the wavefront count.
-Two regimes are tuned apart: small class launches (fewer than 65536 shell
-quartets, latency-bound: more lanes per quartet pay) and large ones
-(throughput-bound); a plan picks the regime per class from its quartet count.
+Three regimes are tuned apart: small class launches (fewer than 65536 shell
+quartets, latency-bound: more lanes per quartet pay), large ones
+(throughput-bound) -- a plan picks one of these per class from its quartet
+count -- and the launches of Coulomb/exchange plans (MI_ORDER_JK: the block
+is contracted with the density instead of written out).
 Method: each candidate setting is applied to all classes at once, one
 execute per workload is profiled with HIP events between launches (median
 of ``reps``), and each class keeps, per regime, the candidate with the
@@ -17,7 +19,8 @@ smallest sum over that regime's workloads of its time relative to the
 workload's best candidate.  The result is written as hgp_tuning.txt rows
 "class bra_bucket ket_bucket variant G regime", read by mi_create.
 
-Usage (on the GPU):  python -m mi_amd.autotune [--out PATH] [--reps N]
+Usage (on the GPU):  python -m mi_amd.autotune [--out PATH] [--reps N] [--regimes 0,1,2 [--merge]]
+(--merge keeps the rows of the regimes not tuned from the existing file).
 """
 from __future__ import annotations
 
@@ -27,7 +30,7 @@ import sys
 
 import numpy as np
 
-from . import configs
+from . import _lib, configs
 from .eri import Context
 from .hgp_tables import quartet_classes
 
@@ -64,16 +67,34 @@ def rows_for(cls_choice, has_reg, w_ok, g_ok):
     return rows
 
 
+class JKWorkload:
+    """A Coulomb/exchange build (random symmetric density) under plan options kw."""
+
+    def __init__(self, fb, seed=0, **kw):
+        self.fb, self.kw = fb, kw
+        rng = np.random.Generator(np.random.PCG64(seed))
+        d = rng.normal(size=(fb.nbf, fb.nbf)) / fb.nbf
+        self.D = 0.5 * (d + d.T)
+
+
 def class_times(ctx, wl, rows, reps):
-    """Per-class time (ms) of one execute of workload wl: a FlatBasis or a
-    list of them (a batched plan)."""
+    """Per-class time (ms) of one execute of workload wl: a FlatBasis, a list
+    of them (a batched plan) or a JKWorkload."""
     import torch
     ctx.set_tuning(rows)
-    plan = ctx.batch_plan(wl) if isinstance(wl, list) else ctx.plan(wl, order=1)
-    out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
     sptr = torch.cuda.current_stream().cuda_stream
-    launches = plan.launches()
-    prof = np.array([plan.profile(out.data_ptr(), sptr) for _ in range(reps)])
+    if isinstance(wl, JKWorkload):
+        plan = ctx.plan(wl.fb, order=_lib.MI_ORDER_JK, **wl.kw)
+        D = torch.from_numpy(wl.D).cuda()
+        J = torch.empty_like(D)
+        K = torch.empty_like(D)
+        launches = plan.launches()
+        prof = np.array([plan.jk_profile(D.data_ptr(), J.data_ptr(), K.data_ptr(), sptr) for _ in range(reps)])
+    else:
+        plan = ctx.batch_plan(wl) if isinstance(wl, list) else ctx.plan(wl, order=1)
+        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+        launches = plan.launches()
+        prof = np.array([plan.profile(out.data_ptr(), sptr) for _ in range(reps)])
     med = np.median(prof, axis=0)
     per = {}
     for i, (cls, nqs, nqp, nblk, kern) in enumerate(launches):
@@ -83,8 +104,9 @@ def class_times(ctx, wl, rows, reps):
 
 
 def autotune(workloads, regimes, reps=7, verbose=True):
-    """workloads: FlatBasis or lists of them (batched plans); regimes[i]: the
-    tuning regime (0 small class launches, 1 large) workload i stands for."""
+    """workloads: FlatBasis, lists of them (batched plans) or JKWorkload;
+    regimes[i]: the tuning regime (0 small class launches, 1 large, 2 J/K
+    plans) workload i stands for.  Returns rows for the regimes present."""
     classes = quartet_classes()
     cid = {c: k for k, c in enumerate(classes)}
     ctx = Context(0)
@@ -139,19 +161,19 @@ def autotune(workloads, regimes, reps=7, verbose=True):
 
     name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
     rows = []
-    picks = []
-    for regime in (0, 1):
-        wis = [wi for wi, r in enumerate(regimes) if r == regime]
-        picks.append(pick(wis))
-    for regime in (0, 1):
+    tuned = sorted(set(regimes))
+    picks = {regime: pick([wi for wi, r in enumerate(regimes) if r == regime]) for regime in tuned}
+    for regime in tuned:
         best, score = picks[regime]
-        other = picks[1 - regime][0]
+        others = [picks[r][0] for r in tuned if r != regime]
         choice = {}
         for k in range(len(classes)):
             if k in best:
                 choice[k] = best[k]
-            elif k in other:  # class absent from this regime's workloads
-                choice[k] = other[k]
+            elif regime == 2:
+                continue  # J/K plans fall back to the size regimes for classes not seen here
+            elif any(k in o for o in others):  # class absent from this regime's workloads
+                choice[k] = next(o[k] for o in others if k in o)
             else:
                 choice[k] = (1, 4) if has_reg[k] else fallback(k, w_ok, g_ok)
             if verbose and k in score:
@@ -169,21 +191,41 @@ def main():
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                                   "hgp_tuning.txt"))
     ap.add_argument("--reps", type=int, default=7)
+    ap.add_argument("--regimes", default="0,1,2", help="regimes to tune (comma separated)")
+    ap.add_argument("--merge", action="store_true", help="keep the other regimes' rows of the existing --out file")
     args = ap.parse_args()
+    want = sorted({int(x) for x in args.regimes.split(",")})
+    kept = []
+    if args.merge and os.path.exists(args.out):
+        for line in open(args.out):
+            f = line.split()
+            if line.startswith("#") or len(f) != 6:
+                continue
+            if int(f[5]) not in want:
+                kept.append(tuple(int(x) for x in f))
     os.environ["MI_HGP_TUNING"] = "none"  # tune from the heuristic baseline
     # small class launches (latency-bound): the 60-function molecule and a
     # 4-atom cluster-recipe molecule with f shells; large ones (throughput): a
     # batch of 48 molecules of 3-24 atoms and a 16-atom cluster-recipe molecule
+    # J/K plans: mixed precision with screening on a 16-atom cluster-recipe
+    # molecule and a 40-atom split-valence one (the serving J/K requests)
     rng = np.random.Generator(np.random.PCG64(9))
     wls = [configs.ethane_split_valence(0).flat(),
            configs.random_molecule(rng, 4, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat(),
            [m.flat() for m in configs.batch_molecules(1, nmol=48)],
            configs.random_molecule(rng, 16, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat()]
-    rows = autotune(wls, [0, 0, 1, 1], args.reps)
+    regimes = [0, 0, 1, 1]
+    mixed = dict(precision=_lib.MI_PREC_MIXED, tau=1e-8, schwarz_cut=1e-12)
+    wls += [JKWorkload(configs.random_molecule(rng, 16, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat(), 1, **mixed),
+            JKWorkload(configs.random_molecule(rng, 40, configs.SPLIT_HEAVY, configs.SPLIT_LIGHT).flat(), 2, **mixed)]
+    regimes += [2, 2]
+    sel = [i for i, r in enumerate(regimes) if r in want]
+    rows = kept + autotune([wls[i] for i in sel], [regimes[i] for i in sel], args.reps)
     with open(args.out, "w") as f:
         f.write("# HGP ERI kernel choice per (class, bra bucket, ket bucket, regime): variant (1 register "
                 "kernel, 0 LDS wave kernel, 2 global-table kernel) and G (lanes per quartet, or wavefronts);\n"
-                "# regime 0 = class launches of fewer than 65536 shell quartets, 1 = larger ones.\n"
+                "# regime 0 = class launches of fewer than 65536 shell quartets, 1 = larger ones, "
+                "2 = launches of J/K plans.\n"
                 "# Written by mi_amd/autotune.py on the GPU it describes; columns: class bra_bucket ket_bucket "
                 "variant G regime\n")
         for r in rows:
