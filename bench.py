@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a 256 MiB device copy gives); profiles/hbm_traffic_ethane_split_valence_60bf.md
 # holds the commands and the per-kernel table.
-MEASURED_TRAFFIC_BYTES = {"hgp_w_kernel": 1396586}
+MEASURED_TRAFFIC_BYTES = {"hgp_w_kernel": 1477714}
 
 
 def log(*a):
