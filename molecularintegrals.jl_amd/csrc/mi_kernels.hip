@@ -141,7 +141,8 @@ __host__ __device__ __forceinline__ int w_desc_doubles(const WClassDesc& D) { re
 // a run are summed in LDS and written once per run or target change.
 // LDS: [op tables][hrr: nhrr][scales: 4*W_SCL][coefficients: nw * W_CF][vrr: nw * nvrr]
 //      [J/K run sums: na nb + na nc + nb nc][run index: 1], J/K build with jk_run > 0
-template <int ACC>
+// JR: instantiation with the J/K run code (the other keeps the ERI path's registers).
+template <int ACC, bool JR>
 __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W,
                                                     int64_t nq_launch) {
   extern __shared__ double lds[];
@@ -168,7 +169,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     const int x = k * 64 + lane;
     asrc[k] = x < D.nacc ? (int)(sA[x] & 0xFFFF) : 0;  // bit 31 marks a diagonal (e0|e0)
   }
-  const bool jrun = args.mode == MI_MODE_JK && args.jk_run > 0;
+  const bool jrun = JR && args.mode == MI_MODE_JK && args.jk_run > 0;
   double* racc = jrun ? vrr_all + nw * D.nvrr : nullptr;
   JkRun run{kNoRun, kNoRun, kNoRun, 0, 0, 0};
   if (jrun)
@@ -327,7 +328,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
 // J/K build with args.jk_run > 0: runs of quartets as in the wave kernel.
 // LDS: [scales: 4*W_SCL][vrr: nvrr][hrr: nhrr if HL][tables if TL]
 //      [J/K run sums: na nb + na nc + nb nc][run index: 1], J/K build with jk_run > 0
-template <bool HL, bool TL>
+template <bool HL, bool TL, bool JR>
 __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const WClassDesc D,
                                                      const uint32_t* __restrict__ gblob,
                                                      double* __restrict__ scratch, int64_t nq_launch) {
@@ -355,7 +356,7 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
   const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
   const int mtop = D.mmax - 1;
   const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
-  const bool jrun = args.mode == MI_MODE_JK && args.jk_run > 0;
+  const bool jrun = JR && args.mode == MI_MODE_JK && args.jk_run > 0;
   double* racc = jrun ? lds_end : nullptr;
   JkRun run{kNoRun, kNoRun, kNoRun, 0, 0, 0};
   if (jrun)
@@ -563,14 +564,18 @@ size_t wclass_lds_bytes(int c, int nw) {
   return sizeof(double) * (size_t)(w_desc_doubles(D) + D.nhrr + 4 * W_SCL + (size_t)nw * (W_CF + D.nvrr));
 }
 
-// Instantiation holding the class's contracted block: ACC values per lane.
-static const void* w_kernel_fn(int c) {
+// Instantiation holding the class's contracted block: ACC values per lane;
+// jr: with the J/K run code.
+template <bool JR>
+static const void* w_kernel_fn_t(int c) {
   const int nacc = kWClassDesc[c].nacc;
-  if (nacc <= 64 * 2) return (const void*)hgp_w_kernel<2>;
-  if (nacc <= 64 * 4) return (const void*)hgp_w_kernel<4>;
-  if (nacc <= 64 * 8) return (const void*)hgp_w_kernel<8>;
-  return (const void*)hgp_w_kernel<16>;
+  if (nacc <= 64 * 2) return (const void*)hgp_w_kernel<2, JR>;
+  if (nacc <= 64 * 4) return (const void*)hgp_w_kernel<4, JR>;
+  if (nacc <= 64 * 8) return (const void*)hgp_w_kernel<8, JR>;
+  return (const void*)hgp_w_kernel<16, JR>;
 }
+
+static const void* w_kernel_fn(int c, bool jr = false) { return jr ? w_kernel_fn_t<true>(c) : w_kernel_fn_t<false>(c); }
 
 int wave_kernel_blocks_per_cu(int c, int nw) {
   int nb = 0;
@@ -603,6 +608,7 @@ static size_t g_lds_limit = 65536;  // dynamic LDS a wave-kernel block may use (
 // with it (the blocks loop over the launch's quartets, any grid is correct).
 static EriArgs with_jk_run(const EriArgs& args, int c, const void* fn, int nw, size_t& lds, int64_t nq_launch,
                            int& nblocks) {
+  // fn: the kernel instantiation with the J/K run code
   EriArgs a = args;
   a.jk_run = 0;
   const int64_t run = std::min<int64_t>(MI_JK_RUN, nq_launch / (4 * (int64_t)std::max(nblocks, 1)));
@@ -624,9 +630,9 @@ hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, in
   WTablesDev W{t->blob};
   const WClassDesc D = kWClassDesc[c];
   size_t lds = wclass_lds_bytes(c, nw);
-  EriArgs a = with_jk_run(args, c, w_kernel_fn(c), nw, lds, nq_launch, nblocks);
+  EriArgs a = with_jk_run(args, c, w_kernel_fn(c, true), nw, lds, nq_launch, nblocks);
   void* kargs[] = {&a, const_cast<WClassDesc*>(&D), &W, &nq_launch};
-  return hipLaunchKernel(w_kernel_fn(c), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
+  return hipLaunchKernel(w_kernel_fn(c, a.jk_run > 0), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
 // The global-table kernel keeps its HRR area in LDS too when the whole
@@ -649,9 +655,14 @@ bool wg_tables_fit(int c) {
          sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + D.nhrr) + wg_table_bytes(c) <= WG_HRR_LDS_MAX;
 }
 
-static const void* wg_kernel_fn(int c, bool tl) {
-  if (tl && wg_tables_fit(c)) return (const void*)hgp_wg_kernel<true, true>;
-  return wg_hrr_in_lds(c) ? (const void*)hgp_wg_kernel<true, false> : (const void*)hgp_wg_kernel<false, false>;
+template <bool JR>
+static const void* wg_kernel_fn_t(int c, bool tl) {
+  if (tl && wg_tables_fit(c)) return (const void*)hgp_wg_kernel<true, true, JR>;
+  return wg_hrr_in_lds(c) ? (const void*)hgp_wg_kernel<true, false, JR> : (const void*)hgp_wg_kernel<false, false, JR>;
+}
+
+static const void* wg_kernel_fn(int c, bool tl, bool jr = false) {
+  return jr ? wg_kernel_fn_t<true>(c, tl) : wg_kernel_fn_t<false>(c, tl);
 }
 
 size_t wgclass_lds_bytes(int c, bool tl) {
@@ -680,9 +691,9 @@ hipError_t launch_class_wave_global(int c, int nw, bool tl, const EriArgs& args,
   const WClassDesc D = kWClassDesc[c];
   const uint32_t* gb = t->gblob;
   size_t lds = wgclass_lds_bytes(c, tl);
-  EriArgs a = with_jk_run(args, c, wg_kernel_fn(c, tl), nw, lds, nq_launch, nblocks);
+  EriArgs a = with_jk_run(args, c, wg_kernel_fn(c, tl, true), nw, lds, nq_launch, nblocks);
   void* kargs[] = {&a, const_cast<WClassDesc*>(&D), &gb, &scratch, &nq_launch};
-  return hipLaunchKernel(wg_kernel_fn(c, tl), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
+  return hipLaunchKernel(wg_kernel_fn(c, tl, a.jk_run > 0), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream) {
@@ -700,14 +711,17 @@ hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream) {
 }
 
 hipError_t set_wave_kernel_lds_limit(size_t bytes) {
-  const void* wk[] = {(const void*)hgp_w_kernel<2>, (const void*)hgp_w_kernel<4>, (const void*)hgp_w_kernel<8>,
-                      (const void*)hgp_w_kernel<16>};
+  const void* wk[] = {(const void*)hgp_w_kernel<2, false>,  (const void*)hgp_w_kernel<4, false>,
+                      (const void*)hgp_w_kernel<8, false>,  (const void*)hgp_w_kernel<16, false>,
+                      (const void*)hgp_w_kernel<2, true>,   (const void*)hgp_w_kernel<4, true>,
+                      (const void*)hgp_w_kernel<8, true>,   (const void*)hgp_w_kernel<16, true>};
   for (const void* f : wk) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
   }
-  const void* gk[] = {(const void*)hgp_wg_kernel<false, false>, (const void*)hgp_wg_kernel<true, false>,
-                      (const void*)hgp_wg_kernel<true, true>};
+  const void* gk[] = {(const void*)hgp_wg_kernel<false, false, false>, (const void*)hgp_wg_kernel<true, false, false>,
+                      (const void*)hgp_wg_kernel<true, true, false>,   (const void*)hgp_wg_kernel<false, false, true>,
+                      (const void*)hgp_wg_kernel<true, false, true>,   (const void*)hgp_wg_kernel<true, true, true>};
   for (const void* f : gk) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
