@@ -472,23 +472,12 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
 }
 
 // Zero a device array (a kernel, not a memset node, so captured graphs replay it safely).
-// 16-byte non-temporal stores (the array is usually far larger than the
-// caches and is next written scattered), one leading and one trailing
-// element written singly when x is not 16-byte aligned or n is odd.
-typedef double zero_d2 __attribute__((ext_vector_type(2)));
+// Plain 8-byte stores, 4096 x 256 lanes: 5.8 TB/s on 40 GiB, as fast as
+// hipMemsetAsync (5.9); 16-byte, non-temporal and unrolled variants measured
+// 5.0-5.8 TB/s.
 __global__ void __launch_bounds__(256) zero_kernel(double* __restrict__ x, int64_t n) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nthr = (int64_t)gridDim.x * blockDim.x;
-  const int64_t head = (reinterpret_cast<uintptr_t>(x) & 15) ? 1 : 0;
-  if (n <= head) {
-    if (tid == 0 && n > 0) x[0] = 0.0;
-    return;
-  }
-  if (tid == 0 && head) x[0] = 0.0;
-  zero_d2* v = reinterpret_cast<zero_d2*>(x + head);
-  const int64_t nv = (n - head) >> 1;
-  const zero_d2 z = {0.0, 0.0};
-  for (int64_t i = tid; i < nv; i += nthr) __builtin_nontemporal_store(z, v + i);
-  if (tid == 0 && ((n - head) & 1)) x[n - 1] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
 }
 
 // J = Jacc + Jacc^T, K = Kacc + Kacc^T, in place (one lane per i <= j).
