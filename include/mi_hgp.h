@@ -109,6 +109,16 @@ int mi_ctx_set_tuning(mi_ctx *ctx, const int32_t *rows, int32_t nrows);
  * streams overlap only if the process runs with GPU_MAX_HW_QUEUES raised to
  * match).  Affects plans created afterwards. */
 int mi_ctx_set_streams(mi_ctx *ctx, int32_t nstreams);
+/* Launch fusion (no counterpart in the reference, which has no device
+ * launches).  on = 1 (the default): a plan that writes the unique array
+ * issues its class launches of fewer than 65,536 shell quartets as fused
+ * launches -- one for the generated register kernels, one per (accumulator
+ * count, wavefronts per block) of the LDS wave kernel -- so latency-bound
+ * classes run side by side instead of one launch each.  on = 0: one launch
+ * per class.  Results are bitwise the same; J/K executes and per-launch
+ * profiles always use one launch per class.  Affects plans created
+ * afterwards. */
+int mi_ctx_set_fusion(mi_ctx *ctx, int32_t on);
 
 /* One-shot drop-in: host basis in, host array out (caller-owned, out_len
  * must be mi_n_unique(nbf)).  Replaces all_twoe_ints_chrr (order CHRR,
@@ -174,11 +184,23 @@ int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *
  * writes nlaunch+1 durations in ms (ms[0] = shell-pair kernel). */
 int mi_plan_profile(mi_plan *plan, double *out_device, int64_t out_len, void *stream, float *ms);
 /* One profiled execute (output written as by mi_plan_execute), then the
- * class launches are reassigned to stream lanes by their measured durations
+ * launch units are reassigned to stream lanes by their measured durations
  * (longest first, each to the least-loaded lane) instead of the built-in cost
  * estimate.  Results do not change; a captured graph is dropped (capture
  * again).  For small, latency-bound plans executed many times. */
 int mi_plan_balance(mi_plan *plan, double *out_device, int64_t out_len, void *stream);
+/* Launch units of a unique-array execute (mi_ctx_set_fusion): unit u is one
+ * class launch or a fused launch of several.  kind3 = (kind, wavefronts per
+ * block, member count): kind 0 = one class launch, 1 = fused register
+ * kernels, 2 = fused LDS wave kernels; members receives up to cap indices
+ * (0..nlaunch-1, as mi_plan_launch_info) of the class launches it covers,
+ * heaviest first. */
+int mi_plan_nunit(const mi_plan *plan);
+int mi_plan_unit_info(const mi_plan *plan, int32_t u, int32_t *kind3, int32_t *members, int32_t cap);
+/* One execute issued as mi_plan_execute issues it, with HIP events between
+ * the units on `stream`; synchronises and writes nunit+1 durations in ms
+ * (ms[0] = shell-pair kernel). */
+int mi_plan_profile_units(mi_plan *plan, double *out_device, int64_t out_len, void *stream, float *ms);
 /* The same for one Coulomb/exchange execute (device arrays as in
  * mi_plan_jk_execute); ms[0] includes the Schwarz pass, ms[1] the zeroing
  * of J and K. */

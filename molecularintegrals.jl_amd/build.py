@@ -32,11 +32,14 @@ def build(verbose: bool = True) -> str:
     run([sys.executable, os.path.join(CSRC, "gen_hgp.py"), "--out", gen_dir])
     objs = []
     jobs = []
-    for src in ("mi_kernels.hip", "mi_host.cpp", "mi_server.cpp"):
-        obj = os.path.join(gen_dir, src + ".o")
-        cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if src.endswith(".cpp"):
-            cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    # the fused register kernels: mi_fused.hip once per fusion group (the
+    # heaviest groups first, they take longest to compile)
+    units = [("mi_fused.hip", [f"-DMI_FUSED_GROUP={g}"], f"mi_fused_{g}") for g in (4, 3, 2, 1, 0)]
+    units += [(src, [], src) for src in ("mi_kernels.hip", "mi_host.cpp", "mi_server.cpp")]
+    for src, defs, stem in units:
+        obj = os.path.join(gen_dir, stem + ".o")
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmd = [HIPCC] + COMMON + defs + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
         jobs.append(subprocess.Popen(cmd))
         objs.append(obj)
     for j in jobs:

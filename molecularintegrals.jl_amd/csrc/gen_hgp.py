@@ -115,14 +115,16 @@ def emit_rkernel(c):
     w = L.append
     w(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}): amax={amax} cmax={cmax} mmax={mmax}, "
       f"{g.count_entries()} VRR entries, {len(g.targets)} contracted (e0|f0), {len(h.nodes)} HRR nodes")
-    w(f"__global__ void __launch_bounds__(256) hgp_r_{name}(const EriArgs args) {{")
-    w(f"  __shared__ double sboys[BOYS_NROW * 8];")
+    # The class body takes its block index within the class launch (bid) and
+    # the block's Boys slice area, so the single-class kernel hgp_r_<class> and
+    # the fused multi-class kernel hgp_rf_kernel run the same code.
+    w(f"__device__ __forceinline__ void hgp_r_body_{name}(const EriArgs& args, double* sboys, int bid) {{")
     w(f"  stage_boys_slice(sboys, args.boys, {mmax - 1});")
     w(f"  QuartetCtx qc;")
     # Lanes without a quartet (past the task's end, or screened) return, except
     # in the J/K build: there every lane joins the wavefront's run reduction
     # (jk_block_regs) with a zero block.
-    w(f"  bool live = decode_quartet(args, qc);")
+    w(f"  bool live = decode_quartet(args, qc, bid);")
     w(f"  if (!live && args.mode != MI_MODE_JK) return;")
     w(f"  const PairTopo P = args.topo[qc.bra];")
     w(f"  const PairTopo Q = args.topo[qc.ket];")
@@ -238,6 +240,12 @@ def emit_rkernel(c):
         w("    }")
     w("  }")
     w("}")
+    w("#ifndef MI_R_BODIES_ONLY")
+    w(f"__global__ void __launch_bounds__(256) hgp_r_{name}(const EriArgs args) {{")
+    w(f"  __shared__ double sboys[BOYS_NROW * 8];")
+    w(f"  hgp_r_body_{name}(args, sboys, blockIdx.x);")
+    w("}")
+    w("#endif")
     return "\n".join(L), g, h
 
 
@@ -403,6 +411,23 @@ def is_register_class(c):
     return len(g.targets) <= 100 and g.count_entries() <= 320
 
 
+# Fusion groups of the register classes by total angular momentum (a proxy
+# for their VGPR use: about 70-125 for L <= 2, 130-205 for L = 3, 200-256 for
+# L = 4, the 256 cap above), the heaviest split into d and f classes.
+NRGROUP = 5
+
+
+def register_group(c):
+    L = sum(c)
+    if L <= 2:
+        return 0
+    if L == 3:
+        return 1
+    if L == 4:
+        return 2
+    return 3 if max(c) <= 2 else 4
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -418,12 +443,40 @@ def main():
         parts.append(code)
         parts.append("")
     # dispatch table (class id -> register kernel or null)
+    parts.append("#ifndef MI_R_BODIES_ONLY")
     parts.append("static const void* const kRegisterKernels[] = {")
     for c in classes:
         parts.append(f"  {'(const void*)hgp_r_' + cname(c) if c in rcls else 'nullptr'},  // {cname(c)}")
     parts.append("};")
+    parts.append("#endif")
     with open(os.path.join(args.out, "hgp_rkernels.inc"), "w") as f:
         f.write("\n".join(parts) + "\n")
+    # fused kernels: one launch over several register classes; segment s of
+    # the launch takes blocks [bstart[s], bstart[s+1]) and runs its class body
+    # with its own tasks and block -> task table (heaviest segments first).
+    # Classes are fused within groups of similar register footprint (a fused
+    # kernel takes the VGPRs of its largest body, so light classes keep their
+    # occupancy), one group per translation unit (compiled in parallel).
+    fz = ["// Generated by gen_hgp.py -- do not edit.", "#pragma once", "",
+          f"#define HGP_NRGROUP {NRGROUP}"]
+    for gi in range(NRGROUP):
+        fz += [f"#if defined(MI_FUSED_GROUP) && MI_FUSED_GROUP == {gi}",
+               f"__global__ void __launch_bounds__(256) hgp_rf{gi}_kernel(const EriArgs args, const FusedSegs F) {{",
+               "  __shared__ double sboys[BOYS_NROW * 8];",
+               "  const int s = fused_segment(F, (int)blockIdx.x);",
+               "  EriArgs a = args;",
+               "  a.tasks = args.tasks + F.task_off[s];",
+               "  a.ntasks = F.ntasks[s];",
+               "  a.block_task = args.block_task + F.bt_off[s];",
+               "  const int bid = (int)blockIdx.x - F.bstart[s];",
+               "  switch (F.cls[s]) {"]
+        for k, c in enumerate(classes):
+            if c in rcls and register_group(c) == gi:
+                fz.append(f"    case {k}: hgp_r_body_{cname(c)}(a, sboys, bid); break;")
+        fz += ["    default: break;", "  }", "}", f"static const void* const kFusedRegisterKernel = (const void*)hgp_rf{gi}_kernel;",
+               "#endif"]
+    with open(os.path.join(args.out, "hgp_rfused.inc"), "w") as f:
+        f.write("\n".join(fz) + "\n")
     # LDS-kernel tables: one contiguous blob per class, 16-byte aligned, laid
     # out as [VRR items | HRR items | out words | acc sources], so a block
     # stages it with wide loads.  Only classes whose slots fit 16 bits and whose
@@ -449,16 +502,26 @@ def main():
     t.append(f"#define HGP_NCLASS {len(classes)}")
     t.append(f"#define HGP_MAX_VLEV {maxlev}")
     t.append(f"#define HGP_MAX_HLEV {maxhlev}")
-    t.append("static const WClassDesc kWClassDesc[HGP_NCLASS] = {")
+    rows = []
     for c, d in zip(classes, descs):
         vl = list(d["vlev"]) + [d["vlev"][-1]] * (maxlev - len(d["vlev"]))
         hl = list(d["hlev"]) + [d["hlev"][-1]] * (maxhlev - len(d["hlev"]))
-        t.append(f"  {{{d['la']}, {d['lb']}, {d['lc']}, {d['ld']}, {d['mmax']}, {d['nvrr']}, {d['nhrr']}, "
+        rows.append(f"  {{{d['la']}, {d['lb']}, {d['lc']}, {d['ld']}, {d['mmax']}, {d['nvrr']}, {d['nhrr']}, "
                  f"{d['vt0']}, {len(d['vlev']) - 1}, {len(d['hlev']) - 1}, {d['nacc']}, {d['nout']}, "
                  f"{d['boff']}, {d['bwords']}, {d['gv']}, {d['gh']}, {d['go']}, {d['ga']}, "
                  f"{{{', '.join(map(str, vl))}}}, {{{', '.join(map(str, hl))}}}, "
                  f"{1 if c in rcls else 0}}},  // {cname(c)}")
+    # host copy (launch set-up) and device copy (fused wave launches read the
+    # descriptor of each segment's class)
+    t.append("static const WClassDesc kWClassDesc[HGP_NCLASS] = {")
+    t += rows
     t.append("};")
+    t.append("__constant__ WClassDesc kWClassDescDev[HGP_NCLASS] = {")
+    t += rows
+    t.append("};")
+    t.append("// fusion group of each class's register kernel (-1: none)")
+    t.append("static const int kRegisterGroup[HGP_NCLASS] = {"
+             + ", ".join(str(register_group(c) if c in rcls else -1) for c in classes) + "};")
     t.append("static const uint32_t kWBlob[] __attribute__((aligned(16))) = {")
     for k in range(0, len(BLOB), 12):
         t.append("  " + ", ".join(f"{x}u" for x in BLOB[k:k + 12]) + ",")

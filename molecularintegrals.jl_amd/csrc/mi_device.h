@@ -91,6 +91,29 @@ struct EriArgs {
   unsigned long long* run_ctr;  // J/K runs: next run of this launch (zero before the launch)
 };
 
+// Fused launch: several class launches of one kernel family issued as one
+// grid.  Segment s (one class launch) takes blocks [bstart[s], bstart[s+1])
+// and keeps its own tasks (task_off, ntasks), register-kernel block -> task
+// table (bt_off) and wave-kernel quartet count (nq); within the segment a
+// block runs exactly as block (blockIdx.x - bstart[s]) of the class launch.
+#define MI_FUSE_MAX 24
+struct FusedSegs {
+  int32_t nseg;
+  int32_t bstart[MI_FUSE_MAX + 1];
+  int32_t cls[MI_FUSE_MAX];
+  int32_t task_off[MI_FUSE_MAX];
+  int32_t ntasks[MI_FUSE_MAX];
+  int64_t bt_off[MI_FUSE_MAX];
+  int64_t nq[MI_FUSE_MAX];
+};
+
+// Segment of fused-launch block b (uniform per block; nseg <= MI_FUSE_MAX).
+__device__ __forceinline__ int fused_segment(const FusedSegs& F, int b) {
+  int s = 0;
+  while (s + 1 < F.nseg && b >= F.bstart[s + 1]) ++s;
+  return s;
+}
+
 struct QuartetCtx {
   int bra, ket;
   int G, g;
@@ -259,12 +282,14 @@ __device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, Quar
 // of a launch owns ceil(nq G / 64) whole wavefronts (blockStart = its first
 // wavefront), so a wavefront never spans two tasks, lane groups stay aligned
 // whatever each task's G, and blocks stay full across task boundaries.
-__device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc) {
-  const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x >> 6);
+// bid: the block's index within its class launch (blockIdx.x, or the block's
+// offset in its segment of a fused launch).
+__device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc, int bid) {
+  const int64_t w0 = (int64_t)bid * (blockDim.x >> 6);
   const int64_t w = w0 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   // the task of the block's first wavefront (host table, else a block-uniform search)
-  const int t0 = a.block_task ? a.block_task[blockIdx.x] : find_task(a, w0);
+  const int t0 = a.block_task ? a.block_task[bid] : find_task(a, w0);
   const TaskDesc T = a.tasks[advance_task(a, t0, w)];
   const int G = T.G;
   const int64_t qi = ((w - T.blockStart) * 64 + lane) / G;

@@ -427,6 +427,101 @@ static void assign_lanes(mi_plan* P) {
   }
 }
 
+// Units: one per class launch, except that (with fusion on, for plans that
+// write the unique array) the small register-kernel launches become one fused
+// launch, and so do the small wave-kernel launches of each (accumulator
+// count, wavefronts per block); groups hold at most MI_FUSE_MAX launches, a
+// group of one stays a plain launch.  Launch indices refer to the current
+// order of P->launches.
+static void build_units(mi_plan* P) {
+  mi_ctx* ctx = P->ctx;
+  P->units.clear();
+  const int n = (int)P->launches.size();
+  std::vector<char> used(n, 0);
+  auto add_fused = [&](std::vector<int> idx, int kind, int nw, int acc) {
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int x, int y) { return P->launches[x].cost > P->launches[y].cost; });
+    for (size_t g0 = 0; g0 < idx.size(); g0 += MI_FUSE_MAX) {
+      const size_t g1 = std::min(idx.size(), g0 + MI_FUSE_MAX);
+      if (g1 - g0 < 2) break;
+      Unit U;
+      U.kind = kind;
+      U.nw = nw;
+      U.acc = acc;
+      int64_t blocks = 0;
+      double cmax = 0.0, csum = 0.0;
+      U.segs.nseg = (int32_t)(g1 - g0);
+      for (size_t k = g0; k < g1; ++k) {
+        const Launch& L = P->launches[idx[k]];
+        const int s = (int)(k - g0);
+        U.members.push_back(idx[k]);
+        U.segs.bstart[s] = (int32_t)blocks;
+        U.segs.cls[s] = L.cls;
+        U.segs.task_off[s] = L.task_off;
+        U.segs.ntasks[s] = L.ntasks;
+        U.segs.bt_off[s] = kind == 1 ? L.bt_off : 0;
+        U.segs.nq[s] = L.nq_shell;
+        if (kind == 2) U.lds = std::max(U.lds, mi::wclass_lds_bytes(L.cls, nw));
+        blocks += L.nblocks;
+        cmax = std::max(cmax, L.cost);
+        csum += L.cost;
+        used[idx[k]] = 1;
+      }
+      if (blocks > INT32_MAX) {  // cannot happen for small launches; keep them apart
+        for (int m : U.members) used[m] = 0;
+        continue;
+      }
+      U.segs.bstart[U.segs.nseg] = (int32_t)blocks;
+      U.nblocks = (int)blocks;
+      // members run side by side: about the longest, plus a share of the rest
+      U.cost = cmax + 0.25 * (csum - cmax);
+      P->units.push_back(U);
+    }
+  };
+  if (ctx->fuse && !P->bounds_only && P->order != MI_ORDER_JK) {
+    // register kernels by fusion group (similar VGPR use); wave kernels by
+    // wavefronts per block, in the instantiation of the group's largest
+    // accumulator count (a few more VGPRs for the smaller classes)
+    std::vector<std::vector<int>> reg(8);
+    std::vector<std::vector<int>> wave(5);
+    std::vector<int> wacc(5, 2);
+    for (int i = 0; i < n; ++i) {
+      const Launch& L = P->launches[i];
+      if (L.nq_shell >= MI_SMALL_LAUNCH) continue;
+      const int g = L.var == 1 ? mi::register_group(L.cls) : -1;
+      if (g >= 0 && g < 8) reg[g].push_back(i);
+      if (L.var == 0 && L.nw >= 1 && L.nw <= 4) {
+        wave[L.nw].push_back(i);
+        wacc[L.nw] = std::max(wacc[L.nw], mi::wave_kernel_acc(L.cls));
+      }
+    }
+    for (int g = 0; g < 8; ++g) add_fused(reg[g], 1, 4, g);
+    for (int nw = 1; nw <= 4; ++nw) add_fused(wave[nw], 2, nw, wacc[nw]);
+  }
+  for (int i = 0; i < n; ++i)
+    if (!used[i]) {
+      Unit U;
+      U.members.push_back(i);
+      U.nblocks = P->launches[i].nblocks;
+      U.nw = P->launches[i].nw;
+      U.cost = P->launches[i].cost;
+      P->units.push_back(U);
+    }
+}
+
+// Units by cost, longest first, each to the least-loaded lane (LPT).
+static void assign_unit_lanes(mi_plan* P) {
+  std::stable_sort(P->units.begin(), P->units.end(), [](const Unit& x, const Unit& y) { return x.cost > y.cost; });
+  std::vector<double> load(P->nlanes, 0.0);
+  for (Unit& U : P->units) {
+    int best = 0;
+    for (int k = 1; k < P->nlanes; ++k)
+      if (load[k] < load[best]) best = k;
+    U.lane = best;
+    load[best] += U.cost;
+  }
+}
+
 int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
   int rc;
   const mi_plan::HostData& h = P->h;
@@ -833,6 +928,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   }
   P->nlanes = std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
   assign_lanes(P);
+  build_units(P);
+  assign_unit_lanes(P);
   // Host images of the device buffers; uploaded now, or by the caller of a
   // deferred build (the serving engine admits a plan by its device bytes).
   {
@@ -896,6 +993,20 @@ int mi_plan_mol_offsets(const mi_plan* P, int64_t* offsets) {
 
 }  // extern "C"
 
+// A fused unit: segment tables index from the plan's task and block -> task
+// arrays.
+static int launch_fused(mi_ctx* ctx, const Unit& U, EriArgs a, const mi_plan* P, hipStream_t ls) {
+  a.tasks = P->d_tasks;
+  a.ntasks = 0;
+  a.block_task = P->d_block_task;
+  a.run_ctr = nullptr;
+  if (U.kind == 1)
+    HIP_TRY(ctx, mi::launch_fused_register(U.acc, a, U.segs, U.nblocks, ls));
+  else
+    HIP_TRY(ctx, mi::launch_fused_wave(U.acc, U.nw, a, U.segs, U.lds, U.nblocks, ctx->wtables, ls));
+  return MI_OK;
+}
+
 static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P, hipStream_t ls) {
   a.tasks = P->d_tasks + L.task_off;
   a.ntasks = L.ntasks;
@@ -915,7 +1026,8 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
 // needs bounds, then the class launches (forked over the plan's stream lanes
 // unless ev != NULL, which times each launch on one stream).  jk != NULL
 // accumulates Coulomb/exchange instead of writing the unique array.
-int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev, const JKArgs* jk) {
+int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev, const JKArgs* jk,
+                bool unit_events) {
   if (!P) return MI_EINVAL;
   mi_ctx* ctx = P->ctx;
   if (!jk && !P->bounds_only) {
@@ -978,11 +1090,24 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
       HIP_TRY(ctx, hipStreamWaitEvent(lanes[k], P->ev_fork, 0));
     }
   }
-  for (size_t i = 0; i < P->launches.size(); ++i) {
-    const Launch& L = P->launches[i];
-    int rc = launch_one(ctx, L, a, P, fork ? lanes[L.lane] : s);
-    if (rc) return rc;
-    if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
+  // Unique-array executes issue the plan's units (fused launches included);
+  // J/K and per-launch profiles issue the class launches.
+  const bool use_units = !jk && (!ev || unit_events) && !P->units.empty();
+  if (use_units) {
+    for (size_t i = 0; i < P->units.size(); ++i) {
+      const Unit& U = P->units[i];
+      hipStream_t ls = fork ? lanes[U.lane] : s;
+      int rc = U.kind == 0 ? launch_one(ctx, P->launches[U.members[0]], a, P, ls) : launch_fused(ctx, U, a, P, ls);
+      if (rc) return rc;
+      if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
+    }
+  } else {
+    for (size_t i = 0; i < P->launches.size(); ++i) {
+      const Launch& L = P->launches[i];
+      int rc = launch_one(ctx, L, a, P, fork ? lanes[L.lane] : s);
+      if (rc) return rc;
+      if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
+    }
   }
   if (fork)
     for (int k = 1; k < P->nlanes; ++k) {
@@ -1129,13 +1254,14 @@ int mi_plan_launch_info(const mi_plan* P, int32_t i, int32_t* cls4, int64_t* cou
   return MI_OK;
 }
 
-static int profile_run(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms, const JKArgs* jk) {
+static int profile_run(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms, const JKArgs* jk,
+                       bool units = false) {
   if (!P || !ms) return MI_EINVAL;
   mi_ctx* ctx = P->ctx;
-  const size_t nev = P->launches.size() + 2;
+  const size_t nev = (units ? P->units.size() : P->launches.size()) + 2;
   std::vector<hipEvent_t> ev(nev, nullptr);
   for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
-  int rc = mi_plan_run(P, out, out_len, stream, ev.data(), jk);
+  int rc = mi_plan_run(P, out, out_len, stream, ev.data(), jk, units);
   if (rc == MI_OK) {
     hipError_t e = hipEventSynchronize(ev[nev - 1]);
     if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "event sync failed: %s", hipGetErrorString(e));
@@ -1152,13 +1278,37 @@ int mi_plan_profile(mi_plan* P, double* out, int64_t out_len, void* stream, floa
   return profile_run(P, out, out_len, stream, ms, nullptr);
 }
 
+int mi_plan_profile_units(mi_plan* P, double* out, int64_t out_len, void* stream, float* ms) {
+  return profile_run(P, out, out_len, stream, ms, nullptr, true);
+}
+
+int mi_plan_nunit(const mi_plan* P) { return P ? (int)P->units.size() : -1; }
+
+int mi_plan_unit_info(const mi_plan* P, int32_t u, int32_t* kind3, int32_t* members, int32_t cap) {
+  if (!P || u < 0 || u >= (int)P->units.size() || !kind3 || (cap > 0 && !members)) return MI_EINVAL;
+  const Unit& U = P->units[u];
+  kind3[0] = U.kind;
+  kind3[1] = U.nw;
+  kind3[2] = (int32_t)U.members.size();
+  for (int32_t k = 0; k < cap && k < (int32_t)U.members.size(); ++k) members[k] = U.members[k];
+  return MI_OK;
+}
+
+int mi_ctx_set_fusion(mi_ctx* ctx, int32_t on) {
+  if (!ctx) return MI_EINVAL;
+  ctx->fuse = on != 0;
+  return MI_OK;
+}
+
 int mi_plan_balance(mi_plan* P, double* out, int64_t out_len, void* stream) {
   if (!P) return MI_EINVAL;
-  std::vector<float> ms(P->launches.size() + 1);
-  int rc = profile_run(P, out, out_len, stream, ms.data(), nullptr);
+  // Units are what a unique-array execute issues: time them and reassign
+  // their lanes (class-launch lanes, used by J/K executes, stay as built).
+  std::vector<float> ms(P->units.size() + 1);
+  int rc = profile_run(P, out, out_len, stream, ms.data(), nullptr, true);
   if (rc) return rc;
-  for (size_t i = 0; i < P->launches.size(); ++i) P->launches[i].cost = std::max(ms[i + 1], 1e-6f);
-  assign_lanes(P);
+  for (size_t i = 0; i < P->units.size(); ++i) P->units[i].cost = std::max(ms[i + 1], 1e-6f);
+  assign_unit_lanes(P);
   if (P->gexec) {  // the captured graph has the old lanes
     hipGraphExecDestroy(P->gexec);
     P->gexec = nullptr;
@@ -1184,7 +1334,7 @@ void mi_plan_destroy(mi_plan* P) {
 int mi_plan_stats(const mi_plan* P, int64_t* st, int32_t n) {
   if (!P || !st) return MI_EINVAL;
   const int64_t v[7] = {P->nbf, P->npair, P->npp, P->nq_shell, P->nq_prim, P->n_unique,
-                        (int64_t)P->launches.size() + 1};
+                        (int64_t)P->units.size() + 1};
   for (int i = 0; i < n && i < 7; ++i) st[i] = v[i];
   return MI_OK;
 }

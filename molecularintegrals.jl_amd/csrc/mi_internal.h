@@ -32,6 +32,7 @@ struct mi_ctx {
   size_t lds_limit = 0;
   int ncu = 256;
   int nstreams = 4;                           // class launches spread over this many streams
+  bool fuse = true;                           // fuse small class launches (mi_ctx_set_fusion)
   hipStream_t aux[MI_MAX_STREAMS - 1] = {};   // default-priority streams beside the caller's
   hipStream_t bstream = nullptr;              // Schwarz bounds passes at plan build
   std::vector<TuneEntry> tune;                // [regime][class][bra bucket][ket bucket]
@@ -48,6 +49,22 @@ struct Launch {
   int64_t nq_shell, nq_prim;
   double cost;  // relative device time estimate (longest launches go first)
   int lane;     // stream lane: 0 = caller's stream, 1.. = context aux streams
+};
+
+// Launch unit of a unique-array execute: one class launch (kind 0), or a
+// fused launch of several (kind 1: register kernels, kind 2: wave kernels of
+// one accumulator count and nw wavefronts per block).  Class launches with
+// fewer than MI_SMALL_LAUNCH shell quartets are latency-bound and leave most
+// of the chip idle, so fusing them runs their classes side by side.
+struct Unit {
+  int kind = 0;
+  std::vector<int> members;  // indices into mi_plan::launches, heaviest first
+  int nblocks = 0, nw = 0;
+  int acc = 0;               // kind 1: fusion group; kind 2: accumulator count of the instantiation
+  size_t lds = 0;            // dynamic LDS (fused wave kernels)
+  double cost = 0.0;
+  int lane = 0;
+  FusedSegs segs{};
 };
 
 struct mi_plan {
@@ -73,6 +90,7 @@ struct mi_plan {
   size_t scratch_doubles = 0;
   std::vector<Launch> launches;   // in issue order (cost descending)
   std::vector<Launch> slaunches;  // Schwarz pass (diagonal quartets), run before the main launches
+  std::vector<Unit> units;        // unique-array executes issue these (J/K and profiles: launches)
   bool need_schwarz = false;
   std::vector<int32_t> pair_mol;  // molecule of each pair record
   double tau2 = 0.0, cut2 = 0.0;
@@ -127,4 +145,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
 int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream);
 // One execute on stream (ev: per-launch timing events or nullptr; jk: the
 // Coulomb/exchange arrays or nullptr for the unique array).
-int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev, const JKArgs* jk);
+// unit_events: with ev, time the launch units (fused launches as executed)
+// instead of each class launch.
+int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent_t* ev, const JKArgs* jk,
+                bool unit_events = false);

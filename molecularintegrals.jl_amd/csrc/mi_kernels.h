@@ -36,6 +36,14 @@ int wg_kernel_blocks_per_cu(int c, int nw, bool tl);
 size_t wg_scratch_doubles(int c, int nblocks);
 hipError_t launch_class_wave_global(int c, int nw, bool tl, const EriArgs& args, int nblocks, int64_t nq_launch,
                                     void* wtables, double* scratch, hipStream_t stream);
+// fused launches (segments in F, bases of tasks / block_task in args):
+// register kernels (mi_fused.hip), wave kernels of accumulator count acc
+// (wave_kernel_acc of each member class) with nw wavefronts per block
+hipError_t launch_fused_register(int group, const EriArgs& args, const FusedSegs& F, int nblocks, hipStream_t stream);
+int register_group(int c);  // fusion group of class c's register kernel, -1 if none
+int wave_kernel_acc(int c);
+hipError_t launch_fused_wave(int acc, int nw, const EriArgs& args, const FusedSegs& F, size_t lds, int nblocks,
+                             void* wtables, hipStream_t stream);
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream);
 hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream);
 

@@ -18,6 +18,7 @@
 
 #include "gen/hgp_wtables.inc"
 #include "gen/hgp_rkernels.inc"
+#include "gen/hgp_rfused.inc"  // HGP_NRGROUP (no kernel without MI_FUSED_GROUP)
 
 // ---------------------------------------------------------------------------
 // Shell-pair precompute (reference: the pair part of vrr!'s prefactor block,
@@ -142,10 +143,11 @@ __host__ __device__ __forceinline__ int w_desc_doubles(const WClassDesc& D) { re
 // LDS: [op tables][hrr: nhrr][scales: 4*W_SCL][coefficients: nw * W_CF][vrr: nw * nvrr]
 //      [J/K run sums: na nb + na nc + nb nc][run index: 1], J/K build with jk_run > 0
 // JR: instantiation with the J/K run code (the other keeps the ERI path's registers).
+// bid / nblk: the block's index and the block count of its class launch
+// (blockIdx.x / gridDim.x, or its segment's in a fused launch).
 template <int ACC, bool JR>
-__global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W,
-                                                    int64_t nq_launch) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void w_body(const EriArgs& args, const WClassDesc& D, const WTablesDev& W,
+                                       int64_t nq_launch, int bid, int nblk, double* lds) {
   const int nv = D.vlev[D.nvlev], nh = D.hlev[D.nhlev];
   const uint4* sV = reinterpret_cast<const uint4*>(lds);
   uint2_t* sH = reinterpret_cast<uint2_t*>(lds) + 2 * nv;
@@ -177,7 +179,7 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
   unsigned long long* s_run = jrun ? reinterpret_cast<unsigned long long*>(racc + na * nb + na * nc + nb * nc) : nullptr;
   const int64_t CH = jrun ? args.jk_run : 1;
   int ti = 0;
-  int64_t c0 = jrun ? next_run(args, s_run) : (int64_t)blockIdx.x;
+  int64_t c0 = jrun ? next_run(args, s_run) : (int64_t)bid;
   while (c0 < nq_launch) {
   const int64_t c1 = c0 + CH < nq_launch ? c0 + CH : nq_launch;
   for (int64_t q = c0; q < c1; ++q) {
@@ -305,8 +307,29 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
     __syncthreads();
   }
   if (jrun) jk_run_flush(args, run, racc, na, nb, nc, true, true, true, tid, nthr);
-  c0 = jrun ? next_run(args, s_run) : c0 + gridDim.x;
+  c0 = jrun ? next_run(args, s_run) : c0 + nblk;
   }
+}
+
+template <int ACC, bool JR>
+__global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W,
+                                                    int64_t nq_launch) {
+  extern __shared__ double lds[];
+  w_body<ACC, JR>(args, D, W, nq_launch, (int)blockIdx.x, (int)gridDim.x, lds);
+}
+
+// Fused launch over several wave-kernel classes of one accumulator count
+// (unique-array build only); class descriptors come from kWClassDescDev, the
+// dynamic LDS is sized for the segment that needs the most.
+template <int ACC>
+__global__ void __launch_bounds__(256) hgp_wf_kernel(const EriArgs args, const FusedSegs F, const WTablesDev W) {
+  extern __shared__ double lds[];
+  const int s = fused_segment(F, (int)blockIdx.x);
+  EriArgs a = args;
+  a.tasks = args.tasks + F.task_off[s];
+  a.ntasks = F.ntasks[s];
+  w_body<ACC, false>(a, kWClassDescDev[F.cls[s]], W, F.nq[s], (int)blockIdx.x - F.bstart[s],
+                     F.bstart[s + 1] - F.bstart[s], lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -624,6 +647,43 @@ hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, in
   return hipLaunchKernel(w_kernel_fn(c, a.jk_run > 0), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
+hipError_t launch_fused_register_g0(const EriArgs&, const FusedSegs&, int, hipStream_t);
+hipError_t launch_fused_register_g1(const EriArgs&, const FusedSegs&, int, hipStream_t);
+hipError_t launch_fused_register_g2(const EriArgs&, const FusedSegs&, int, hipStream_t);
+hipError_t launch_fused_register_g3(const EriArgs&, const FusedSegs&, int, hipStream_t);
+hipError_t launch_fused_register_g4(const EriArgs&, const FusedSegs&, int, hipStream_t);
+static_assert(HGP_NRGROUP == 5, "one launch_fused_register_g<k> per fusion group");
+
+hipError_t launch_fused_register(int group, const EriArgs& args, const FusedSegs& F, int nblocks,
+                                 hipStream_t stream) {
+  switch (group) {
+    case 0: return launch_fused_register_g0(args, F, nblocks, stream);
+    case 1: return launch_fused_register_g1(args, F, nblocks, stream);
+    case 2: return launch_fused_register_g2(args, F, nblocks, stream);
+    case 3: return launch_fused_register_g3(args, F, nblocks, stream);
+    case 4: return launch_fused_register_g4(args, F, nblocks, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int register_group(int c) { return kRegisterGroup[c]; }
+
+int wave_kernel_acc(int c) {
+  const int nacc = kWClassDesc[c].nacc;
+  return nacc <= 64 * 2 ? 2 : (nacc <= 64 * 4 ? 4 : (nacc <= 64 * 8 ? 8 : 16));
+}
+
+hipError_t launch_fused_wave(int acc, int nw, const EriArgs& args, const FusedSegs& F, size_t lds, int nblocks,
+                             void* wtables, hipStream_t stream) {
+  const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
+  WTablesDev W{t->blob};
+  const void* fn = acc <= 2 ? (const void*)hgp_wf_kernel<2>
+                            : (acc <= 4 ? (const void*)hgp_wf_kernel<4>
+                                        : (acc <= 8 ? (const void*)hgp_wf_kernel<8> : (const void*)hgp_wf_kernel<16>));
+  void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<FusedSegs*>(&F), &W};
+  return hipLaunchKernel(fn, dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
+}
+
 // The global-table kernel keeps its HRR area in LDS too when the whole
 // block's LDS stays within this (several blocks per CU remain resident).
 #define WG_HRR_LDS_MAX (64 * 1024)
@@ -703,7 +763,9 @@ hipError_t set_wave_kernel_lds_limit(size_t bytes) {
   const void* wk[] = {(const void*)hgp_w_kernel<2, false>,  (const void*)hgp_w_kernel<4, false>,
                       (const void*)hgp_w_kernel<8, false>,  (const void*)hgp_w_kernel<16, false>,
                       (const void*)hgp_w_kernel<2, true>,   (const void*)hgp_w_kernel<4, true>,
-                      (const void*)hgp_w_kernel<8, true>,   (const void*)hgp_w_kernel<16, true>};
+                      (const void*)hgp_w_kernel<8, true>,   (const void*)hgp_w_kernel<16, true>,
+                      (const void*)hgp_wf_kernel<2>,        (const void*)hgp_wf_kernel<4>,
+                      (const void*)hgp_wf_kernel<8>,        (const void*)hgp_wf_kernel<16>};
   for (const void* f : wk) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;

@@ -21,6 +21,7 @@ MI_PREC_FP64, MI_PREC_MIXED = 0, 1
 EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique", "mi_eri_unique",
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
             "mi_plan_launch_info", "mi_plan_profile", "mi_plan_jk_profile", "mi_plan_balance", "mi_ctx_set_tuning", "mi_ctx_set_streams",
+            "mi_ctx_set_fusion", "mi_plan_nunit", "mi_plan_unit_info", "mi_plan_profile_units",
             "mi_batch_plan_create", "mi_plan_out_len", "mi_plan_mol_offsets", "mi_plan_graph_capture",
             "mi_plan_graph_launch", "mi_jk", "mi_plan_jk_execute", "mi_plan_pair_bounds",
             "mi_server_create", "mi_server_destroy", "mi_server_last_error", "mi_server_submit", "mi_server_step",
@@ -119,6 +120,15 @@ def load():
     L.mi_ctx_set_tuning.restype = ctypes.c_int
     L.mi_ctx_set_streams.argtypes = [vp, ctypes.c_int32]
     L.mi_ctx_set_streams.restype = ctypes.c_int
+    L.mi_ctx_set_fusion.argtypes = [vp, ctypes.c_int32]
+    L.mi_ctx_set_fusion.restype = ctypes.c_int
+    L.mi_plan_nunit.argtypes = [vp]
+    L.mi_plan_nunit.restype = ctypes.c_int
+    L.mi_plan_unit_info.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
+    L.mi_plan_unit_info.restype = ctypes.c_int
+    L.mi_plan_profile_units.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.POINTER(ctypes.c_float)]
+    L.mi_plan_profile_units.restype = ctypes.c_int
     L.mi_batch_plan_create.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.c_int32, ctypes.POINTER(MiOpts),
                                        ctypes.POINTER(vp)]
     L.mi_batch_plan_create.restype = ctypes.c_int

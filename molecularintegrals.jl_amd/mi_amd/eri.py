@@ -83,6 +83,10 @@ class Context:
     def set_streams(self, n: int):
         self.check(self.L.mi_ctx_set_streams(self.h, int(n)))
 
+    def set_fusion(self, on: bool):
+        """Fuse the small class launches of plans built afterwards (default on)."""
+        self.check(self.L.mi_ctx_set_fusion(self.h, 1 if on else 0))
+
     def plan(self, fb: FlatBasis, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
              schwarz_cut=0.0) -> "Plan":
         return Plan(self, [fb], order, precision, tau, schwarz_cut)
@@ -167,6 +171,26 @@ class Plan:
             kern = "R" if k2[0] == 1 else ("W" if k2[0] == 0 else "G") + str(k2[1])
             out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), kern))
         return out
+
+    def units(self) -> list:
+        """Launch units of an execute: (kind, wavefronts per block, [launch
+        indices]); kind 0 = one class launch, 1 = fused register kernels,
+        2 = fused wave kernels."""
+        out = []
+        for u in range(self.ctx.L.mi_plan_nunit(self.h)):
+            k3 = (ctypes.c_int32 * 3)()
+            mem = (ctypes.c_int32 * 64)()
+            self.ctx.check(self.ctx.L.mi_plan_unit_info(self.h, u, k3, mem, 64))
+            out.append((int(k3[0]), int(k3[1]), [int(mem[i]) for i in range(min(int(k3[2]), 64))]))
+        return out
+
+    def profile_units(self, out_ptr: int, stream: int = 0) -> list:
+        """Per-unit durations (ms) of one execute as issued: [pair kernel, unit 0, ...]."""
+        n = self.ctx.L.mi_plan_nunit(self.h) + 1
+        ms = (ctypes.c_float * n)()
+        self.ctx.check(self.ctx.L.mi_plan_profile_units(self.h, ctypes.c_void_p(out_ptr), self.n_unique,
+                                                        ctypes.c_void_p(stream), ms))
+        return [float(x) for x in ms]
 
     def profile(self, out_ptr: int, stream: int = 0) -> list:
         """Per-launch durations (ms) of one execute: [pair kernel, class 0, ...]."""

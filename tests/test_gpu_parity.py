@@ -587,3 +587,56 @@ def test_single_shell_and_size_limits(ctx):
     with pytest.raises(MiError) as e:
         ctx.plan(big)
     assert e.value.code == MI_EUNSUPPORTED
+
+
+def _fused_units_cover_launches(plan):
+    units = plan.units()
+    members = sorted(m for _, _, mem in units for m in mem)
+    assert members == list(range(len(plan.launches())))
+    return units
+
+
+@pytest.mark.parametrize("which", ["split_chrr", "split_iindex", "f", "batch", "mixed_screened"])
+def test_fused_launches_equal_unfused(which):
+    """Small class launches fused into one grid per kernel family (register
+    kernels; wave kernels per accumulator count) give the same bits as one
+    launch per class: each quartet runs the same code with the same
+    reduction order.  Units cover every class launch exactly once."""
+    import torch
+    from mi_amd import Context, configs, flat_from_shells
+    order = O.ORDER_IINDEX if which == "split_iindex" else O.ORDER_CHRR
+    kw = {}
+    if which == "f":
+        rng = np.random.Generator(np.random.PCG64(77))
+        fbs = [flat_from_shells(random_spec(rng, 8, lmax=3, kmax=3))]
+    elif which == "batch":
+        fbs = [m.flat() for m in configs.batch_molecules(11, nmol=5, natoms_lo=3, natoms_hi=8)]
+    else:
+        fbs = [configs.ethane_split_valence(0).flat()]
+        if which == "mixed_screened":
+            kw = dict(precision=1, tau=1e-8, schwarz_cut=1e-12)
+    outs = []
+    for fuse in (False, True):
+        c = Context(0)
+        c.set_fusion(fuse)
+        plan = c.batch_plan(fbs, order=order, **kw) if len(fbs) > 1 else c.plan(fbs[0], order=order, **kw)
+        units = _fused_units_cover_launches(plan)
+        kinds = {k for k, _, _ in units}
+        if not fuse:
+            assert kinds == {0}
+        elif which.startswith("split"):
+            assert 1 in kinds and 2 in kinds, units
+        out = torch.full((plan.n_unique,), float("nan"), dtype=torch.float64, device="cuda")
+        s = torch.cuda.current_stream().cuda_stream
+        plan.execute(out.data_ptr(), s)
+        plan.balance(out.data_ptr(), s)  # unit lanes from measured times
+        out.fill_(float("nan"))
+        plan.execute(out.data_ptr(), s)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+        plan.close()
+        c.close()
+    assert np.array_equal(outs[0].view(np.int64), outs[1].view(np.int64)), which
+    if which in ("split_chrr", "f"):
+        ref = O.eri_unique_hgp(fbs[0], order=order, nthreads=8)
+        assert_eri_close(outs[1], ref, f"fused {which}", fbs[0], order)
