@@ -45,7 +45,8 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a 256 MiB device copy gives); profiles/hbm_traffic_ethane_split_valence_60bf.md
 # holds the commands and the per-kernel table.
-MEASURED_TRAFFIC_BYTES = {"hgp_w_kernel": 1477714}
+MEASURED_TRAFFIC_BYTES = {"hgp_wg_kernel": 1313800, "hgp_rf2_kernel": 14963400, "hgp_wf_kernel": 4995100,
+                          "hgp_rf0_kernel": 11214900, "hgp_rf1_kernel": 11616900, "hgp_w_kernel": 1452900}
 
 
 def log(*a):
@@ -94,6 +95,12 @@ def timed(fn, reps, stream):
     return e0.elapsed_time(e1) / reps
 
 
+def plan_flops(plan):
+    """Algorithmic FLOPs of one execute of a plan (its class launches)."""
+    from mi_amd import flops
+    return sum(flops.class_flops(cls, nqs, nqp) for cls, nqs, nqp, _, _ in plan.launches())
+
+
 def leg_batched(nmol, seed=0):
     """Many molecules of uneven size (3-24 atoms, split-valence recipe):
     the plain path (one plan per molecule, executed one after the other) vs
@@ -120,6 +127,7 @@ def leg_batched(nmol, seed=0):
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     bplan.capture(out.data_ptr())
     ms_fp64 = timed(lambda: bplan.replay(sp), 2, st)
+    flops_fp64 = plan_flops(bplan)
     # plain: per-molecule plans, eager execute into the same slices
     plans = [ctx.plan(fb) for fb in fbs]
     base = out.data_ptr()
@@ -138,6 +146,7 @@ def leg_batched(nmol, seed=0):
     mplan = ctx.batch_plan(fbs, precision=1, tau=1e-8, schwarz_cut=1e-12)
     mplan.capture(out.data_ptr())
     ms_mixed = timed(lambda: mplan.replay(sp), 2, st)
+    flops_mixed = plan_flops(mplan)  # the quartets computed (screened ones never enter a task)
     scratch = torch.empty(int(max(off[m + 1] - off[m] for m in range(nmol))), dtype=torch.float64, device="cuda")
     err_mixed = 0.0
     for m, p in enumerate(plans):
@@ -153,14 +162,21 @@ def leg_batched(nmol, seed=0):
     ctx.close()
     r = {"molecules": nmol, "unique_eris": int(n), "output_gb": n * 8 / 1e9,
          "plain_per_molecule": {"ms": ms_plain, "value": n / (ms_plain * 1e-3)},
-         "batched_fp64_graph": {"ms": ms_fp64, "value": n / (ms_fp64 * 1e-3)},
+         "batched_fp64_graph": {"ms": ms_fp64, "value": n / (ms_fp64 * 1e-3),
+                                "algorithmic_flop": flops_fp64,
+                                "achieved_tflops": flops_fp64 / (ms_fp64 * 1e-3) / 1e12,
+                                "frac_fp64_peak": flops_fp64 / (ms_fp64 * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS},
          "batched_mixed_graph": {"ms": ms_mixed, "value": n / (ms_mixed * 1e-3), "tau": 1e-8,
-                                 "schwarz_cut": 1e-12},
+                                 "schwarz_cut": 1e-12, "algorithmic_flop_computed": flops_mixed,
+                                 "achieved_tflops": flops_mixed / (ms_mixed * 1e-3) / 1e12},
          "speedup_batched_mixed_vs_plain": ms_plain / ms_mixed,
          "max_abs_err_mixed_vs_fp64": err_mixed,
          "max_abs_err_fp64_vs_cpu_oracle_3_smallest": err_oracle}
     log(f"[bench] batched: plain {ms_plain:.1f} ms, batched fp64 {ms_fp64:.1f} ms, mixed {ms_mixed:.1f} ms, "
         f"speedup {ms_plain / ms_mixed:.2f}x, err mixed {err_mixed:.2e}, err vs oracle {err_oracle:.2e}")
+    log(f"[bench] batched: fp64 {flops_fp64 / (ms_fp64 * 1e-3) / 1e12:.2f} TFLOP/s algorithmic "
+        f"({100 * flops_fp64 / (ms_fp64 * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS:.1f}% of FP64 peak), "
+        f"mixed {flops_mixed / (ms_mixed * 1e-3) / 1e12:.2f} TFLOP/s on the quartets computed")
     return r
 
 
@@ -215,9 +231,12 @@ def leg_serving(nreq, rate, seed=0):
     for name, mixed in (("mixed", True), ("plain_fp64", False)):
         r = serve(reqs, mixed=mixed)
         res[name] = r
+        lat = {k: np.array([r.latency_s[q.rid] for q in reqs if q.kind == k]) * 1e3 for k in ("eri", "jk")}
         out[name] = {"value": r.throughput, "unit": "unique ERIs/s", "p50_ms": r.p(0.5) * 1e3,
                      "p99_ms": r.p(0.99) * 1e3, "makespan_s": r.makespan_s, "batches": r.stats["batches"],
-                     "peak_device_gb": r.stats["peak_bytes"] / 1e9}
+                     "peak_device_gb": r.stats["peak_bytes"] / 1e9,
+                     "p50_ms_eri_requests": float(np.median(lat["eri"])) if lat["eri"].size else None,
+                     "p50_ms_jk_requests": float(np.median(lat["jk"])) if lat["jk"].size else None}
         log(f"[bench] serving {name}: p50 {r.p(0.5) * 1e3:.1f} ms p99 {r.p(0.99) * 1e3:.1f} ms "
             f"{r.throughput:.3e} unique ERIs/s, makespan {r.makespan_s:.2f} s")
     err = {"eri": 0.0, "jk": 0.0}
@@ -265,6 +284,7 @@ def main():
 
     import torch
     from mi_amd import Context, configs, flops
+    from mi_amd.hgp_tables import register_group
 
     wl = configs.ethane_split_valence(args.seed)
     fb = wl.flat()
@@ -292,35 +312,47 @@ def main():
     value = n / (ms_per_step * 1e-3)
     log(f"[bench] {ms_per_step * 1e3:.1f} us/step, {value:.3e} unique ERIs/s")
 
-    # Per-launch durations (HIP events between launches, same stream).  The
-    # roofline is taken for the dominant kernel SYMBOL: hgp_r_<class> is one
-    # class, hgp_w_kernel / hgp_wg_kernel serve several classes, so their time
-    # and FLOPs are summed over the launches of one step and the mean launch
-    # duration is total / launches (what rocprofv3 --stats reports per symbol).
+    # Per-unit durations (HIP events between the launch units of an execute,
+    # issued as executed, on the launch stream).  A unit is one class launch
+    # or a fused launch of several classes (hgp_rf<g>_kernel: register
+    # kernels of fusion group g; hgp_wf_kernel: wave kernels).  The roofline
+    # is taken for the dominant kernel SYMBOL: hgp_r_<class> is one class,
+    # the others serve several classes, so their time and FLOPs are summed
+    # over the units of one step and the mean launch duration is
+    # total / launches (what rocprofv3 --stats reports per symbol).
     launches = plan.launches()
+    units = plan.units()
     nprof = max(5, min(args.steps, 50))
-    acc = np.zeros(len(launches) + 1)
+    acc = np.zeros(len(units) + 1)
     for _ in range(nprof):
-        acc += np.array(plan.profile(out.data_ptr(), sptr))
+        acc += np.array(plan.profile_units(out.data_ptr(), sptr))
     mean_ms = acc / nprof
+    cname = lambda cls: "".join("spdf"[x] for x in cls)
     per = []
     step_flops = 0
     sym = {}
-    for i, (cls, nqs, nqp, nblk, kern) in enumerate(launches):
-        f = flops.class_flops(cls, nqs, nqp)
+    for i, (kind, nw, members) in enumerate(units):
+        f = sum(flops.class_flops(launches[m][0], launches[m][1], launches[m][2]) for m in members)
         step_flops += f
-        per.append((mean_ms[i + 1], cls, f, nqs, nqp, kern))
-        name = {"R": "hgp_r_" + "".join("spdf"[x] for x in cls), "W": "hgp_w_kernel", "G": "hgp_wg_kernel"}[kern[0]]
+        kern = launches[members[0]][4]
+        if kind == 1:
+            name = f"hgp_rf{register_group(launches[members[0]][0])}_kernel"
+        elif kind == 2:
+            name = "hgp_wf_kernel"
+        else:
+            name = {"R": "hgp_r_" + cname(launches[members[0]][0]), "W": "hgp_w_kernel",
+                    "G": "hgp_wg_kernel"}[kern[0]]
+        label = " ".join(cname(launches[m][0]) for m in members) + f" [{name}{'' if kind else ' ' + kern}]"
+        per.append((mean_ms[i + 1], label, f))
         t, fl, nl = sym.get(name, (0.0, 0, 0))
         sym[name] = (t + mean_ms[i + 1], fl + f, nl + 1)
     name, (dom_ms_total, dom_flops, dom_n) = max(sym.items(), key=lambda kv: kv[1][0])
     achieved = dom_flops / (dom_ms_total * 1e-3) / 1e12
-    for ms_, cls, f, nqs, nqp, kern in sorted(per, key=lambda x: -x[0])[:8]:
-        log(f"[bench]   {''.join('spdf'[x] for x in cls)} {kern} {ms_ * 1e3:8.2f} us "
-            f"{f / (ms_ * 1e-3) / 1e12:7.3f} TFLOP/s  quartets={nqs} prim={nqp}")
+    for ms_, label, f in sorted(per, key=lambda x: -x[0])[:8]:
+        log(f"[bench]   {ms_ * 1e3:8.2f} us {f / (ms_ * 1e-3) / 1e12:7.3f} TFLOP/s  {label}")
     for nm, (t, fl, nl) in sorted(sym.items(), key=lambda kv: -kv[1][0])[:5]:
         log(f"[bench]   symbol {nm}: {nl} launch(es), {t * 1e3:.2f} us/step, {fl / (t * 1e-3) / 1e12:.3f} TFLOP/s")
-    log(f"[bench]   pair kernel {mean_ms[0] * 1e3:.2f} us; sum of launches {mean_ms.sum() * 1e3:.1f} us")
+    log(f"[bench]   pair kernel {mean_ms[0] * 1e3:.2f} us; sum of units {mean_ms.sum() * 1e3:.1f} us")
 
     # parity spot check of the timed output against the committed fixture
     g = np.load(os.path.join(ROOT, "tests", "golden", "ethane_split_valence_seed0.npz"))

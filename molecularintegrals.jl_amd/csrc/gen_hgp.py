@@ -35,7 +35,7 @@ from collections import OrderedDict
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
 from mi_amd.hgp_tables import (AO2M, LMAX_SHELL, HrrGraph, VrrGraph, cname, lval, nao, ncart,  # noqa: E402
-                               quartet_classes, sdir, shell_aos, sidx, sidx_plus)
+                               quartet_classes, sdir, shell_aos, sidx, sidx_plus, NRGROUP, register_group)
 
 # ---------------------------------------------------------------- R emitter
 
@@ -409,23 +409,6 @@ def pack32(vi, hi, ou, am):
 def is_register_class(c):
     g = VrrGraph(*c)
     return len(g.targets) <= 100 and g.count_entries() <= 320
-
-
-# Fusion groups of the register classes by total angular momentum (a proxy
-# for their VGPR use: about 70-125 for L <= 2, 130-205 for L = 3, 200-256 for
-# L = 4, the 256 cap above), the heaviest split into d and f classes.
-NRGROUP = 5
-
-
-def register_group(c):
-    L = sum(c)
-    if L <= 2:
-        return 0
-    if L == 3:
-        return 1
-    if L == 4:
-        return 2
-    return 3 if max(c) <= 2 else 4
 
 
 def main():

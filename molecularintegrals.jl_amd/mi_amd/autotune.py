@@ -20,7 +20,9 @@ workload's best candidate.  The result is written as hgp_tuning.txt rows
 "class bra_bucket ket_bucket variant G regime", read by mi_create.
 
 Usage (on the GPU):  python -m mi_amd.autotune [--out PATH] [--reps N] [--regimes 0,1,2 [--merge]]
-(--merge keeps the rows of the regimes not tuned from the existing file).
+(--merge keeps the rows of the regimes not tuned from the existing file);
+python -m mi_amd.autotune --refine-fused then refines the small-launch regime
+on whole-execute times with launch fusion (refine_fused).
 """
 from __future__ import annotations
 
@@ -186,6 +188,130 @@ def autotune(workloads, regimes, reps=7, verbose=True):
     return rows
 
 
+def read_rows(path):
+    """Rows (class, bra_bucket, ket_bucket, variant, G, regime) of a tuning file."""
+    rows = []
+    for line in open(path):
+        f = line.split()
+        if not line.startswith("#") and len(f) == 6:
+            rows.append(tuple(int(x) for x in f))
+    return rows
+
+
+def probe_variants(ctx, nclass):
+    """Which kernel variants each class has (probed by setting a row)."""
+    has_reg, w_ok, g_ok = {}, {}, {}
+    for k in range(nclass):
+        try:
+            ctx.set_tuning([(k, 0, 0, 1, 1)])
+            has_reg[k] = True
+        except Exception:
+            has_reg[k] = False
+        w_ok[k], g_ok[k] = {}, {}
+        for nw in (1, 2, 4, 8):
+            for var, ok in ((0, w_ok[k]), (2, g_ok[k])):
+                try:
+                    ctx.set_tuning([(k, 0, 0, var, nw)])
+                    ok[nw] = True
+                except Exception:
+                    ok[nw] = False
+    return has_reg, w_ok, g_ok
+
+
+def step_time(ctx, fb, reps, n=40):
+    """Median over reps of the mean time (ms) of n unique-array executes of
+    fb, as executed (fused units, stream lanes balanced on measured times)."""
+    import torch
+    s = torch.cuda.current_stream()
+    sp = s.cuda_stream
+    plan = ctx.plan(fb, order=1)
+    out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+    for _ in range(5):
+        plan.execute(out.data_ptr(), sp)
+    plan.balance(out.data_ptr(), sp)
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(n):
+            plan.execute(out.data_ptr(), sp)
+        e1.record(s)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) / n)
+    plan.close()
+    return float(np.median(ts))
+
+
+def refine_fused(workloads, rows, reps=5, passes=2, gain=0.015, verbose=True):
+    """Coordinate descent on the small-launch regime (0) with launch fusion:
+    per-class launch times measured one class at a time miss that fused
+    classes share the chip, so each class present in a small workload
+    (heaviest first) tries every kernel choice, and keeps the one that lowers
+    the sum over workloads of whole-execute time / the starting time by more
+    than ``gain``.  Returns the updated rows (other regimes unchanged)."""
+    classes = quartet_classes()
+    ctx = Context(0)
+    has_reg, w_ok, g_ok = probe_variants(ctx, len(classes))
+    cands = [(0, 1), (0, 2), (0, 4), (2, 1), (2, 2), (2, 4), (2, 8)] + [(1, t) for t in TAUS]
+
+    def valid(k, cand):
+        var, tau = cand
+        return (var == 1 and has_reg[k]) or (var == 0 and w_ok[k].get(tau, False)) or \
+            (var == 2 and g_ok[k].get(tau, False))
+
+    def with_choice(rs, k, cand):
+        keep = [r for r in rs if not (r[0] == k and r[5] == 0)]
+        return keep + [r + (0,) for r in rows_for({k: cand}, has_reg, w_ok, g_ok)]
+
+    def times(rs):
+        ctx.set_tuning(rs)
+        return [step_time(ctx, wl, reps) for wl in workloads]
+
+    base = times(rows)
+    score = lambda t: sum(x / b for x, b in zip(t, base))
+    cur = score(base)
+    # classes of the small launches, heaviest first (per-launch times)
+    import torch
+    weight = {}
+    ctx.set_tuning(rows)
+    for wl in workloads:
+        plan = ctx.plan(wl, order=1)
+        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+        prof = plan.profile(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        for i, (cls, nqs, _, _, _) in enumerate(plan.launches()):
+            if nqs < 65536:
+                weight[classes.index(cls)] = weight.get(classes.index(cls), 0.0) + prof[i + 1]
+        plan.close()
+    order = sorted(weight, key=lambda k: -weight[k])
+    name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
+    if verbose:
+        print(f"[refine] start: " + ", ".join(f"{b * 1e3:.1f} us" for b in base), flush=True)
+    for p in range(passes):
+        changed = False
+        for k in order:
+            best, bcand = cur, None
+            for cand in cands:
+                if not valid(k, cand):
+                    continue
+                sc = score(times(with_choice(rows, k, cand)))
+                if sc < best * (1.0 - gain):
+                    best, bcand = sc, cand
+            if bcand is not None:
+                rows = with_choice(rows, k, bcand)
+                cur = best
+                changed = True
+            if verbose:
+                print(f"[refine] pass {p} {''.join('spdf'[x] for x in classes[k])}: "
+                      f"{name(*bcand) if bcand else 'kept'}, score {cur:.3f}", flush=True)
+        if not changed:
+            break
+    if verbose:
+        print(f"[refine] end: " + ", ".join(f"{t * 1e3:.1f} us" for t in times(rows)), flush=True)
+    ctx.close()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
@@ -193,7 +319,16 @@ def main():
     ap.add_argument("--reps", type=int, default=7)
     ap.add_argument("--regimes", default="0,1,2", help="regimes to tune (comma separated)")
     ap.add_argument("--merge", action="store_true", help="keep the other regimes' rows of the existing --out file")
+    ap.add_argument("--refine-fused", action="store_true",
+                    help="refine the small-launch regime of the existing --out file on fused execute times")
     args = ap.parse_args()
+    if args.refine_fused:
+        rng = np.random.Generator(np.random.PCG64(9))
+        wls = [configs.ethane_split_valence(0).flat(),
+               configs.random_molecule(rng, 4, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat()]
+        rows = refine_fused(wls, read_rows(args.out), reps=args.reps)
+        write_rows(args.out, rows)
+        return 0
     want = sorted({int(x) for x in args.regimes.split(",")})
     kept = []
     if args.merge and os.path.exists(args.out):
@@ -221,7 +356,11 @@ def main():
     regimes += [2, 2]
     sel = [i for i, r in enumerate(regimes) if r in want]
     rows = kept + autotune([wls[i] for i in sel], [regimes[i] for i in sel], args.reps)
-    with open(args.out, "w") as f:
+    write_rows(args.out, rows)
+
+
+def write_rows(path, rows):
+    with open(path, "w") as f:
         f.write("# HGP ERI kernel choice per (class, bra bucket, ket bucket, regime): variant (1 register "
                 "kernel, 0 LDS wave kernel, 2 global-table kernel) and G (lanes per quartet, or wavefronts);\n"
                 "# regime 0 = class launches of fewer than 65536 shell quartets, 1 = larger ones, "
@@ -230,7 +369,7 @@ def main():
                 "variant G regime\n")
         for r in rows:
             f.write(" ".join(str(int(x)) for x in r) + "\n")
-    print(f"[autotune] wrote {args.out} ({len(rows)} rows)", flush=True)
+    print(f"[autotune] wrote {path} ({len(rows)} rows)", flush=True)
 
 
 if __name__ == "__main__":

@@ -214,3 +214,22 @@ class HrrGraph:
         return 0
 
 
+
+
+# Fusion groups of the register-kernel classes by total angular momentum (a
+# proxy for their VGPR use: about 70-125 for L <= 2, 130-205 for L = 3,
+# 200-256 for L = 4, the 256 cap above), the heaviest split into d and f
+# classes.  A fused kernel takes the VGPRs of its largest class body, so light
+# classes keep their occupancy.
+NRGROUP = 5
+
+
+def register_group(c):
+    L = sum(c)
+    if L <= 2:
+        return 0
+    if L == 3:
+        return 1
+    if L == 4:
+        return 2
+    return 3 if max(c) <= 2 else 4
