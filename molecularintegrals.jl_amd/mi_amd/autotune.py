@@ -21,8 +21,8 @@ workload's best candidate.  The result is written as hgp_tuning.txt rows
 
 Usage (on the GPU):  python -m mi_amd.autotune [--out PATH] [--reps N] [--regimes 0,1,2 [--merge]]
 (--merge keeps the rows of the regimes not tuned from the existing file);
-python -m mi_amd.autotune --refine-fused then refines the small-launch regime
-on whole-execute times with launch fusion (refine_fused).
+python -m mi_amd.autotune --refine 0 (then --refine 1) refines the small-
+(large-) launch regime on whole-execute times (refine_regime).
 """
 from __future__ import annotations
 
@@ -218,15 +218,17 @@ def probe_variants(ctx, nclass):
     return has_reg, w_ok, g_ok
 
 
-def step_time(ctx, fb, reps, n=40):
+def step_time(ctx, wl, reps, n=40):
     """Median over reps of the mean time (ms) of n unique-array executes of
-    fb, as executed (fused units, stream lanes balanced on measured times)."""
+    workload wl = (FlatBasis or list of them, plan options), as executed
+    (fused units, stream lanes balanced on measured times)."""
     import torch
+    fbs, kw = wl
     s = torch.cuda.current_stream()
     sp = s.cuda_stream
-    plan = ctx.plan(fb, order=1)
+    plan = ctx.batch_plan(fbs, **kw) if isinstance(fbs, list) else ctx.plan(fbs, order=1, **kw)
     out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
-    for _ in range(5):
+    for _ in range(2 if n < 10 else 5):
         plan.execute(out.data_ptr(), sp)
     plan.balance(out.data_ptr(), sp)
     ts = []
@@ -240,16 +242,20 @@ def step_time(ctx, fb, reps, n=40):
         torch.cuda.synchronize()
         ts.append(e0.elapsed_time(e1) / n)
     plan.close()
+    del out
     return float(np.median(ts))
 
 
-def refine_fused(workloads, rows, reps=5, passes=2, gain=0.015, verbose=True):
-    """Coordinate descent on the small-launch regime (0) with launch fusion:
-    per-class launch times measured one class at a time miss that fused
-    classes share the chip, so each class present in a small workload
-    (heaviest first) tries every kernel choice, and keeps the one that lowers
-    the sum over workloads of whole-execute time / the starting time by more
-    than ``gain``.  Returns the updated rows (other regimes unchanged)."""
+def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015, verbose=True, checkpoint=None):
+    """Coordinate descent on one tuning regime over whole-execute times.
+    Per-class launch times, measured one class at a time, miss that classes
+    share the chip (fused small launches; stream lanes), and that the
+    quartets a screened or mixed plan keeps have another contraction profile
+    than the full set.  So each class of the regime present in the workloads
+    (heaviest first) tries every kernel choice and keeps the one that lowers
+    the sum over workloads of execute time / starting time by more than
+    ``gain``.  workloads: (FlatBasis or list of them, plan options).  Returns
+    the updated rows (other regimes unchanged)."""
     classes = quartet_classes()
     ctx = Context(0)
     has_reg, w_ok, g_ok = probe_variants(ctx, len(classes))
@@ -261,32 +267,34 @@ def refine_fused(workloads, rows, reps=5, passes=2, gain=0.015, verbose=True):
             (var == 2 and g_ok[k].get(tau, False))
 
     def with_choice(rs, k, cand):
-        keep = [r for r in rs if not (r[0] == k and r[5] == 0)]
-        return keep + [r + (0,) for r in rows_for({k: cand}, has_reg, w_ok, g_ok)]
+        keep = [r for r in rs if not (r[0] == k and r[5] == regime)]
+        return keep + [r + (regime,) for r in rows_for({k: cand}, has_reg, w_ok, g_ok)]
 
     def times(rs):
         ctx.set_tuning(rs)
-        return [step_time(ctx, wl, reps) for wl in workloads]
+        return [step_time(ctx, wl, reps, n) for wl in workloads]
 
     base = times(rows)
     score = lambda t: sum(x / b for x, b in zip(t, base))
     cur = score(base)
-    # classes of the small launches, heaviest first (per-launch times)
+    # classes of the regime's launches, heaviest first (per-launch times)
     import torch
     weight = {}
     ctx.set_tuning(rows)
-    for wl in workloads:
-        plan = ctx.plan(wl, order=1)
+    for fbs, kw in workloads:
+        plan = ctx.batch_plan(fbs, **kw) if isinstance(fbs, list) else ctx.plan(fbs, order=1, **kw)
         out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
         prof = plan.profile(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
         for i, (cls, nqs, _, _, _) in enumerate(plan.launches()):
-            if nqs < 65536:
-                weight[classes.index(cls)] = weight.get(classes.index(cls), 0.0) + prof[i + 1]
+            if (nqs < 65536) == (regime == 0):
+                k = classes.index(cls)
+                weight[k] = weight.get(k, 0.0) + prof[i + 1]
         plan.close()
+        del out
     order = sorted(weight, key=lambda k: -weight[k])
     name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
     if verbose:
-        print(f"[refine] start: " + ", ".join(f"{b * 1e3:.1f} us" for b in base), flush=True)
+        print(f"[refine] regime {regime} start: " + ", ".join(f"{b:.3f} ms" for b in base), flush=True)
     for p in range(passes):
         changed = False
         for k in order:
@@ -301,13 +309,15 @@ def refine_fused(workloads, rows, reps=5, passes=2, gain=0.015, verbose=True):
                 rows = with_choice(rows, k, bcand)
                 cur = best
                 changed = True
+                if checkpoint:
+                    checkpoint(rows)
             if verbose:
                 print(f"[refine] pass {p} {''.join('spdf'[x] for x in classes[k])}: "
                       f"{name(*bcand) if bcand else 'kept'}, score {cur:.3f}", flush=True)
         if not changed:
             break
     if verbose:
-        print(f"[refine] end: " + ", ".join(f"{t * 1e3:.1f} us" for t in times(rows)), flush=True)
+        print(f"[refine] end: " + ", ".join(f"{t:.3f} ms" for t in times(rows)), flush=True)
     ctx.close()
     return rows
 
@@ -319,14 +329,27 @@ def main():
     ap.add_argument("--reps", type=int, default=7)
     ap.add_argument("--regimes", default="0,1,2", help="regimes to tune (comma separated)")
     ap.add_argument("--merge", action="store_true", help="keep the other regimes' rows of the existing --out file")
-    ap.add_argument("--refine-fused", action="store_true",
-                    help="refine the small-launch regime of the existing --out file on fused execute times")
+    ap.add_argument("--refine", type=int, default=-1, metavar="REGIME",
+                    help="refine regime 0 (small launches, fused) or 1 (large launches) of the existing --out "
+                         "file on whole-execute times")
+    ap.add_argument("--passes", type=int, default=2, help="refinement passes over the classes")
     args = ap.parse_args()
-    if args.refine_fused:
+    if args.refine >= 0:
         rng = np.random.Generator(np.random.PCG64(9))
-        wls = [configs.ethane_split_valence(0).flat(),
-               configs.random_molecule(rng, 4, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat()]
-        rows = refine_fused(wls, read_rows(args.out), reps=args.reps)
+        small = configs.random_molecule(rng, 4, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat()
+        batch = [m.flat() for m in configs.batch_molecules(1, nmol=48)]
+        cluster = configs.random_molecule(rng, 16, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat()
+        mixed = dict(precision=_lib.MI_PREC_MIXED, tau=1e-8, schwarz_cut=1e-12)
+        if args.refine == 0:
+            # small class launches (fused): the 60-function molecule and a
+            # 4-atom cluster-recipe molecule
+            wls, n = [(configs.ethane_split_valence(0).flat(), {}), (small, {})], 40
+        else:
+            # large ones: 48 batched molecules in FP64 and in the mixed,
+            # screened form, and a 16-atom cluster-recipe molecule (mixed)
+            wls, n = [(batch, {}), (batch, mixed), (cluster, mixed)], 3
+        rows = refine_regime(wls, read_rows(args.out), regime=args.refine, reps=args.reps, n=n,
+                             passes=args.passes, checkpoint=lambda r: write_rows(args.out, r))
         write_rows(args.out, rows)
         return 0
     want = sorted({int(x) for x in args.regimes.split(",")})
