@@ -1081,39 +1081,44 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
   }
   // Class launches are independent (disjoint output slots): fork them over the
   // plan's stream lanes and join back into the caller's stream.
-  const bool fork = !ev && P->nlanes > 1;
-  hipStream_t lanes[MI_MAX_STREAMS] = {s};
-  if (fork) {
-    HIP_TRY(ctx, hipEventRecord(P->ev_fork, s));
-    for (int k = 1; k < P->nlanes; ++k) {
-      lanes[k] = ctx->aux[k - 1];
-      HIP_TRY(ctx, hipStreamWaitEvent(lanes[k], P->ev_fork, 0));
-    }
-  }
   // Unique-array executes issue the plan's units (fused launches included);
   // J/K and per-launch profiles issue the class launches.
   const bool use_units = !jk && (!ev || unit_events) && !P->units.empty();
-  if (use_units) {
-    for (size_t i = 0; i < P->units.size(); ++i) {
-      const Unit& U = P->units[i];
-      hipStream_t ls = fork ? lanes[U.lane] : s;
-      int rc = U.kind == 0 ? launch_one(ctx, P->launches[U.members[0]], a, P, ls) : launch_fused(ctx, U, a, P, ls);
-      if (rc) return rc;
-      if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
-    }
-  } else {
-    for (size_t i = 0; i < P->launches.size(); ++i) {
-      const Launch& L = P->launches[i];
-      int rc = launch_one(ctx, L, a, P, fork ? lanes[L.lane] : s);
-      if (rc) return rc;
-      if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
-    }
-  }
+  const int nitem = (int)(use_units ? P->units.size() : P->launches.size());
+  auto lane_of = [&](int i) { return use_units ? P->units[i].lane : P->launches[i].lane; };
+  // Fork/join over the stream lanes.  Host submission costs a few
+  // microseconds per call, so each lane waits for the fork just before its
+  // first launch and records its join just after its last one (lanes without
+  // launches do neither): the first kernels start as early as possible.
+  const bool fork = !ev && P->nlanes > 1;
+  int first[MI_MAX_STREAMS], last[MI_MAX_STREAMS];
+  for (int k = 0; k < MI_MAX_STREAMS; ++k) first[k] = last[k] = -1;
+  bool any_aux = false;
   if (fork)
-    for (int k = 1; k < P->nlanes; ++k) {
-      HIP_TRY(ctx, hipEventRecord(P->ev_join[k], lanes[k]));
-      HIP_TRY(ctx, hipStreamWaitEvent(s, P->ev_join[k], 0));
+    for (int i = 0; i < nitem; ++i) {
+      const int k = lane_of(i);
+      if (first[k] < 0) first[k] = i;
+      last[k] = i;
+      any_aux = any_aux || k > 0;
     }
+  if (any_aux) HIP_TRY(ctx, hipEventRecord(P->ev_fork, s));
+  for (int i = 0; i < nitem; ++i) {
+    const int k = fork ? lane_of(i) : 0;
+    hipStream_t ls = k > 0 ? ctx->aux[k - 1] : s;
+    if (k > 0 && first[k] == i) HIP_TRY(ctx, hipStreamWaitEvent(ls, P->ev_fork, 0));
+    int rc;
+    if (use_units) {
+      const Unit& U = P->units[i];
+      rc = U.kind == 0 ? launch_one(ctx, P->launches[U.members[0]], a, P, ls) : launch_fused(ctx, U, a, P, ls);
+    } else {
+      rc = launch_one(ctx, P->launches[i], a, P, ls);
+    }
+    if (rc) return rc;
+    if (ev) HIP_TRY(ctx, hipEventRecord(ev[i + 2], s));
+    if (k > 0 && last[k] == i) HIP_TRY(ctx, hipEventRecord(P->ev_join[k], ls));
+  }
+  for (int k = 1; fork && k < P->nlanes; ++k)
+    if (last[k] >= 0) HIP_TRY(ctx, hipStreamWaitEvent(s, P->ev_join[k], 0));
   if (jk) HIP_TRY(ctx, mi::launch_symmetrize(jk->J, jk->K, P->nbf, s));
   return MI_OK;
 }
