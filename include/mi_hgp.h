@@ -95,7 +95,8 @@ int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
 
 /* Kernel selection (no counterpart in the reference).  rows: nrows x 6 int32
  * (quartet class, bra contraction bucket, ket contraction bucket, variant
- * [1 generated register kernel, 0 LDS wave kernel, 2 global-table kernel],
+ * [1 generated register kernel, 0 LDS wave kernel, 2 global-table kernel,
+ * 3 LDS wave kernel reading its op tables from global memory],
  * G = lanes per quartet for the register kernel or wavefronts per quartet /
  * block for the wave kernels, regime [0: class launches of fewer than 65536
  * shell quartets, 1: larger ones, 2: launches of MI_ORDER_JK plans (where a
@@ -178,7 +179,8 @@ int mi_plan_nlaunch(const mi_plan *plan);
 /* cls4 = (la,lb,lc,ld); counts3 = (shell quartets, primitive quartets,
  * blocks); kernel2 = (variant, wavefronts per block): variant 1 = generated
  * register kernel (wavefronts 0), 0 = LDS wave kernel, 2 = global-table
- * kernel.  Returns MI_OK or MI_EINVAL on a bad index. */
+ * kernel, 3 = LDS wave kernel with op tables in global memory.  Returns MI_OK
+ * or MI_EINVAL on a bad index. */
 int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *counts3, int32_t *kernel2);
 /* One execute with HIP events between launches on `stream`; synchronises and
  * writes nlaunch+1 durations in ms (ms[0] = shell-pair kernel). */

@@ -109,9 +109,10 @@ bool tuning_row_ok(const mi_ctx* c, int cls, int bb, int kb, int var, int G, int
   if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET) return false;
   if (regime < -1 || regime >= MI_NREGIME) return false;
   if (G < 1 || G > 64 || (G & (G - 1))) return false;
-  if (var < 0 || var > 2) return false;
+  if (var < 0 || var > 3) return false;
   if (var == 1 && !mi::has_register_kernel(cls)) return false;
   if (var == 0 && !mi::wclass_fits_nw(cls, G, c->lds_limit)) return false;
+  if (var == 3 && !mi::wclass_fits_nw(cls, G, c->lds_limit, true)) return false;
   if (var == 2 && !mi::wgclass_fits_nw(cls, G, c->lds_limit)) return false;
   return true;
 }
@@ -330,6 +331,7 @@ static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, 
     }
   }
   if ((variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) ||
+      (variant == 3 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit, true)) ||
       (variant == 2 && !mi::wgclass_fits_nw(t.cls, G, ctx->lds_limit)))
     return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
   t.G = G;
@@ -345,10 +347,11 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
   // work per block come first so the longest blocks start earliest.
   for (int c = 0; c < mi::nclass(); ++c) {
     // vk 0: register kernel (each task its own G); 1-3: wave kernel with
-    // 1, 2, 4 wavefronts; 4-7: global-table kernel with 1, 2, 4, 8 wavefronts
-    for (int vk = 0; vk < 8; ++vk) {
-      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : 2);
-      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (1 << (vk - 4)));
+    // 1, 2, 4 wavefronts; 4-7: global-table kernel with 1, 2, 4, 8
+    // wavefronts; 8-10: wave kernel with global op tables, 1, 2, 4 wavefronts
+    for (int vk = 0; vk < 11; ++vk) {
+      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : 3));
+      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (vk < 8 ? (1 << (vk - 4)) : (1 << (vk - 8))));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
         if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
@@ -383,7 +386,8 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
           P->h.block_task.push_back(ti);
         }
       }
-      if (var == 0) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw));
+      if (var == 0 || var == 3)
+        blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw, var == 3));
       L.tl = false;
       if (var == 2) {
         // tables in LDS only where that costs no resident blocks: a launch
@@ -1014,8 +1018,8 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
   a.run_ctr = (a.mode == MI_MODE_JK && P->d_run_ctr) ? P->d_run_ctr + (&L - P->launches.data()) : nullptr;
   if (L.var == 1)
     HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
-  else if (L.var == 0)
-    HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
+  else if (L.var == 0 || L.var == 3)
+    HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, L.var == 3, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
   else
     HIP_TRY(ctx, mi::launch_class_wave_global(L.cls, L.nw, L.tl, a, L.nblocks, L.nq_shell, ctx->wtables,
                                               P->d_scratch + L.scratch_off, ls));

@@ -18,7 +18,9 @@
 
 // Kernel choice for one (quartet class, bra bucket, ket bucket): variant 1 =
 // generated register kernel with G lanes per quartet, 0 = LDS wave kernel with
-// G wavefronts per quartet, 2 = global-table kernel with G wavefronts per block.
+// G wavefronts per quartet, 2 = global-table kernel with G wavefronts per block,
+// 3 = LDS wave kernel with its op tables read from global memory (less LDS per
+// block, more blocks resident).
 struct TuneEntry {
   int variant = -1;  // -1: not tuned, use the heuristic
   int G = 1;
@@ -40,7 +42,7 @@ struct mi_ctx {
 
 struct Launch {
   int cls;
-  int var;  // 1 register kernel, 0 LDS wave kernel, 2 global-table kernel
+  int var;  // 1 register kernel, 0 LDS wave kernel, 2 global-table kernel, 3 wave kernel, global tables
   int nw;   // wave kernels: wavefronts per block
   bool tl;  // global-table kernel: op tables staged in LDS
   size_t scratch_off;  // global-table kernel: offset of its slice of the plan scratch (doubles)

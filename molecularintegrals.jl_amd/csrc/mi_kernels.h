@@ -19,11 +19,13 @@ hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const int32_t* pp_
                              const int32_t* shell_prim_off, const double* prim_exp, const double* prim_cn,
                              PrimPair* pp, PairGeom* geom, const double2* bounds, hipStream_t stream);
 hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStream_t stream);
-hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
-                             hipStream_t stream);
-int wave_kernel_blocks_per_cu(int c, int nw);
-bool wclass_fits_nw(int c, int nw, size_t lds_limit);
-size_t wclass_lds_bytes(int c, int nw);
+// LDS wave kernel; tg: op tables read from global memory (tuning variant 3)
+// instead of staged into LDS (variant 0)
+hipError_t launch_class_wave(int c, int nw, bool tg, const EriArgs& args, int nblocks, int64_t nq_launch,
+                             void* wtables, hipStream_t stream);
+int wave_kernel_blocks_per_cu(int c, int nw, bool tg = false);
+bool wclass_fits_nw(int c, int nw, size_t lds_limit, bool tg = false);
+size_t wclass_lds_bytes(int c, int nw, bool tg = false);
 bool wclass_fits(int c);
 hipError_t set_wave_kernel_lds_limit(size_t bytes);
 // global-table kernel (variant 2): nw in {1, 2, 4, 8} wavefronts per block,

@@ -145,23 +145,29 @@ __host__ __device__ __forceinline__ int w_desc_doubles(const WClassDesc& D) { re
 // JR: instantiation with the J/K run code (the other keeps the ERI path's registers).
 // bid / nblk: the block's index and the block count of its class launch
 // (blockIdx.x / gridDim.x, or its segment's in a fused launch).
-template <int ACC, bool JR>
+// TG: the op tables are read from the global blob (through L1/L2) instead of
+// being staged into LDS; the block's LDS then starts with the HRR area, so
+// more blocks stay resident per CU (large launches).
+template <int ACC, bool JR, bool TG>
 __device__ __forceinline__ void w_body(const EriArgs& args, const WClassDesc& D, const WTablesDev& W,
                                        int64_t nq_launch, int bid, int nblk, double* lds) {
   const int nv = D.vlev[D.nvlev], nh = D.hlev[D.nhlev];
-  const uint4* sV = reinterpret_cast<const uint4*>(lds);
-  uint2_t* sH = reinterpret_cast<uint2_t*>(lds) + 2 * nv;
-  uint32_t* sO = reinterpret_cast<uint32_t*>(sH + nh);
-  uint32_t* sA = sO + D.nout;
-  double* hrr = lds + w_desc_doubles(D);
+  const uint32_t* tb = TG ? W.blob + 4 * (size_t)D.boff : reinterpret_cast<const uint32_t*>(lds);
+  const uint4* sV = reinterpret_cast<const uint4*>(tb);
+  const uint2_t* sH = reinterpret_cast<const uint2_t*>(tb) + 2 * nv;
+  const uint32_t* sO = reinterpret_cast<const uint32_t*>(sH + nh);
+  const uint32_t* sA = sO + D.nout;
+  double* hrr = lds + (TG ? 0 : w_desc_doubles(D));
   double* scl = hrr + D.nhrr;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int nw = nthr >> 6, wv = tid >> 6, lane = tid & 63;
   double* cf = scl + 4 * W_SCL + wv * W_CF;
   double* vrr_all = scl + 4 * W_SCL + nw * W_CF;
   double* vrr = vrr_all + wv * D.nvrr;
-  stage_chunks(reinterpret_cast<uint4*>(lds), reinterpret_cast<const uint4*>(W.blob) + D.boff, D.bwords / 4);
-  __syncthreads();
+  if (!TG) {
+    stage_chunks(reinterpret_cast<uint4*>(lds), reinterpret_cast<const uint4*>(W.blob) + D.boff, D.bwords / 4);
+    __syncthreads();
+  }
   const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
   const int mtop = D.mmax - 1;
   const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
@@ -311,11 +317,11 @@ __device__ __forceinline__ void w_body(const EriArgs& args, const WClassDesc& D,
   }
 }
 
-template <int ACC, bool JR>
+template <int ACC, bool JR, bool TG>
 __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WClassDesc D, const WTablesDev W,
                                                     int64_t nq_launch) {
   extern __shared__ double lds[];
-  w_body<ACC, JR>(args, D, W, nq_launch, (int)blockIdx.x, (int)gridDim.x, lds);
+  w_body<ACC, JR, TG>(args, D, W, nq_launch, (int)blockIdx.x, (int)gridDim.x, lds);
 }
 
 // Fused launch over several wave-kernel classes of one accumulator count
@@ -328,7 +334,7 @@ __global__ void __launch_bounds__(256) hgp_wf_kernel(const EriArgs args, const F
   EriArgs a = args;
   a.tasks = args.tasks + F.task_off[s];
   a.ntasks = F.ntasks[s];
-  w_body<ACC, false>(a, kWClassDescDev[F.cls[s]], W, F.nq[s], (int)blockIdx.x - F.bstart[s],
+  w_body<ACC, false, false>(a, kWClassDescDev[F.cls[s]], W, F.nq[s], (int)blockIdx.x - F.bstart[s],
                      F.bstart[s + 1] - F.bstart[s], lds);
 }
 
@@ -571,36 +577,40 @@ hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStr
   return hipLaunchKernel(fn, dim3(nblocks), dim3(256), kargs, 0, stream);
 }
 
-size_t wclass_lds_bytes(int c, int nw) {
+size_t wclass_lds_bytes(int c, int nw, bool tg) {
   const WClassDesc& D = kWClassDesc[c];
-  return sizeof(double) * (size_t)(w_desc_doubles(D) + D.nhrr + 4 * W_SCL + (size_t)nw * (W_CF + D.nvrr));
+  return sizeof(double) *
+         (size_t)((tg ? 0 : w_desc_doubles(D)) + D.nhrr + 4 * W_SCL + (size_t)nw * (W_CF + D.nvrr));
 }
 
 // Instantiation holding the class's contracted block: ACC values per lane;
-// jr: with the J/K run code.
-template <bool JR>
+// JR: with the J/K run code; TG: op tables read from global memory.
+template <bool JR, bool TG>
 static const void* w_kernel_fn_t(int c) {
   const int nacc = kWClassDesc[c].nacc;
-  if (nacc <= 64 * 2) return (const void*)hgp_w_kernel<2, JR>;
-  if (nacc <= 64 * 4) return (const void*)hgp_w_kernel<4, JR>;
-  if (nacc <= 64 * 8) return (const void*)hgp_w_kernel<8, JR>;
-  return (const void*)hgp_w_kernel<16, JR>;
+  if (nacc <= 64 * 2) return (const void*)hgp_w_kernel<2, JR, TG>;
+  if (nacc <= 64 * 4) return (const void*)hgp_w_kernel<4, JR, TG>;
+  if (nacc <= 64 * 8) return (const void*)hgp_w_kernel<8, JR, TG>;
+  return (const void*)hgp_w_kernel<16, JR, TG>;
 }
 
-static const void* w_kernel_fn(int c, bool jr = false) { return jr ? w_kernel_fn_t<true>(c) : w_kernel_fn_t<false>(c); }
+static const void* w_kernel_fn(int c, bool jr, bool tg) {
+  return jr ? (tg ? w_kernel_fn_t<true, true>(c) : w_kernel_fn_t<true, false>(c))
+            : (tg ? w_kernel_fn_t<false, true>(c) : w_kernel_fn_t<false, false>(c));
+}
 
-int wave_kernel_blocks_per_cu(int c, int nw) {
+int wave_kernel_blocks_per_cu(int c, int nw, bool tg) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, w_kernel_fn(c), 64 * nw,
-                                                   wclass_lds_bytes(c, nw)) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, w_kernel_fn(c, false, tg), 64 * nw,
+                                                   wclass_lds_bytes(c, nw, tg)) != hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;
 }
 
 bool wclass_fits(int c) { return kWClassDesc[c].boff >= 0 && kWClassDesc[c].nacc <= 64 * W_ACC_MAX; }
 
-bool wclass_fits_nw(int c, int nw, size_t lds_limit) {
-  return wclass_fits(c) && nw >= 1 && nw <= 4 && wclass_lds_bytes(c, nw) <= lds_limit;
+bool wclass_fits_nw(int c, int nw, size_t lds_limit, bool tg) {
+  return wclass_fits(c) && nw >= 1 && nw <= 4 && wclass_lds_bytes(c, nw, tg) <= lds_limit;
 }
 
 // J/K build: LDS for the run sums of the wave kernels (one double per J_ab,
@@ -636,15 +646,15 @@ static EriArgs with_jk_run(const EriArgs& args, int c, const void* fn, int nw, s
   return a;
 }
 
-hipError_t launch_class_wave(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
-                             hipStream_t stream) {
+hipError_t launch_class_wave(int c, int nw, bool tg, const EriArgs& args, int nblocks, int64_t nq_launch,
+                             void* wtables, hipStream_t stream) {
   const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
   WTablesDev W{t->blob};
   const WClassDesc D = kWClassDesc[c];
-  size_t lds = wclass_lds_bytes(c, nw);
-  EriArgs a = with_jk_run(args, c, w_kernel_fn(c, true), nw, lds, nq_launch, nblocks);
+  size_t lds = wclass_lds_bytes(c, nw, tg);
+  EriArgs a = with_jk_run(args, c, w_kernel_fn(c, true, tg), nw, lds, nq_launch, nblocks);
   void* kargs[] = {&a, const_cast<WClassDesc*>(&D), &W, &nq_launch};
-  return hipLaunchKernel(w_kernel_fn(c, a.jk_run > 0), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
+  return hipLaunchKernel(w_kernel_fn(c, a.jk_run > 0, tg), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
 hipError_t launch_fused_register_g0(const EriArgs&, const FusedSegs&, int, hipStream_t);
@@ -760,10 +770,14 @@ hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream) {
 }
 
 hipError_t set_wave_kernel_lds_limit(size_t bytes) {
-  const void* wk[] = {(const void*)hgp_w_kernel<2, false>,  (const void*)hgp_w_kernel<4, false>,
-                      (const void*)hgp_w_kernel<8, false>,  (const void*)hgp_w_kernel<16, false>,
-                      (const void*)hgp_w_kernel<2, true>,   (const void*)hgp_w_kernel<4, true>,
-                      (const void*)hgp_w_kernel<8, true>,   (const void*)hgp_w_kernel<16, true>,
+  const void* wk[] = {(const void*)hgp_w_kernel<2, false, false>, (const void*)hgp_w_kernel<4, false, false>,
+                      (const void*)hgp_w_kernel<8, false, false>, (const void*)hgp_w_kernel<16, false, false>,
+                      (const void*)hgp_w_kernel<2, true, false>,  (const void*)hgp_w_kernel<4, true, false>,
+                      (const void*)hgp_w_kernel<8, true, false>,  (const void*)hgp_w_kernel<16, true, false>,
+                      (const void*)hgp_w_kernel<2, false, true>,  (const void*)hgp_w_kernel<4, false, true>,
+                      (const void*)hgp_w_kernel<8, false, true>,  (const void*)hgp_w_kernel<16, false, true>,
+                      (const void*)hgp_w_kernel<2, true, true>,   (const void*)hgp_w_kernel<4, true, true>,
+                      (const void*)hgp_w_kernel<8, true, true>,   (const void*)hgp_w_kernel<16, true, true>,
                       (const void*)hgp_wf_kernel<2>,        (const void*)hgp_wf_kernel<4>,
                       (const void*)hgp_wf_kernel<8>,        (const void*)hgp_wf_kernel<16>};
   for (const void* f : wk) {

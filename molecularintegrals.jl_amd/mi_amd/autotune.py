@@ -2,8 +2,9 @@
 
 For every quartet class it picks, on the GPU it runs on:
   * the kernel variant: generated register kernel (lane group per shell
-    quartet), the LDS wave kernel (1, 2 or 4 wavefronts per shell quartet) or
-    the global-table kernel (1, 2, 4 or 8 wavefronts per block), and
+    quartet), the LDS wave kernel (1, 2 or 4 wavefronts per shell quartet;
+    op tables staged in LDS or read from global memory) or the global-table
+    kernel (1, 2, 4 or 8 wavefronts per block), and
   * for the register kernel, the lanes per quartet G as a function of the
     contraction depth: G = pow2floor(K_bra * K_ket / tau), clamped to [1, 64],
     with tau chosen per class; for the wave kernels, the wavefront count.
@@ -39,6 +40,9 @@ from .hgp_tables import quartet_classes
 NBUCKET = 5
 BUCKET_K = [1, 3, 7, 20, 48]          # representative pair contraction per bucket
 TAUS = [1, 2, 4, 8, 16, 32, 64, 1 << 30]
+# kernel choices: (0, nw) LDS wave kernel, (2, nw) global-table kernel, (3, nw)
+# LDS wave kernel with global op tables, (1, tau) register kernel
+CANDS = [(0, 1), (0, 2), (0, 4), (2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 2), (3, 4)] + [(1, t) for t in TAUS]
 
 
 def pow2floor(x):
@@ -54,13 +58,21 @@ def fallback(c, w_ok, g_ok):
     return (0, max(w)) if w else (2, 4)
 
 
-def rows_for(cls_choice, has_reg, w_ok, g_ok):
+def available(c, var, tau, has_reg, w_ok, g_ok, t_ok=None):
+    """Whether class c has kernel choice (var, tau)."""
+    if var == 1:
+        return has_reg[c]
+    ok = {0: w_ok, 2: g_ok, 3: t_ok}.get(var)
+    return ok is not None and ok[c].get(int(tau), False)
+
+
+def rows_for(cls_choice, has_reg, w_ok, g_ok, t_ok=None):
     """cls_choice[c] = (1, tau) for the register kernel, (0, nw) for the LDS
-    wave kernel, (2, nw) for the global-table kernel."""
+    wave kernel, (2, nw) for the global-table kernel, (3, nw) for the LDS wave
+    kernel with global op tables."""
     rows = []
     for c, (var, tau) in cls_choice.items():
-        if (var == 1 and not has_reg[c]) or (var == 0 and not w_ok[c].get(int(tau), False)) or \
-                (var == 2 and not g_ok[c].get(int(tau), False)):
+        if not available(c, var, tau, has_reg, w_ok, g_ok, t_ok):
             var, tau = fallback(c, w_ok, g_ok)
         for bb in range(NBUCKET):
             for kb in range(NBUCKET):
@@ -114,33 +126,18 @@ def autotune(workloads, regimes, reps=7, verbose=True):
     ctx = Context(0)
     ctx.set_streams(1)
     # which variants each class has: probe the library by setting a row
-    has_reg, w_ok, g_ok = {}, {}, {}
-    for k in range(len(classes)):
-        try:
-            ctx.set_tuning([(k, 0, 0, 1, 1)])
-            has_reg[k] = True
-        except Exception:
-            has_reg[k] = False
-        w_ok[k], g_ok[k] = {}, {}
-        for nw in (1, 2, 4, 8):
-            for var, ok in ((0, w_ok[k]), (2, g_ok[k])):
-                try:
-                    ctx.set_tuning([(k, 0, 0, var, nw)])
-                    ok[nw] = True
-                except Exception:
-                    ok[nw] = False
-    cands = [(0, 1), (0, 2), (0, 4), (2, 1), (2, 2), (2, 4), (2, 8)] + [(1, t) for t in TAUS]
+    has_reg, w_ok, g_ok, t_ok = probe_variants(ctx, len(classes))
+    cands = CANDS
     # times[w][k][cand]
     times = [{k: {} for k in range(len(classes))} for _ in workloads]
     for var, tau in cands:
         choice = {k: (var, tau) for k in range(len(classes))}
-        rows = rows_for(choice, has_reg, w_ok, g_ok)
+        rows = rows_for(choice, has_reg, w_ok, g_ok, t_ok)
         for wi, wl in enumerate(workloads):
             per = class_times(ctx, wl, rows, reps)
             for cls, t in per.items():
                 k = cid[cls]
-                if (var == 1 and not has_reg[k]) or (var == 0 and not w_ok[k].get(tau)) or \
-                        (var == 2 and not g_ok[k].get(tau)):
+                if not available(k, var, tau, has_reg, w_ok, g_ok, t_ok):
                     continue
                 times[wi][k][(var, tau)] = t
     def pick(wis):
@@ -161,7 +158,7 @@ def autotune(workloads, regimes, reps=7, verbose=True):
                 score[k] = tot
         return best, score
 
-    name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
+    name = lambda v, tt: ({0: "W", 2: "G", 3: "T"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
     rows = []
     tuned = sorted(set(regimes))
     picks = {regime: pick([wi for wi, r in enumerate(regimes) if r == regime]) for regime in tuned}
@@ -183,7 +180,7 @@ def autotune(workloads, regimes, reps=7, verbose=True):
                                 for (v, tt), x in sorted(score[k].items(), key=lambda kv: kv[1])[:4])
                 print(f"[autotune] regime {regime} {''.join('spdf'[x] for x in classes[k])}: "
                       f"{name(*choice[k])}  ({alt})", flush=True)
-        rows += [r + (regime,) for r in rows_for(choice, has_reg, w_ok, g_ok)]
+        rows += [r + (regime,) for r in rows_for(choice, has_reg, w_ok, g_ok, t_ok)]
     ctx.close()
     return rows
 
@@ -200,22 +197,22 @@ def read_rows(path):
 
 def probe_variants(ctx, nclass):
     """Which kernel variants each class has (probed by setting a row)."""
-    has_reg, w_ok, g_ok = {}, {}, {}
+    has_reg, w_ok, g_ok, t_ok = {}, {}, {}, {}
     for k in range(nclass):
         try:
             ctx.set_tuning([(k, 0, 0, 1, 1)])
             has_reg[k] = True
         except Exception:
             has_reg[k] = False
-        w_ok[k], g_ok[k] = {}, {}
+        w_ok[k], g_ok[k], t_ok[k] = {}, {}, {}
         for nw in (1, 2, 4, 8):
-            for var, ok in ((0, w_ok[k]), (2, g_ok[k])):
+            for var, ok in ((0, w_ok[k]), (2, g_ok[k]), (3, t_ok[k])):
                 try:
                     ctx.set_tuning([(k, 0, 0, var, nw)])
                     ok[nw] = True
                 except Exception:
                     ok[nw] = False
-    return has_reg, w_ok, g_ok
+    return has_reg, w_ok, g_ok, t_ok
 
 
 def step_time(ctx, wl, reps, n=40):
@@ -258,17 +255,15 @@ def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015,
     the updated rows (other regimes unchanged)."""
     classes = quartet_classes()
     ctx = Context(0)
-    has_reg, w_ok, g_ok = probe_variants(ctx, len(classes))
-    cands = [(0, 1), (0, 2), (0, 4), (2, 1), (2, 2), (2, 4), (2, 8)] + [(1, t) for t in TAUS]
+    has_reg, w_ok, g_ok, t_ok = probe_variants(ctx, len(classes))
+    cands = CANDS
 
     def valid(k, cand):
-        var, tau = cand
-        return (var == 1 and has_reg[k]) or (var == 0 and w_ok[k].get(tau, False)) or \
-            (var == 2 and g_ok[k].get(tau, False))
+        return available(k, cand[0], cand[1], has_reg, w_ok, g_ok, t_ok)
 
     def with_choice(rs, k, cand):
         keep = [r for r in rs if not (r[0] == k and r[5] == regime)]
-        return keep + [r + (regime,) for r in rows_for({k: cand}, has_reg, w_ok, g_ok)]
+        return keep + [r + (regime,) for r in rows_for({k: cand}, has_reg, w_ok, g_ok, t_ok)]
 
     def times(rs):
         ctx.set_tuning(rs)
@@ -292,7 +287,7 @@ def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015,
         plan.close()
         del out
     order = sorted(weight, key=lambda k: -weight[k])
-    name = lambda v, tt: ({0: "W", 2: "G"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
+    name = lambda v, tt: ({0: "W", 2: "G", 3: "T"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
     if verbose:
         print(f"[refine] regime {regime} start: " + ", ".join(f"{b:.3f} ms" for b in base), flush=True)
     for p in range(passes):

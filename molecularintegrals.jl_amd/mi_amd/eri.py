@@ -168,7 +168,7 @@ class Plan:
             n3 = (ctypes.c_int64 * 3)()
             k2 = (ctypes.c_int32 * 2)()
             self.ctx.check(self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3, k2))
-            kern = "R" if k2[0] == 1 else ("W" if k2[0] == 0 else "G") + str(k2[1])
+            kern = "R" if k2[0] == 1 else {0: "W", 2: "G", 3: "T"}[k2[0]] + str(k2[1])
             out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), kern))
         return out
 

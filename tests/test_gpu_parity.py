@@ -229,11 +229,13 @@ def test_errors_cross_the_abi_as_codes(ctx):
     assert rc == MI_ESIZE
 
 
-@pytest.mark.parametrize("variant,G", [(0, 1), (0, 2), (0, 4), (1, 1), (1, 4), (1, 64), (2, 1), (2, 4), (2, 8)])
+@pytest.mark.parametrize("variant,G", [(0, 1), (0, 2), (0, 4), (1, 1), (1, 4), (1, 64), (2, 1), (2, 4), (2, 8),
+                                       (3, 1), (3, 2), (3, 4)])
 def test_every_kernel_variant(variant, G):
     """Force one kernel variant for every class (the autotuner's search space)
     and check the full array; register kernels also at G lanes per quartet,
-    wave kernels at G wavefronts per quartet or block.  Bases include f shells."""
+    wave kernels at G wavefronts per quartet or block (variant 3: the wave
+    kernel reading its op tables from global memory).  Bases include f shells."""
     from mi_amd import Context, flat_from_shells
     from mi_amd.hgp_tables import quartet_classes
     c = Context(0)
@@ -452,7 +454,7 @@ def _forced_rows(ctx, variant, G):
     return rows
 
 
-@pytest.mark.parametrize("kernels", ["tuned", "R1", "R8", "W1", "W4", "G1", "G4", "eri_layout"])
+@pytest.mark.parametrize("kernels", ["tuned", "R1", "R8", "W1", "W4", "G1", "G4", "T1", "T4", "eri_layout"])
 def test_jk_layout_and_kernel_families(kernels):
     """J/K of a 14-atom split-valence molecule (class launches of thousands of
     quartets, so the wave kernels sum J_ab/K_ac/K_bc over runs of quartets and
@@ -474,7 +476,7 @@ def test_jk_layout_and_kernel_families(kernels):
     Jr, Kr = _ORACLE_CACHE[key]
     c = Context(0)
     if kernels not in ("tuned", "eri_layout"):
-        var = {"R": 1, "W": 0, "G": 2}[kernels[0]]
+        var = {"R": 1, "W": 0, "G": 2, "T": 3}[kernels[0]]
         c.set_tuning(_forced_rows(c, var, int(kernels[1:])))
     order = _lib.MI_ORDER_IINDEX if kernels == "eri_layout" else _lib.MI_ORDER_JK
     plan = c.plan(fb, order=order)
