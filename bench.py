@@ -180,10 +180,56 @@ def leg_batched(nmol, seed=0):
     return r
 
 
+def eri_spot_check(ctx, fb, nsets, seed=0):
+    """Checker for a basis whose unique array is too large to hold (the
+    920-function cluster: 9e10 integrals): nsets random sets of 4 distinct
+    shells, each a sub-basis with the cluster's own norms (its integrals are
+    the cluster's), computed in one batched plan in plain FP64 and in the
+    mixed, screened form, against the CPU oracle of each sub-basis.  Screening
+    and FP32 choices depend only on the quartet's pairs, so the mixed values
+    are those the cluster's own J/K build uses.  Returns max |dERI| per form."""
+    import torch
+    from mi_amd import sub_basis
+    from oracle import oracle as O
+
+    rng = np.random.Generator(np.random.PCG64(seed + 77))
+    subs = [sub_basis(fb, rng.choice(fb.nshell, 4, replace=False)) for _ in range(nsets)]
+    # the reference algorithm in FP64, and in 80-bit: with f shells the FP64
+    # algorithm loses digits to HRR cancellation on some quartets (the
+    # reference keeps a diffuse low-L shell on A), the 80-bit one does not
+    refs = {"fp64_oracle": [O.eri_unique_hgp(sb, order=O.ORDER_CHRR, nthreads=16) for sb in subs],
+            "80bit_oracle": [O.eri_unique_hgp_ext(sb, order=O.ORDER_CHRR, nthreads=16) for sb in subs]}
+    sp = torch.cuda.current_stream().cuda_stream
+    res = {"shell_sets": nsets, "shell_quartets": nsets * 55,
+           "integrals": int(sum(r.size for r in refs["fp64_oracle"]))}
+    for name, kw in (("fp64", {}), ("mixed_screened", dict(precision=1, tau=1e-8, schwarz_cut=1e-12))):
+        plan = ctx.batch_plan(subs, **kw)
+        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+        plan.execute(out.data_ptr(), sp)
+        got = out.cpu().numpy()
+        off = plan.mol_offsets()
+        for rn, rl in refs.items():
+            d = [np.abs(got[off[m]:off[m + 1]] - rl[m]) for m in range(nsets)]
+            res[f"max_abs_err_{name}_vs_{rn}"] = max(float(x.max()) for x in d)
+            # integrals outside the parity bar (1e-12 absolute or 1e-10 relative), and above 1e-10 absolute
+            res[f"n_outside_parity_bar_{name}_vs_{rn}"] = int(sum(
+                np.count_nonzero((x > 1e-12) & (x > 1e-10 * np.abs(r))) for x, r in zip(d, rl)))
+            res[f"n_above_1e-10_{name}_vs_{rn}"] = int(sum(np.count_nonzero(x > 1e-10) for x in d))
+        plan.close()
+    res["max_abs_err_fp64_oracle_vs_80bit_oracle"] = max(
+        float(np.abs(a - b).max()) for a, b in zip(refs["fp64_oracle"], refs["80bit_oracle"]))
+    res["n_above_1e-10_fp64_oracle_vs_80bit_oracle"] = int(sum(
+        np.count_nonzero(np.abs(a - b) > 1e-10) for a, b in zip(refs["fp64_oracle"], refs["80bit_oracle"])))
+    res["max_abs_eri"] = max(float(np.abs(r).max()) for r in refs["fp64_oracle"])
+    return res
+
+
 def leg_jk(seed=0):
     """Direct J/K on the 40-atom cluster (920 functions, l <= 3): plain FP64
     (no screening) vs screening at 1e-12 vs mixed (FP32 VRR below 1e-8) with
-    screening; errors of J and K against the plain FP64 GPU result."""
+    screening; errors of J and K against the plain FP64 GPU result, and the
+    cluster's ERIs checked against the CPU oracle on 1820 sampled sets of 4
+    shells (100,100 shell quartets)."""
     import torch
     from mi_amd import Context, _lib, configs
 
@@ -210,6 +256,14 @@ def leg_jk(seed=0):
     for name in ("fp64_screened", "mixed_screened"):
         res[name]["max_abs_err_J"] = float(np.abs(mats[name][0] - J0).max())
         res[name]["max_abs_err_K"] = float(np.abs(mats[name][1] - K0).max())
+    # ERIs of the cluster against the CPU oracle on sampled shell quartets
+    res["eri_spot_check"] = eri_spot_check(ctx, fb, 1820, seed)
+    sc = res["eri_spot_check"]
+    log(f"[bench] J/K cluster ERI spot check: {sc['shell_quartets']} shell quartets, {sc['integrals']} integrals; "
+        f"max |dERI| vs 80-bit oracle: fp64 {sc['max_abs_err_fp64_vs_80bit_oracle']:.2e}, mixed+screened "
+        f"{sc['max_abs_err_mixed_screened_vs_80bit_oracle']:.2e} (FP64 oracle vs 80-bit: "
+        f"{sc['max_abs_err_fp64_oracle_vs_80bit_oracle']:.2e}); integrals above 1e-10: fp64 "
+        f"{sc['n_above_1e-10_fp64_vs_80bit_oracle']}, mixed {sc['n_above_1e-10_mixed_screened_vs_80bit_oracle']}")
     ctx.close()
     res.update({"nbf": n, "unique_eris": nu, "screening": 1e-12,
                 "speedup_mixed_screened_vs_plain": res["plain_fp64"]["ms"] / res["mixed_screened"]["ms"]})

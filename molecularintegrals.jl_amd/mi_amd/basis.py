@@ -324,3 +324,34 @@ def flat_from_shells(shells_spec) -> FlatBasis:
                      np.array(off, dtype=np.int32), np.array(ex, dtype=np.float64),
                      np.array(co, dtype=np.float64), np.array(pn, dtype=np.float64),
                      np.array(sc, dtype=np.float64))
+
+
+def sub_basis(fb: FlatBasis, shells) -> FlatBasis:
+    """The basis of the given shells of ``fb`` (in that order), with ``fb``'s
+    primitive norms and function scales unchanged, so every integral over its
+    functions equals the same integral of ``fb``.  Used to sample integrals of
+    bases whose whole unique array is too large to hold (a 920-function
+    cluster has 9e10 unique integrals)."""
+    shells = [int(s) for s in shells]
+    nbf_shell = [ncart(int(L)) for L in fb.shell_l]
+    bf0 = np.concatenate([[0], np.cumsum(nbf_shell)]).astype(np.int64)
+    off, ex, co, pn, sc = [0], [], [], [], []
+    for s in shells:
+        p0, p1 = int(fb.shell_prim_off[s]), int(fb.shell_prim_off[s + 1])
+        ex.append(fb.prim_exp[p0:p1])
+        co.append(fb.prim_coef[p0:p1])
+        pn.append(fb.prim_norm[p0:p1])
+        off.append(off[-1] + p1 - p0)
+        sc.append(fb.bf_scale[bf0[s]:bf0[s + 1]])
+    return FlatBasis(shell_xyz=np.ascontiguousarray(fb.shell_xyz[shells], dtype=np.float64),
+                     shell_l=np.ascontiguousarray(fb.shell_l[shells], dtype=np.int32),
+                     shell_prim_off=np.asarray(off, dtype=np.int32),
+                     prim_exp=np.concatenate(ex), prim_coef=np.concatenate(co), prim_norm=np.concatenate(pn),
+                     bf_scale=np.concatenate(sc))
+
+
+def shell_functions(fb: FlatBasis, shells) -> np.ndarray:
+    """Indices (0-based) in ``fb`` of the functions of ``sub_basis(fb, shells)``, in order."""
+    nbf_shell = [ncart(int(L)) for L in fb.shell_l]
+    bf0 = np.concatenate([[0], np.cumsum(nbf_shell)]).astype(np.int64)
+    return np.concatenate([np.arange(bf0[int(s)], bf0[int(s) + 1]) for s in shells])

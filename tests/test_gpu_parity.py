@@ -642,3 +642,36 @@ def test_fused_launches_equal_unfused(which):
     if which in ("split_chrr", "f"):
         ref = O.eri_unique_hgp(fbs[0], order=order, nthreads=8)
         assert_eri_close(outs[1], ref, f"fused {which}", fbs[0], order)
+
+
+def test_cluster_920_sampled_shell_quartets(ctx):
+    """The 40-atom cluster (920 functions, f shells): its unique array (9e10
+    integrals) is never held, so random sets of 4 shells are computed as
+    sub-bases with the cluster's own norms, in one batched plan, against the
+    CPU oracle -- plain FP64 to the parity bar (with the 80-bit oracle
+    arbitrating HRR cancellation), the mixed screened form (FP32 VRR below
+    1e-8, screening 1e-12, as the cluster's J/K build) within 1e-10 of the
+    80-bit oracle (the FP64 oracle itself is up to ~5e-7 off on some f-shell
+    quartets of this cluster)."""
+    import torch
+    from mi_amd import configs, sub_basis
+    wl, _ = configs.cluster_40(0)
+    fb = wl.flat()
+    rng = np.random.Generator(np.random.PCG64(123))
+    subs = [sub_basis(fb, rng.choice(fb.nshell, 4, replace=False)) for _ in range(120)]
+    refs = [O.eri_unique_hgp(sb, order=O.ORDER_CHRR, nthreads=8) for sb in subs]
+    sp = torch.cuda.current_stream().cuda_stream
+    for kw in ({}, dict(precision=1, tau=1e-8, schwarz_cut=1e-12)):
+        plan = ctx.batch_plan(subs, **kw)
+        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+        plan.execute(out.data_ptr(), sp)
+        got = out.cpu().numpy()
+        off = plan.mol_offsets()
+        for m, sb in enumerate(subs):
+            g, r = got[off[m]:off[m + 1]], refs[m]
+            if kw:
+                ext = O.eri_unique_hgp_ext(sb, order=O.ORDER_CHRR, nthreads=8)
+                assert np.abs(g - ext).max() <= 1e-10, (m, float(np.abs(g - ext).max()))
+            else:
+                assert_eri_close(g, r, f"cluster sub-basis {m}", sb, O.ORDER_CHRR)
+        plan.close()

@@ -111,3 +111,40 @@ def test_flop_model_matches_published_table():
              (3, 3): 5920, (4, 2): 4585, (4, 3): 11841, (4, 4): 24887}
     for k, v in table.items():
         assert vrr_ops(*k) == v
+
+
+def test_sub_basis_integrals_equal_the_full_basis():
+    """Every unique integral of sub_basis(fb, shells) equals the integral of
+    the same four functions in fb (oracle, iindex order, both arrays): the
+    helper keeps each shell's primitives, norms and function scales."""
+    from mi_amd import flat_from_shells, iindex, shell_functions, sub_basis
+    rng = np.random.Generator(np.random.PCG64(5))
+    spec = []
+    for s in range(7):
+        L = int(rng.integers(0, 3))
+        k = int(rng.integers(1, 4))
+        e = np.sort(np.exp(rng.uniform(np.log(0.2), np.log(20.0), k)))[::-1]
+        spec.append((rng.normal(0.0, 1.5, 3), L, e.tolist(), rng.uniform(0.1, 1.0, k).tolist()))
+    fb = flat_from_shells(spec)
+    full = O.eri_unique_hgp(fb, order=O.ORDER_IINDEX)
+    shells = [5, 1, 3, 6]
+    sb = sub_basis(fb, shells)
+    fmap = shell_functions(fb, shells) + 1  # 1-based full-basis index of each sub-basis function
+    assert sb.nbf == fmap.size
+    sub = O.eri_unique_hgp(sb, order=O.ORDER_IINDEX)
+    n = sb.nbf
+    got, ref = [], []
+    for i in range(1, n + 1):
+        for j in range(1, i + 1):
+            for k in range(1, n + 1):
+                for l in range(1, k + 1):
+                    if k * (k - 1) // 2 + l > i * (i - 1) // 2 + j:  # each pair of pairs once (ij >= kl)
+                        continue
+                    got.append(sub[iindex(i, j, k, l) - 1])
+                    ref.append(full[iindex(int(fmap[i - 1]), int(fmap[j - 1]), int(fmap[k - 1]),
+                                           int(fmap[l - 1])) - 1])
+    got, ref = np.array(got), np.array(ref)
+    assert got.size == O.n_unique(n)
+    d = np.abs(got - ref)
+    assert np.all((d <= 1e-12) | (d <= 1e-10 * np.abs(ref))), d.max()
+    assert np.abs(ref).max() > 1e-2  # non-trivial values
