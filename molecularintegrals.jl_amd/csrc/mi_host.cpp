@@ -432,11 +432,11 @@ static void assign_lanes(mi_plan* P) {
 }
 
 // Units: one per class launch, except that (with fusion on, for plans that
-// write the unique array) the small register-kernel launches become one fused
-// launch, and so do the small wave-kernel launches of each (accumulator
-// count, wavefronts per block); groups hold at most MI_FUSE_MAX launches, a
-// group of one stays a plain launch.  Launch indices refer to the current
-// order of P->launches.
+// write the unique array) the small register-kernel launches of each fusion
+// group (similar VGPR use) become one fused launch, and so do the small
+// LDS-wave-kernel launches of each wavefront count; groups hold at most
+// MI_FUSE_MAX launches, a group of one stays a plain launch.  Launch indices
+// refer to the current order of P->launches.
 static void build_units(mi_plan* P) {
   mi_ctx* ctx = P->ctx;
   P->units.clear();

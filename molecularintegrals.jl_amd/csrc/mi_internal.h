@@ -54,8 +54,9 @@ struct Launch {
 };
 
 // Launch unit of a unique-array execute: one class launch (kind 0), or a
-// fused launch of several (kind 1: register kernels, kind 2: wave kernels of
-// one accumulator count and nw wavefronts per block).  Class launches with
+// fused launch of several (kind 1: register kernels of one fusion group,
+// kind 2: LDS wave kernels of nw wavefronts per block, run in the
+// instantiation of the members' largest accumulator count).  Class launches with
 // fewer than MI_SMALL_LAUNCH shell quartets are latency-bound and leave most
 // of the chip idle, so fusing them runs their classes side by side.
 struct Unit {

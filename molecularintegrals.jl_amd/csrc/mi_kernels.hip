@@ -324,9 +324,10 @@ __global__ void __launch_bounds__(256) hgp_w_kernel(const EriArgs args, const WC
   w_body<ACC, JR, TG>(args, D, W, nq_launch, (int)blockIdx.x, (int)gridDim.x, lds);
 }
 
-// Fused launch over several wave-kernel classes of one accumulator count
-// (unique-array build only); class descriptors come from kWClassDescDev, the
-// dynamic LDS is sized for the segment that needs the most.
+// Fused launch over several wave-kernel classes of one wavefront count
+// (unique-array build only), in the instantiation of their largest
+// accumulator count; class descriptors come from kWClassDescDev, the dynamic
+// LDS is sized for the segment that needs the most.
 template <int ACC>
 __global__ void __launch_bounds__(256) hgp_wf_kernel(const EriArgs args, const FusedSegs F, const WTablesDev W) {
   extern __shared__ double lds[];
