@@ -163,6 +163,8 @@ def leg_batched(nmol, seed=0):
     r = {"molecules": nmol, "unique_eris": int(n), "output_gb": n * 8 / 1e9,
          "plain_per_molecule": {"ms": ms_plain, "value": n / (ms_plain * 1e-3)},
          "batched_fp64_graph": {"ms": ms_fp64, "value": n / (ms_fp64 * 1e-3),
+                                "output_write_gbs": n * 8 / (ms_fp64 * 1e-3) / 1e9,
+                                "output_write_frac_hbm": n * 8 / (ms_fp64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                 "algorithmic_flop": flops_fp64,
                                 "achieved_tflops": flops_fp64 / (ms_fp64 * 1e-3) / 1e12,
                                 "frac_fp64_peak": flops_fp64 / (ms_fp64 * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS},
@@ -317,6 +319,9 @@ def main():
     ap.add_argument("--batch-mols", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--profile-batch", type=int, default=0, metavar="NMOL",
+                    help="only execute one batched plan of NMOL molecules of 3-24 atoms (seed 1), one stream, "
+                         "for counter profiles, and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -330,6 +335,22 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8,
                           "copies": 5}), flush=True)
+        return
+    if args.profile_batch:
+        import torch
+        from mi_amd import Context, configs
+        fbs = [m.flat() for m in configs.batch_molecules(1, nmol=args.profile_batch)]
+        ctx = Context(0)
+        ctx.set_streams(1)
+        plan = ctx.batch_plan(fbs)
+        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+        plan.execute(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_batch_molecules": args.profile_batch, "unique_eris": plan.n_unique,
+                          "launches": [["".join("spdf"[x] for x in l[0]), l[4], l[1], l[2]]
+                                       for l in plan.launches()]}), flush=True)
+        plan.close()
+        ctx.close()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -438,7 +459,8 @@ def main():
                      "kernel_mean_us": dom_ms_total * 1e3 / dom_n, "kernel_algorithmic_flop": dom_flops / dom_n,
                      "step_algorithmic_flop": step_flops,
                      "step_achieved_tflops": step_flops / (ms_per_step * 1e-3) / 1e12,
-                     "step_output_write_gbs": n * 8 / (ms_per_step * 1e-3) / 1e9},
+                     "step_output_write_gbs": n * 8 / (ms_per_step * 1e-3) / 1e9,
+                     "step_output_write_frac_hbm": n * 8 / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "max_abs_err_vs_golden": err,
     }
     if not args.no_cpu:
