@@ -390,7 +390,8 @@ def main():
     # Per-unit durations (HIP events between the launch units of an execute,
     # issued as executed, on the launch stream).  A unit is one class launch
     # or a fused launch of several classes (hgp_rf<g>_kernel: register
-    # kernels of fusion group g; hgp_wf_kernel: wave kernels).  The roofline
+    # kernels of fusion group g; hgp_wf_kernel: wave kernels; hgp_wgf_kernel:
+    # global-table kernels).  The roofline
     # is taken for the dominant kernel SYMBOL: hgp_r_<class> is one class,
     # the others serve several classes, so their time and FLOPs are summed
     # over the units of one step and the mean launch duration is
@@ -414,6 +415,8 @@ def main():
             name = f"hgp_rf{register_group(launches[members[0]][0])}_kernel"
         elif kind == 2:
             name = "hgp_wf_kernel"
+        elif kind == 3:
+            name = "hgp_wgf_kernel"
         else:
             name = {"R": "hgp_r_" + cname(launches[members[0]][0]), "W": "hgp_w_kernel",
                     "T": "hgp_w_kernel", "G": "hgp_wg_kernel"}[kern[0]]

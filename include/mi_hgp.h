@@ -113,9 +113,9 @@ int mi_ctx_set_streams(mi_ctx *ctx, int32_t nstreams);
 /* Launch fusion (no counterpart in the reference, which has no device
  * launches).  on = 1 (the default): a plan that writes the unique array
  * issues its class launches of fewer than 65,536 shell quartets as fused
- * launches -- one for the generated register kernels, one per (accumulator
- * count, wavefronts per block) of the LDS wave kernel -- so latency-bound
- * classes run side by side instead of one launch each.  on = 0: one launch
+ * launches -- the generated register kernels in groups of similar register
+ * use, the LDS wave kernel and the global-table kernel per wavefront count --
+ * so latency-bound classes run side by side instead of one launch each.  on = 0: one launch
  * per class.  Results are bitwise the same; J/K executes and per-launch
  * profiles always use one launch per class.  Affects plans created
  * afterwards. */
@@ -194,7 +194,8 @@ int mi_plan_balance(mi_plan *plan, double *out_device, int64_t out_len, void *st
 /* Launch units of a unique-array execute (mi_ctx_set_fusion): unit u is one
  * class launch or a fused launch of several.  kind3 = (kind, wavefronts per
  * block, member count): kind 0 = one class launch, 1 = fused register
- * kernels, 2 = fused LDS wave kernels; members receives up to cap indices
+ * kernels, 2 = fused LDS wave kernels, 3 = fused global-table kernels;
+ * members receives up to cap indices
  * (0..nlaunch-1, as mi_plan_launch_info) of the class launches it covers,
  * heaviest first. */
 int mi_plan_nunit(const mi_plan *plan);

@@ -434,7 +434,8 @@ static void assign_lanes(mi_plan* P) {
 // Units: one per class launch, except that (with fusion on, for plans that
 // write the unique array) the small register-kernel launches of each fusion
 // group (similar VGPR use) become one fused launch, and so do the small
-// LDS-wave-kernel launches of each wavefront count; groups hold at most
+// LDS-wave-kernel launches of each wavefront count and the small
+// global-table launches of each wavefront count; groups hold at most
 // MI_FUSE_MAX launches, a group of one stays a plain launch.  Launch indices
 // refer to the current order of P->launches.
 static void build_units(mi_plan* P) {
@@ -463,9 +464,11 @@ static void build_units(mi_plan* P) {
         U.segs.cls[s] = L.cls;
         U.segs.task_off[s] = L.task_off;
         U.segs.ntasks[s] = L.ntasks;
-        U.segs.bt_off[s] = kind == 1 ? L.bt_off : 0;
+        // kind 1: block -> task table; kind 3: the launch's slice of the HRR scratch
+        U.segs.bt_off[s] = kind == 1 ? L.bt_off : (kind == 3 ? (int64_t)L.scratch_off : 0);
         U.segs.nq[s] = L.nq_shell;
         if (kind == 2) U.lds = std::max(U.lds, mi::wclass_lds_bytes(L.cls, nw));
+        if (kind == 3) U.lds = std::max(U.lds, mi::wgclass_lds_bytes(L.cls, false));
         blocks += L.nblocks;
         cmax = std::max(cmax, L.cost);
         csum += L.cost;
@@ -487,7 +490,7 @@ static void build_units(mi_plan* P) {
     // wavefronts per block, in the instantiation of the group's largest
     // accumulator count (a few more VGPRs for the smaller classes)
     std::vector<std::vector<int>> reg(8);
-    std::vector<std::vector<int>> wave(5);
+    std::vector<std::vector<int>> wave(5), glob(9);
     std::vector<int> wacc(5, 2);
     for (int i = 0; i < n; ++i) {
       const Launch& L = P->launches[i];
@@ -498,9 +501,12 @@ static void build_units(mi_plan* P) {
         wave[L.nw].push_back(i);
         wacc[L.nw] = std::max(wacc[L.nw], mi::wave_kernel_acc(L.cls));
       }
+      // global-table launches by wavefronts per block
+      if (L.var == 2 && L.nw >= 1 && L.nw <= 8) glob[L.nw].push_back(i);
     }
     for (int g = 0; g < 8; ++g) add_fused(reg[g], 1, 4, g);
     for (int nw = 1; nw <= 4; ++nw) add_fused(wave[nw], 2, nw, wacc[nw]);
+    for (int nw = 1; nw <= 8; nw *= 2) add_fused(glob[nw], 3, nw, 0);
   }
   for (int i = 0; i < n; ++i)
     if (!used[i]) {
@@ -1006,8 +1012,10 @@ static int launch_fused(mi_ctx* ctx, const Unit& U, EriArgs a, const mi_plan* P,
   a.run_ctr = nullptr;
   if (U.kind == 1)
     HIP_TRY(ctx, mi::launch_fused_register(U.acc, a, U.segs, U.nblocks, ls));
-  else
+  else if (U.kind == 2)
     HIP_TRY(ctx, mi::launch_fused_wave(U.acc, U.nw, a, U.segs, U.lds, U.nblocks, ctx->wtables, ls));
+  else
+    HIP_TRY(ctx, mi::launch_fused_wave_global(U.nw, a, U.segs, U.lds, U.nblocks, ctx->wtables, P->d_scratch, ls));
   return MI_OK;
 }
 

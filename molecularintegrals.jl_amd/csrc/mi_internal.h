@@ -56,7 +56,8 @@ struct Launch {
 // Launch unit of a unique-array execute: one class launch (kind 0), or a
 // fused launch of several (kind 1: register kernels of one fusion group,
 // kind 2: LDS wave kernels of nw wavefronts per block, run in the
-// instantiation of the members' largest accumulator count).  Class launches with
+// instantiation of the members' largest accumulator count, kind 3:
+// global-table kernels of nw wavefronts per block).  Class launches with
 // fewer than MI_SMALL_LAUNCH shell quartets are latency-bound and leave most
 // of the chip idle, so fusing them runs their classes side by side.
 struct Unit {

@@ -358,14 +358,19 @@ __global__ void __launch_bounds__(256) hgp_wf_kernel(const EriArgs args, const F
 // J/K build with args.jk_run > 0: runs of quartets as in the wave kernel.
 // LDS: [scales: 4*W_SCL][vrr: nvrr][hrr: nhrr if HL][tables if TL]
 //      [J/K run sums: na nb + na nc + nb nc][run index: 1], J/K build with jk_run > 0
+// The global-table kernel keeps its HRR area in LDS too when the whole
+// block's LDS stays within this (several blocks per CU remain resident).
+#define WG_HRR_LDS_MAX (64 * 1024)
+
+// bid / nblk: the block's index and the block count of its class launch
+// (blockIdx.x / gridDim.x, or its segment's in a fused launch).
 template <bool HL, bool TL, bool JR>
-__global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const WClassDesc D,
-                                                     const uint32_t* __restrict__ gblob,
-                                                     double* __restrict__ scratch, int64_t nq_launch) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void wg_body(const EriArgs& args, const WClassDesc& D, const uint32_t* __restrict__ gblob,
+                                        double* __restrict__ scratch, int64_t nq_launch, int bid, int nblk,
+                                        double* lds) {
   double* scl = lds;
   double* vrr = lds + 4 * W_SCL;
-  double* hrr = HL ? vrr + D.nvrr : scratch + (size_t)blockIdx.x * D.nhrr;
+  double* hrr = HL ? vrr + D.nvrr : scratch + (size_t)bid * D.nhrr;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const uint32_t* tab = gblob;  // the class's tables start at word D.gv
   double* lds_end = vrr + D.nvrr + (HL ? D.nhrr : 0);
@@ -394,7 +399,7 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
   unsigned long long* s_run = jrun ? reinterpret_cast<unsigned long long*>(racc + na * nb + na * nc + nb * nc) : nullptr;
   const int64_t CH = jrun ? args.jk_run : 1;
   int ti = 0;
-  int64_t c0 = jrun ? next_run(args, s_run) : (int64_t)blockIdx.x;
+  int64_t c0 = jrun ? next_run(args, s_run) : (int64_t)bid;
   while (c0 < nq_launch) {
   const int64_t c1 = c0 + CH < nq_launch ? c0 + CH : nq_launch;
   for (int64_t q = c0; q < c1; ++q) {
@@ -497,8 +502,39 @@ __global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const W
     __syncthreads();
   }
   if (jrun) jk_run_flush(args, run, racc, na, nb, nc, true, true, true, tid, nthr);
-  c0 = jrun ? next_run(args, s_run) : c0 + gridDim.x;
+  c0 = jrun ? next_run(args, s_run) : c0 + nblk;
   }
+}
+
+template <bool HL, bool TL, bool JR>
+__global__ void __launch_bounds__(512) hgp_wg_kernel(const EriArgs args, const WClassDesc D,
+                                                     const uint32_t* __restrict__ gblob,
+                                                     double* __restrict__ scratch, int64_t nq_launch) {
+  extern __shared__ double lds[];
+  wg_body<HL, TL, JR>(args, D, gblob, scratch, nq_launch, (int)blockIdx.x, (int)gridDim.x, lds);
+}
+
+// Fused launch over several global-table classes of one wavefront count
+// (unique-array build only; op tables read through L1/L2); class
+// descriptors come from kWClassDescDev, the dynamic LDS is sized for the
+// segment that needs the most.  A class keeps its contracted block and HRR
+// levels in LDS when they fit WG_HRR_LDS_MAX with the VRR area (as in its
+// own launch), else in its launch's slice of the plan scratch, whose offset
+// the segment carries in bt_off.
+__global__ void __launch_bounds__(512) hgp_wgf_kernel(const EriArgs args, const FusedSegs F,
+                                                      const uint32_t* __restrict__ gblob,
+                                                      double* __restrict__ scratch) {
+  extern __shared__ double lds[];
+  const int s = fused_segment(F, (int)blockIdx.x);
+  EriArgs a = args;
+  a.tasks = args.tasks + F.task_off[s];
+  a.ntasks = F.ntasks[s];
+  const WClassDesc& D = kWClassDescDev[F.cls[s]];
+  const int bid = (int)blockIdx.x - F.bstart[s], nblk = F.bstart[s + 1] - F.bstart[s];
+  if (sizeof(double) * (size_t)(4 * W_SCL + D.nvrr + D.nhrr) <= WG_HRR_LDS_MAX)
+    wg_body<true, false, false>(a, D, gblob, nullptr, F.nq[s], bid, nblk, lds);
+  else
+    wg_body<false, false, false>(a, D, gblob, scratch + F.bt_off[s], F.nq[s], bid, nblk, lds);
 }
 
 // Zero a device array (a kernel, not a memset node, so captured graphs replay it safely).
@@ -695,9 +731,6 @@ hipError_t launch_fused_wave(int acc, int nw, const EriArgs& args, const FusedSe
   return hipLaunchKernel(fn, dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
-// The global-table kernel keeps its HRR area in LDS too when the whole
-// block's LDS stays within this (several blocks per CU remain resident).
-#define WG_HRR_LDS_MAX (64 * 1024)
 
 static size_t wg_table_bytes(int c) {
   const WClassDesc& D = kWClassDesc[c];
@@ -743,6 +776,16 @@ int wg_kernel_blocks_per_cu(int c, int nw, bool tl) {
   return nb > 0 ? nb : 1;
 }
 
+bool wgclass_hrr_in_lds(int c) { return wg_hrr_in_lds(c); }
+
+hipError_t launch_fused_wave_global(int nw, const EriArgs& args, const FusedSegs& F, size_t lds, int nblocks,
+                                    void* wtables, double* scratch, hipStream_t stream) {
+  const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
+  const uint32_t* gb = t->gblob;
+  void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<FusedSegs*>(&F), &gb, &scratch};
+  return hipLaunchKernel((const void*)hgp_wgf_kernel, dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
+}
+
 size_t wg_scratch_doubles(int c, int nblocks) { return wg_hrr_in_lds(c) ? 0 : (size_t)nblocks * kWClassDesc[c].nhrr; }
 
 hipError_t launch_class_wave_global(int c, int nw, bool tl, const EriArgs& args, int nblocks, int64_t nq_launch,
@@ -785,7 +828,8 @@ hipError_t set_wave_kernel_lds_limit(size_t bytes) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
   }
-  const void* gk[] = {(const void*)hgp_wg_kernel<false, false, false>, (const void*)hgp_wg_kernel<true, false, false>,
+  const void* gk[] = {(const void*)hgp_wgf_kernel,
+                      (const void*)hgp_wg_kernel<false, false, false>, (const void*)hgp_wg_kernel<true, false, false>,
                       (const void*)hgp_wg_kernel<true, true, false>,   (const void*)hgp_wg_kernel<false, false, true>,
                       (const void*)hgp_wg_kernel<true, false, true>,   (const void*)hgp_wg_kernel<true, true, true>};
   for (const void* f : gk) {

@@ -175,7 +175,7 @@ class Plan:
     def units(self) -> list:
         """Launch units of an execute: (kind, wavefronts per block, [launch
         indices]); kind 0 = one class launch, 1 = fused register kernels,
-        2 = fused wave kernels."""
+        2 = fused wave kernels, 3 = fused global-table kernels."""
         out = []
         for u in range(self.ctx.L.mi_plan_nunit(self.h)):
             k3 = (ctypes.c_int32 * 3)()
