@@ -45,8 +45,9 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a 256 MiB device copy gives); profiles/hbm_traffic_ethane_split_valence_60bf.md
 # holds the commands and the per-kernel table.
-MEASURED_TRAFFIC_BYTES = {"hgp_wg_kernel": 1313800, "hgp_rf2_kernel": 14963400, "hgp_wf_kernel": 4995100,
-                          "hgp_rf0_kernel": 11214900, "hgp_rf1_kernel": 11616900, "hgp_w_kernel": 1452900}
+MEASURED_TRAFFIC_BYTES = {"hgp_rf2_kernel": 14961800, "hgp_wg_kernel": 1391100, "hgp_wf_kernel": 4991100,
+                          "hgp_w_kernel": 1445700, "hgp_rf0_kernel": 11216400, "hgp_rf1_kernel": 11617100,
+                          "hgp_wgf_kernel": 2572600}
 
 
 def log(*a):
