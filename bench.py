@@ -45,9 +45,8 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a 256 MiB device copy gives); profiles/hbm_traffic_ethane_split_valence_60bf.md
 # holds the commands and the per-kernel table.
-MEASURED_TRAFFIC_BYTES = {"hgp_rf2_kernel": 14961800, "hgp_wg_kernel": 1391100, "hgp_wf_kernel": 4991100,
-                          "hgp_w_kernel": 1445700, "hgp_rf0_kernel": 11216400, "hgp_rf1_kernel": 11617100,
-                          "hgp_wgf_kernel": 2572600}
+MEASURED_TRAFFIC_BYTES = {"hgp_wf_kernel": 3855600, "hgp_rf2_kernel": 14963400, "hgp_rf0_kernel": 11218500,
+                          "hgp_rf1_kernel": 11617300, "hgp_wgf_kernel": 2572000}
 
 
 def log(*a):
