@@ -47,8 +47,7 @@ int wave_kernel_acc(int c);
 hipError_t launch_fused_wave(int acc, int nw, const EriArgs& args, const FusedSegs& F, size_t lds, int nblocks,
                              void* wtables, hipStream_t stream);
 // global-table kernels of nw wavefronts per block, fused; a segment whose
-// HRR area is not in LDS (wgclass_hrr_in_lds) uses scratch + F.bt_off[s]
-bool wgclass_hrr_in_lds(int c);
+// HRR area does not fit in LDS uses scratch + F.bt_off[s]
 hipError_t launch_fused_wave_global(int nw, const EriArgs& args, const FusedSegs& F, size_t lds, int nblocks,
                                     void* wtables, double* scratch, hipStream_t stream);
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream);

@@ -776,8 +776,6 @@ int wg_kernel_blocks_per_cu(int c, int nw, bool tl) {
   return nb > 0 ? nb : 1;
 }
 
-bool wgclass_hrr_in_lds(int c) { return wg_hrr_in_lds(c); }
-
 hipError_t launch_fused_wave_global(int nw, const EriArgs& args, const FusedSegs& F, size_t lds, int nblocks,
                                     void* wtables, double* scratch, hipStream_t stream) {
   const WTablesHost* t = static_cast<const WTablesHost*>(wtables);

@@ -216,10 +216,31 @@ def probe_variants(ctx, nclass):
 
 
 def step_time(ctx, wl, reps, n=40):
-    """Median over reps of the mean time (ms) of n unique-array executes of
-    workload wl = (FlatBasis or list of them, plan options), as executed
-    (fused units, stream lanes balanced on measured times)."""
+    """Median over reps of the mean time (ms) of n executes of workload wl:
+    (FlatBasis or list of them, plan options) for the unique array, as
+    executed (fused units, stream lanes balanced on measured times), or a
+    JKWorkload for Coulomb/exchange builds."""
     import torch
+    if isinstance(wl, JKWorkload):
+        s = torch.cuda.current_stream()
+        sp = s.cuda_stream
+        plan = ctx.plan(wl.fb, order=_lib.MI_ORDER_JK, **wl.kw)
+        D = torch.from_numpy(wl.D).cuda()
+        J = torch.empty_like(D)
+        K = torch.empty_like(D)
+        plan.jk_execute(D.data_ptr(), J.data_ptr(), K.data_ptr(), sp)
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(n):
+                plan.jk_execute(D.data_ptr(), J.data_ptr(), K.data_ptr(), sp)
+            e1.record(s)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) / n)
+        plan.close()
+        return float(np.median(ts))
     fbs, kw = wl
     s = torch.cuda.current_stream()
     sp = s.cuda_stream
@@ -276,20 +297,30 @@ def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015,
     import torch
     weight = {}
     ctx.set_tuning(rows)
-    for fbs, kw in workloads:
-        plan = ctx.batch_plan(fbs, **kw) if isinstance(fbs, list) else ctx.plan(fbs, order=1, **kw)
-        out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
-        prof = plan.profile(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    sp = torch.cuda.current_stream().cuda_stream
+    for wl in workloads:
+        if isinstance(wl, JKWorkload):
+            plan = ctx.plan(wl.fb, order=_lib.MI_ORDER_JK, **wl.kw)
+            D = torch.from_numpy(wl.D).cuda()
+            J = torch.empty_like(D)
+            K = torch.empty_like(D)
+            prof = plan.jk_profile(D.data_ptr(), J.data_ptr(), K.data_ptr(), sp)
+        else:
+            fbs, kw = wl
+            plan = ctx.batch_plan(fbs, **kw) if isinstance(fbs, list) else ctx.plan(fbs, order=1, **kw)
+            out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+            prof = plan.profile(out.data_ptr(), sp)
         for i, (cls, nqs, _, _, _) in enumerate(plan.launches()):
-            if (nqs < 65536) == (regime == 0):
+            # regime 2: every launch of a J/K plan; else small / large launches
+            if regime == 2 or (nqs < 65536) == (regime == 0):
                 k = classes.index(cls)
                 weight[k] = weight.get(k, 0.0) + prof[i + 1]
         plan.close()
-        del out
     order = sorted(weight, key=lambda k: -weight[k])
     name = lambda v, tt: ({0: "W", 2: "G", 3: "T"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
     if verbose:
-        print(f"[refine] regime {regime} start: " + ", ".join(f"{b:.3f} ms" for b in base), flush=True)
+        print(f"[refine] regime {regime} start: " + ", ".join(f"{b:.3f} ms" for b in base) +
+              f"; {len(order)} classes", flush=True)
     for p in range(passes):
         changed = False
         for k in order:
@@ -325,8 +356,8 @@ def main():
     ap.add_argument("--regimes", default="0,1,2", help="regimes to tune (comma separated)")
     ap.add_argument("--merge", action="store_true", help="keep the other regimes' rows of the existing --out file")
     ap.add_argument("--refine", type=int, default=-1, metavar="REGIME",
-                    help="refine regime 0 (small launches, fused) or 1 (large launches) of the existing --out "
-                         "file on whole-execute times")
+                    help="refine regime 0 (small launches, fused), 1 (large launches) or 2 (J/K plans) of the "
+                         "existing --out file on whole-execute times")
     ap.add_argument("--passes", type=int, default=2, help="refinement passes over the classes")
     args = ap.parse_args()
     if args.refine >= 0:
@@ -339,6 +370,11 @@ def main():
             # small class launches (fused): the 60-function molecule and a
             # 4-atom cluster-recipe molecule
             wls, n = [(configs.ethane_split_valence(0).flat(), {}), (small, {})], 40
+        elif args.refine == 2:
+            # J/K plans: the mixed, screened J/K workloads of the per-class tuning
+            jk16 = configs.random_molecule(rng, 16, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT).flat()
+            jk40 = configs.random_molecule(rng, 40, configs.SPLIT_HEAVY, configs.SPLIT_LIGHT).flat()
+            wls, n = [JKWorkload(jk16, 1, **mixed), JKWorkload(jk40, 2, **mixed)], 2
         else:
             # large ones: 48 batched molecules in FP64 and in the mixed,
             # screened form, and a 16-atom cluster-recipe molecule (mixed)
