@@ -148,3 +148,17 @@ def test_sub_basis_integrals_equal_the_full_basis():
     d = np.abs(got - ref)
     assert np.all((d <= 1e-12) | (d <= 1e-10 * np.abs(ref))), d.max()
     assert np.abs(ref).max() > 1e-2  # non-trivial values
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with status 2 and says why
+    (before touching the GPU or the library)."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
