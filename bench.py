@@ -2,10 +2,10 @@
 """Throughput benchmark of the HGP two-electron integral path on one MI355X.
 
 A step is one full pass of the hot path over the 60-function split-valence
-ethane-like workload (seed 0): the shell-pair kernel plus every per-class ERI
-kernel, writing all 1,675,365 unique (ij|kl) in the chrr order into a
-device-resident buffer.  Inputs (basis topology and primitives) are resident
-in HBM before the timed region.
+ethane-like workload (seed 0): the shell-pair kernel plus every class launch
+(issued as fused launch units), writing all 1,675,365 unique (ij|kl) in the
+chrr order into a device-resident buffer.  Inputs (basis topology and
+primitives) are resident in HBM before the timed region.
 
 Prints ONE JSON line.  Extra keys:
   legs          the other workloads, each timed the same way (HIP events,
@@ -14,13 +14,20 @@ Prints ONE JSON line.  Extra keys:
                                       per-molecule plans vs one batched plan
                                       replayed as a graph, FP64 and mixed
                   jk_cluster_40atoms  direct J/K, 920 functions with f shells:
-                                      plain FP64 vs screened vs mixed+screened
+                                      plain FP64 vs screened vs mixed+screened;
+                                      its ERIs checked on 100,100 sampled shell
+                                      quartets against the FP64 and 80-bit
+                                      oracles (eri_spot_check)
                   serving             seeded Poisson schedule through the
                                       continuous-batching server: p50/p99
                                       latency and throughput, FP64 and mixed
-  roofline      dominant kernel: algorithmic FP64 FLOP per launch (FLOP model
-                of mi_amd/flops.py) / its mean HIP-event duration, against the
-                FP64 vector peak (the recurrences do not use MFMA).
+  roofline      dominant kernel symbol: algorithmic FP64 FLOP per launch
+                (FLOP model of mi_amd/flops.py) / its mean HIP-event duration
+                between launch units, against the FP64 vector peak (the
+                recurrences do not use MFMA); traffic from rocprofv3 FETCH/WRITE
+                counters (MEASURED_TRAFFIC_BYTES).
+Other modes: --calibrate-copy MIB (HBM counter calibration), --profile-batch
+NMOL (one batched execute for counter profiles).
   cpu_baseline  the CPU oracle (C restatement of the reference HGP algorithm,
                 OpenMP over shell quartets) on this host, whole workload.
 """
