@@ -675,3 +675,32 @@ def test_cluster_920_sampled_shell_quartets(ctx):
             else:
                 assert_eri_close(g, r, f"cluster sub-basis {m}", sb, O.ORDER_CHRR)
         plan.close()
+
+
+def test_fused_groups_split_at_the_segment_limit():
+    """More small wave-kernel launches than one fused grid holds (24
+    segments): the group splits into launches of at most 24 segments, and
+    every split executes to the same bits as one launch per class."""
+    import torch
+    from mi_amd import Context, flat_from_shells
+    rng = np.random.Generator(np.random.PCG64(31))
+    fb = flat_from_shells(random_spec(rng, 14, lmax=3, kmax=2))
+    outs, sizes = [], []
+    for fuse in (False, True):
+        c = Context(0)
+        c.set_tuning(_forced_rows(c, 0, 1))  # the wave kernel (1 wavefront) wherever it exists
+        c.set_fusion(fuse)
+        plan = c.plan(fb)
+        units = _fused_units_cover_launches(plan)
+        nwave = sum(1 for l in plan.launches() if l[4] == "W1")
+        if fuse:
+            sizes = sorted((len(m) for k, _, m in units if k == 2), reverse=True)
+            assert nwave > 24, nwave
+            assert sizes[0] == 24 and sum(sizes) == nwave, sizes
+        out = torch.full((plan.n_unique,), float("nan"), dtype=torch.float64, device="cuda")
+        plan.execute(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+        plan.close()
+        c.close()
+    assert np.array_equal(outs[0].view(np.int64), outs[1].view(np.int64)), sizes
