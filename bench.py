@@ -426,7 +426,7 @@ def main():
             name = "hgp_wgf_kernel"
         else:
             name = {"R": "hgp_r_" + cname(launches[members[0]][0]), "W": "hgp_w_kernel",
-                    "T": "hgp_w_kernel", "G": "hgp_wg_kernel"}[kern[0]]
+                    "T": "hgp_w_kernel", "G": "hgp_wg_kernel", "X": "hgp_wx_kernel"}[kern[0]]
         label = " ".join(cname(launches[m][0]) for m in members) + f" [{name}{'' if kind else ' ' + kern}]"
         per.append((mean_ms[i + 1], label, f))
         t, fl, nl = sym.get(name, (0.0, 0, 0))

@@ -48,7 +48,7 @@ extern "C" {
 
 /* precision modes */
 #define MI_PREC_FP64 0     /* everything in FP64                               */
-#define MI_PREC_MIXED 1    /* FP32 VRR for quartets whose Schwarz bound < tau  */
+#define MI_PREC_MIXED 1    /* FP32 VRR for quartets whose VRR-output bound < tau */
 
 typedef struct mi_ctx mi_ctx;
 typedef struct mi_plan mi_plan;
@@ -80,7 +80,13 @@ typedef struct {
 typedef struct {
     int32_t order;       /* MI_ORDER_*                                         */
     int32_t precision;   /* MI_PREC_*                                          */
-    double tau;          /* MIXED: quartets with Q_ab*Q_cd < tau use FP32 VRR  */
+    double tau;          /* MIXED: quartets with V_ab*V_cd < tau use FP32 VRR,
+                            V = sqrt(max |(e0|e0)|) over a pair's contracted
+                            VRR outputs (the V of mi_plan_pair_bounds): every
+                            (e0|f0) of the quartet, i.e. every HRR input, is at
+                            most V_ab V_cd.  Unlike schwarz_cut's Q, which bounds
+                            the final integrals, V also bounds what the HRR
+                            cancels, so FP32 rounding stays below ~tau * 1e-7. */
     double schwarz_cut;  /* 0: compute every quartet (plain); >0: skip below   */
     void *stream;        /* hipStream_t for *_device calls; NULL = default     */
 } mi_opts;
@@ -148,7 +154,8 @@ int mi_plan_mol_offsets(const mi_plan *plan, int64_t *offsets);
  * the index algebra of make2JmK (src/ERI.jl:114-128; G = 2J - K there).
  * D, J, K are nbf x nbf row-major.  opts->schwarz_cut > 0 skips quartets with
  * Q_ab Q_cd below it (Q = sqrt(max |(ab|ab)|)); MI_PREC_MIXED runs the VRR of
- * quartets with Q_ab Q_cd < tau in FP32.  Accumulation uses FP64 atomics, so
+ * quartets with V_ab V_cd < tau (VRR-output bounds, as mi_opts.tau) in FP32.
+ * Accumulation uses FP64 atomics, so
  * J and K agree across runs to rounding, not bit for bit.
  * mi_jk: host arrays.  mi_plan_jk_execute: device arrays, async on stream. */
 int mi_jk(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, const double *D, double *J, double *K);

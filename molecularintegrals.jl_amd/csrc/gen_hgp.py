@@ -35,7 +35,8 @@ from collections import OrderedDict
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
 from mi_amd.hgp_tables import (AO2M, LMAX_SHELL, HrrGraph, VrrGraph, cname, lval, nao, ncart,  # noqa: E402
-                               quartet_classes, sdir, shell_aos, sidx, sidx_plus, NRGROUP, register_group)
+                               quartet_classes, sdir, shell_aos, sidx, sidx_plus, NRGROUP, register_group,
+                               is_swapped_class)
 
 # ---------------------------------------------------------------- R emitter
 
@@ -407,6 +408,8 @@ def pack32(vi, hi, ou, am):
 # table-driven wave-cooperative kernel.  Chosen by contracted-accumulator count
 # and VRR size so the register kernels stay within the VGPR budget.
 def is_register_class(c):
+    if is_swapped_class(c):  # rare (see pair_classes): table-driven kernels only
+        return False
     g = VrrGraph(*c)
     return len(g.targets) <= 100 and g.count_entries() <= 320
 

@@ -2,7 +2,9 @@
 //
 // HBM layout (one plan = one basis):
 //   PairTopo[npair]   shell-pair records sorted by (pair class, contraction
-//                     bucket); a pair's shell A has L_A >= L_B.
+//                     bucket); a pair's shell A has L_A >= L_B, except in
+//                     the swapped pair classes (L_A < L_B, the tighter
+//                     shell on A; see orient_pair in mi_host.cpp).
 //   PairGeom[npair]   AB = A - B, written by the pair kernel.
 //   PrimPair[npp]     per primitive pair: P, zeta, PA = P - A, the folded
 //                     prefactor K and 1/(2 zeta); 96-byte rows so one lane
@@ -23,7 +25,7 @@ struct uint4_t { uint32_t x, y, z, w; };
 struct uint2_t { uint32_t x, y; };
 
 struct PairTopo {
-  int32_t shA, shB;   // canonical shells (L_A >= L_B)
+  int32_t shA, shB;   // oriented shells (L_A >= L_B, or a swapped pair class)
   int32_t bfA, bfB;   // first function of each shell (0-based)
   int32_t ppoff, npp; // primitive-pair range [ppoff, ppoff+npp)
   int32_t unused;
@@ -149,10 +151,21 @@ struct WClassDesc {
 // T < 30: the top order from a 7-term Taylor expansion about the nearest grid
 // point T_k = k/16 (|T - T_k| <= 1/32, truncation < 6e-15 relative), exp(-T) =
 // exp(-T_k) * exp(T_k - T) by a degree-8 series, then the downward recursion
-// F_m = (2T F_{m+1} + e^{-T}) / (2m+1).  T >= 30: the asymptotic form without
-// the e^{-T} term, as the reference's branch: F_0 = sqrt(pi/(4T)),
-// F_{m+1} = F_m (2m+1)/(2T).
+// F_m = (2T F_{m+1} + e^{-T}) / (2m+1).  T >= 30: boys_large_T below (the
+// Boys function itself; the reference's farray branch there drops e^{-T}).
 // ---------------------------------------------------------------------------
+// T >= 30: F_0 = sqrt(pi/(4T)) erf(sqrt T) with erfc(x) = e^{-x^2}/(x sqrt pi)
+// (1 - 1/(2x^2) + 3/(4x^4) - ...) (three terms: erfc(sqrt 30) = 1.2e-14 is
+// itself below 1e-13 of F_0, so the truncated series is exact to rounding),
+// then the upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable
+// for T > m + 1/2.  The reference's T >= 30 branch (src/Boys.jl:21-26) keeps
+// only the first term and drops e^{-T}: 1e-4 relative on F_12 at T = 30 and
+// up to 5e-10 relative on f-shell integrals, so it is not reproduced.
+__device__ __forceinline__ double boys_large_T_f0(double T, double eT) {
+  const double rT = 1.0 / T;
+  return 0.88622692545275801365 * sqrt(rT) - 0.5 * eT * rT * fma(rT, fma(0.75, rT, -0.5), 1.0);
+}
+
 __device__ __forceinline__ double exp_small(double d) {
   // e^d for |d| <= 1/32: Taylor to d^8 (remainder < 1e-19)
   double r = 1.0 / 40320.0;
@@ -188,11 +201,11 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
       for (int m = MMAX - 2; m >= 0; --m) F[m] = fma(t2, F[m + 1], eT) * (1.0 / (2 * m + 1));
     }
   } else {
-    const double rT = 1.0 / T;
-    F[0] = 0.88622692545275801365 * sqrt(rT);  // sqrt(pi)/2 * T^{-1/2}
-    const double o2T = 0.5 * rT;
+    const double eT = exp(-T);
+    F[0] = boys_large_T_f0(T, eT);
+    const double o2T = 0.5 / T;
 #pragma unroll
-    for (int m = 1; m < MMAX; ++m) F[m] = F[m - 1] * ((2 * m - 1) * o2T);
+    for (int m = 1; m < MMAX; ++m) F[m] = fma((double)(2 * m - 1), F[m - 1], -eT) * o2T;
   }
 }
 

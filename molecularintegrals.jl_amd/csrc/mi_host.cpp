@@ -83,7 +83,63 @@ std::vector<double> make_boys_table() {
   return tab;
 }
 
-int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+// Pair classes: (L_A, L_B) with L_A >= L_B at L_A (L_A+1)/2 + L_B, then the
+// swapped ones (L_A < L_B) after them (the order of the generated class tables).
+int pair_class(int la, int lb) {
+  if (la >= lb) return la * (la + 1) / 2 + lb;
+  const int lm = mi::lmax_shell();
+  return (lm + 1) * (lm + 2) / 2 + lb * (lb - 1) / 2 + la;
+}
+
+// HRR cancellation factor of a shell pair taken in the orientation (A, B).
+// chrr builds (a b| from (e 0| = (a + k, 0| (k <= L_B) through
+// (a, b+1_j| = (a+1_j, b| + AB_j (a, b|.  For a primitive pair the (e 0|
+// terms grow like (|PA| + s)^e with s = 1/sqrt(2 zeta) the width of the
+// product Gaussian, and the result like (|PA| + s)^L_A (|PB| + s)^L_B, so
+// the HRR cancels about ((|PA| + s) / (|PB| + s))^L_B of its inputs' digits:
+// harmless when the tighter shell is A (P near A), up to 1e9 and more when a
+// diffuse A sits far from a tight B (P near B, |PA| ~ |AB|).  The maximum
+// over the pair's primitive pairs (|PA| = b/zeta |AB|, |PB| = a/zeta |AB|).
+static double hrr_amplification(const double* xyz, const std::vector<int32_t>& L, const std::vector<int32_t>& poff,
+                                const std::vector<double>& pexp, int A, int B) {
+  if (L[B] == 0) return 1.0;
+  double r2 = 0.0;
+  for (int k = 0; k < 3; ++k) r2 += (xyz[3 * A + k] - xyz[3 * B + k]) * (xyz[3 * A + k] - xyz[3 * B + k]);
+  const double r = std::sqrt(r2);
+  double amp = 1.0;
+  for (int pa = poff[A]; pa < poff[A + 1]; ++pa)
+    for (int pb = poff[B]; pb < poff[B + 1]; ++pb) {
+      const double a = pexp[pa], b = pexp[pb], z = a + b, s = 1.0 / std::sqrt(2.0 * z);
+      amp = std::max(amp, std::pow((b / z * r + s) / (a / z * r + s), (double)L[B]));
+    }
+  return amp;
+}
+
+// Orientation (A, B) of the shell pair {I, J}: the canonical L_A >= L_B, with
+// the tighter shell on A between shells of equal L (the smaller HRR
+// amplification; plain FP64 HGP otherwise loses up to 1e-8 absolute on
+// far-apart tight/diffuse pairs of f shells), and swapped to L_A < L_B when
+// the canonical orientation amplifies more than MI_SWAP_AMP and the swapped
+// one less.  Swapped pairs form their own pair classes and run in the
+// table-driven kernels; on the seeded workloads 0.1-1% of pairs swap.
+#define MI_SWAP_AMP 100.0
+// Pairs whose best orientation still amplifies more than this (contracted
+// shells mixing tight and diffuse primitives several bohr apart: 0.2% of the
+// pairs of the seeded 920-function cluster) put their quartets on the
+// double-double table kernel (mi_kernels.hip, hgp_wx_kernel), MI_WX_BLOCKS
+// persistent blocks per class launch.
+#define MI_DD_AMP 100.0
+#define MI_WX_BLOCKS 128
+static void orient_pair(const double* xyz, const std::vector<int32_t>& L, const std::vector<int32_t>& poff,
+                        const std::vector<double>& pexp, int I, int J, int& A, int& B) {
+  A = I;
+  B = J;
+  if (L[J] > L[I]) std::swap(A, B);
+  if (I == J) return;
+  const double keep = hrr_amplification(xyz, L, poff, pexp, A, B);
+  if (L[A] > L[B] && keep <= MI_SWAP_AMP) return;
+  if (hrr_amplification(xyz, L, poff, pexp, B, A) < keep) std::swap(A, B);
+}
 int quartet_class(int X, int Y) { return X >= Y ? X * (X + 1) / 2 + Y : Y * (Y + 1) / 2 + X; }
 
 int kbucket(int K) {
@@ -348,10 +404,11 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
   for (int c = 0; c < mi::nclass(); ++c) {
     // vk 0: register kernel (each task its own G); 1-3: wave kernel with
     // 1, 2, 4 wavefronts; 4-7: global-table kernel with 1, 2, 4, 8
-    // wavefronts; 8-10: wave kernel with global op tables, 1, 2, 4 wavefronts
-    for (int vk = 0; vk < 11; ++vk) {
-      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : 3));
-      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (vk < 8 ? (1 << (vk - 4)) : (1 << (vk - 8))));
+    // wavefronts; 8-10: wave kernel with global op tables, 1, 2, 4 wavefronts;
+    // 11: double-double table kernel (4 wavefronts)
+    for (int vk = 0; vk < 12; ++vk) {
+      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : (vk < 11 ? 3 : 4)));
+      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (vk < 8 ? (1 << (vk - 4)) : (vk < 11 ? (1 << (vk - 8)) : 4)));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
         if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
@@ -388,6 +445,9 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
       }
       if (var == 0 || var == 3)
         blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw, var == 3));
+      // double-double kernel: a few persistent blocks (its quartets are rare,
+      // and each block holds a 16-byte-per-entry work area in the scratch)
+      if (var == 4) blocks = std::min<int64_t>(qs, MI_WX_BLOCKS);
       L.tl = false;
       if (var == 2) {
         // tables in LDS only where that costs no resident blocks: a launch
@@ -411,6 +471,7 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
     const WClassDesc& D = mi::wclass(L.cls);
     L.cost = (double)L.nq_prim * (D.nvrr + D.nacc) + (double)L.nq_shell * (D.nhrr + D.nout) +
              2.0e4 * (D.nvlev + D.nhlev + 1);
+    if (L.var == 4) L.cost *= 12.0;  // double-double arithmetic: ~12 FP64 operations per operation
   }
   (void)P;
   return MI_OK;
@@ -645,7 +706,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     bfoff[s2 + 1] = bfoff[s2] + ncart(L[s2]);
     np[s2] = poff[s2 + 1] - poff[s2];
   }
-  // Shell pairs (I >= J within a molecule), canonical shell order L_A >= L_B.
+  // Shell pairs (I >= J within a molecule), oriented by orient_pair (L_A >= L_B
+  // except for the swapped pair classes).
   // Primitive pairs whose prefactor |K_ab| is below MI_PRIM_PAIR_CUT times the
   // largest of their shell pair are dropped (22% of them are exactly zero --
   // exp(-ab|AB|^2/zeta) underflows -- and the rest contribute below FP64
@@ -654,6 +716,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     int mol, pc, bucket, pidx, A, B, K;
     double Q, V;
     int64_t ab0;  // first kept (ia, ib) code in ab_codes
+    int ext;      // HRR amplification above MI_DD_AMP: quartets run in double-double
   };
   std::vector<PairKey> keys;
   std::vector<int32_t> ab_codes;
@@ -661,8 +724,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   for (int m = 0; m < nmol; ++m)
     for (int I = mol_sh0[m]; I < mol_sh0[m + 1]; ++I)
       for (int J = mol_sh0[m]; J <= I; ++J) {
-        int A = I, B = J;
-        if (L[J] > L[I]) std::swap(A, B);
+        int A, B;
+        orient_pair(xyz.data(), L, poff, pexp, I, J, A, B);
         double r2 = 0.0;
         for (int k = 0; k < 3; ++k) r2 += (xyz[3 * A + k] - xyz[3 * B + k]) * (xyz[3 * A + k] - xyz[3 * B + k]);
         kk.assign((size_t)np[A] * np[B], 0.0);
@@ -682,7 +745,18 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
           }
         const int K = (int)((int64_t)ab_codes.size() - ab0);
         const int i = I - mol_sh0[m], j = J - mol_sh0[m];
-        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0, ab0});
+        // amplification of the kept primitive pairs in the chosen orientation
+        double amp = 1.0;
+        if (L[B] > 0) {
+          const double r = std::sqrt(r2);
+          for (int64_t t = ab0; t < (int64_t)ab_codes.size(); ++t) {
+            const double a = pexp[poff[A] + (ab_codes[t] & 0xFFFF)], b = pexp[poff[B] + (ab_codes[t] >> 16)];
+            const double z = a + b, sw = 1.0 / std::sqrt(2.0 * z);
+            amp = std::max(amp, std::pow((b / z * r + sw) / (a / z * r + sw), (double)L[B]));
+          }
+        }
+        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0, ab0,
+                        amp > MI_DD_AMP ? 1 : 0});
       }
   // J/K layout of a plan without a cut: within a (class, bucket) list, pairs
   // of equal first shell A are contiguous, so along a task row (one bra
@@ -695,6 +769,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   std::stable_sort(keys.begin(), keys.end(), [jk_layout](const PairKey& x, const PairKey& y) {
     if (x.mol != y.mol) return x.mol < y.mol;
     if (x.pc != y.pc) return x.pc < y.pc;
+    if (x.ext != y.ext) return x.ext < y.ext;
     if (x.bucket != y.bucket) return x.bucket < y.bucket;
     if (jk_layout && x.A != y.A) return x.A < y.A;
     return x.pidx < y.pidx;
@@ -714,6 +789,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
       std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
         if (x.mol != y.mol) return x.mol < y.mol;
         if (x.pc != y.pc) return x.pc < y.pc;
+        if (x.ext != y.ext) return x.ext < y.ext;
         if (x.bucket != y.bucket) return x.bucket < y.bucket;
         return x.Q > y.Q;
       });
@@ -739,22 +815,24 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   P->npp = (int)ppoff;
   P->pair_mol.resize(npair);
   for (int r = 0; r < npair; ++r) P->pair_mol[r] = keys[r].mol;
-  // Contiguous lists of equal (molecule, pc, bucket).
+  // Contiguous lists of equal (molecule, pc, ext, bucket).
   struct List {
     int start, n, pc, Ksum, Kmax, mol;
     double qmax;  // bound of the list's first pair (its largest, once ordered by bound)
+    int ext;
   };
   std::vector<List> lists;
   std::vector<int> mol_list0(nmol + 1, 0);
   for (int r = 0; r < npair;) {
     int e = r;
     int Ksum = 0, Kmax = 0;
-    while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].bucket == keys[r].bucket) {
+    while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].ext == keys[r].ext &&
+           keys[e].bucket == keys[r].bucket) {
       Ksum += keys[e].K;
       Kmax = std::max(Kmax, keys[e].K);
       ++e;
     }
-    lists.push_back({r, e - r, keys[r].pc, Ksum, Kmax, keys[r].mol, keys[r].Q});
+    lists.push_back({r, e - r, keys[r].pc, Ksum, Kmax, keys[r].mol, keys[r].Q, keys[r].ext});
     r = e;
   }
   for (size_t x = 0, m = 0; m <= (size_t)nmol; ++m) {
@@ -769,6 +847,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   // quartets are screened in the kernels).
   std::vector<std::vector<TaskDesc>> per_class(mi::nclass());
   std::vector<std::vector<double>> kavgs(mi::nclass());
+  std::vector<std::vector<char>> kext(mi::nclass());  // task runs in double-double (an ext list)
   std::vector<int64_t> cls_nq(mi::nclass(), 0);
   int64_t nq_shell = 0, nq_prim = 0;
   std::vector<int64_t> Kpre(npair + 1, 0);  // prefix sums of primitive pairs in record order
@@ -802,6 +881,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         t.G = bb;  // kernel chosen below, once the class's quartet count is known
         t.variant = kb;
         kavgs[t.cls].push_back(kavg);
+        kext[t.cls].push_back((char)(bra->ext || ket->ext));
         const int64_t sk = Kpre[b0 + nb] - Kpre[b0];
         if (same) {
           // sum over p <= q of K_p K_q = ((sum K)^2 + sum K^2) / 2
@@ -858,6 +938,11 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     for (size_t k = 0; k < per_class[c].size(); ++k) {
       TaskDesc& t = per_class[c][k];
       const int bb = t.G, kb = t.variant;
+      if (kext[c][k]) {  // a pair of the quartet cancels beyond FP64: the double-double kernel
+        t.variant = 4;
+        t.G = 4;
+        continue;
+      }
       if ((rc = choose_kernel(ctx, t, bb, kb, kavgs[c][k], cls_nq[c] < MI_SMALL_LAUNCH ? 0 : 1,
                               P->order == MI_ORDER_JK))) {
         delete P;
@@ -897,7 +982,11 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         const int64_t nbm = mol_bf0[m + 1] - mol_bf0[m];
         t.M = nbm * (nbm + 1) / 2;
         const int bk = keys[li.start].bucket;
-        if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n), 0, false))) {
+        if (li.ext) {
+          t.variant = 4;
+          t.G = 4;
+        } else if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n), 0,
+                                       false))) {
           delete P;
           return rc;
         }
@@ -923,17 +1012,19 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   // concurrently (stream lanes), so each gets its own slice; the Schwarz pass
   // runs before them on the caller's stream and reuses the same memory.
   {
+    // slices start on 256-byte boundaries (the double-double kernel reads 16-byte pairs)
+    auto take = [](size_t& o, size_t n) {
+      const size_t at = (o + 31) & ~(size_t)31;
+      o = at + n;
+      return at;
+    };
     size_t off = 0, soff = 0;
     for (Launch& L : P->launches)
-      if (L.var == 2) {
-        L.scratch_off = off;
-        off += mi::wg_scratch_doubles(L.cls, L.nblocks);
-      }
+      if (L.var == 2) L.scratch_off = take(off, mi::wg_scratch_doubles(L.cls, L.nblocks));
+      else if (L.var == 4) L.scratch_off = take(off, mi::wx_scratch_doubles(L.cls, L.nblocks));
     for (Launch& L : P->slaunches)
-      if (L.var == 2) {
-        L.scratch_off = soff;
-        soff += mi::wg_scratch_doubles(L.cls, L.nblocks);
-      }
+      if (L.var == 2) L.scratch_off = take(soff, mi::wg_scratch_doubles(L.cls, L.nblocks));
+      else if (L.var == 4) L.scratch_off = take(soff, mi::wx_scratch_doubles(L.cls, L.nblocks));
     P->scratch_doubles = std::max(off, soff);
   }
   P->nlanes = std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
@@ -1028,6 +1119,9 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
     HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
   else if (L.var == 0 || L.var == 3)
     HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, L.var == 3, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
+  else if (L.var == 4)
+    HIP_TRY(ctx, mi::launch_class_extended(L.cls, a, L.nblocks, L.nq_shell, ctx->wtables,
+                                           P->d_scratch + L.scratch_off, ls));
   else
     HIP_TRY(ctx, mi::launch_class_wave_global(L.cls, L.nw, L.tl, a, L.nblocks, L.nq_shell, ctx->wtables,
                                               P->d_scratch + L.scratch_off, ls));
@@ -1242,8 +1336,19 @@ int mi_plan_pair_bounds(const mi_plan* P, double* Q, double* V, int32_t* mol) {
   if (!P || !Q || !V || !mol) return MI_EINVAL;
   if (!P->need_schwarz && !P->d_bounds)
     return set_err(P->ctx, MI_EINVAL, "plan has no Schwarz bounds (plain FP64 without screening)");
+  if (!P->h.bounds.empty()) {  // computed when the plan was built (no device read)
+    for (int r = 0; r < P->npair; ++r) {
+      Q[r] = std::sqrt(P->h.bounds[r].x);
+      V[r] = std::sqrt(P->h.bounds[r].y);
+      mol[r] = P->pair_mol[r];
+    }
+    return MI_OK;
+  }
+  // filled by the Schwarz pass of the last execute, which may still run on any
+  // stream (also a non-blocking one): wait for the device, then read
   std::vector<PairGeom> g(P->npair);
-  std::vector<PairTopo> t(P->npair);
+  HIP_TRY(P->ctx, hipSetDevice(P->ctx->device));
+  HIP_TRY(P->ctx, hipDeviceSynchronize());
   HIP_TRY(P->ctx, hipMemcpy(g.data(), P->d_geom, sizeof(PairGeom) * g.size(), hipMemcpyDeviceToHost));
   for (int r = 0; r < P->npair; ++r) {
     Q[r] = std::sqrt(g[r].Q);

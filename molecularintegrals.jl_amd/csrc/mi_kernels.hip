@@ -97,8 +97,9 @@ __device__ __forceinline__ double sel3(int i, double x, double y, double z) { re
 __device__ __forceinline__ int ncart_dev(int l) { return (l + 1) * (l + 2) / 2; }
 
 // F_m(T) for one order m <= mtop: grid Taylor value of the top order, then
-// downward recursion to m (T < 30); asymptotic form with upward recursion
-// otherwise (src/Boys.jl:19-40).  btab is the grid of the top order mtop.
+// downward recursion to m (T < 30); F_0 and the upward recursion of
+// boys_large_T_f0 otherwise (src/Boys.jl:19-34).  btab is the grid of the top
+// order mtop.
 __device__ __forceinline__ double boys_lane(const double* __restrict__ btab, double T2, int mtop, int m) {
   double fm;
   if (T2 < BOYS_TMAX) {
@@ -117,10 +118,10 @@ __device__ __forceinline__ double boys_lane(const double* __restrict__ btab, dou
     fm = f;
     for (int j = mtop - 1; j >= m; --j) fm = fma(t2, fm, eT) * kInvOdd[j];
   } else {
-    const double rT = 1.0 / T2;
-    fm = 0.88622692545275801365 * sqrt(rT);
-    const double o2T = 0.5 * rT;
-    for (int j = 1; j <= m; ++j) fm *= (2 * j - 1) * o2T;
+    const double eT = exp(-T2);
+    fm = boys_large_T_f0(T2, eT);
+    const double o2T = 0.5 / T2;
+    for (int j = 1; j <= m; ++j) fm = fma((double)(2 * j - 1), fm, -eT) * o2T;
   }
   return fm;
 }
@@ -537,6 +538,176 @@ __global__ void __launch_bounds__(512) hgp_wgf_kernel(const EriArgs args, const 
     wg_body<false, false, false>(a, D, gblob, scratch + F.bt_off[s], F.nq[s], bid, nblk, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Extended-precision table kernel (tuning variant 4): the global-table
+// kernel's op tables evaluated in double-double arithmetic for the quartets
+// of shell pairs whose HRR would cancel more than MI_DD_AMP of FP64's digits
+// in either orientation (contracted pairs mixing tight and diffuse primitives,
+// several bohr apart: mi_host.cpp, pair orientation).  Inputs of the
+// recurrences (PA, WP, QC, WQ, the n/2zeta factors, AB, CD, the [0|0]^(m)
+// values from FP64 Boys and prefactors) stay FP64 -- they carry the problem's
+// data, and their rounding changes results by ~1e-16 relative -- while every
+// VRR entry, the contraction sums and the HRR levels carry ~106 bits, so the
+// cancellation happens on numbers that are exact to far below the bar.
+// Double-double arithmetic: Knuth's TwoSum, Dekker's FastTwoSum and the
+// fma-based TwoProd (exact product), the usual "sloppy" pair addition plus
+// one renormalisation.  One quartet per block at a time (persistent blocks),
+// VRR and HRR areas in a global slice of the plan scratch per block (16 bytes
+// per entry: a (ff|ff) VRR area is 276 KB, beyond LDS).
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+struct ddv {
+  double hi, lo;
+};
+__device__ __forceinline__ ddv dd_two_sum(double a, double b) {
+  const double s = a + b, bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ ddv dd_fast_two_sum(double a, double b) {
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+__device__ __forceinline__ ddv dd_add(ddv x, ddv y) {
+  ddv s = dd_two_sum(x.hi, y.hi);
+  s.lo += x.lo + y.lo;
+  return dd_fast_two_sum(s.hi, s.lo);
+}
+// x * b, b a double
+__device__ __forceinline__ ddv dd_mul_d(ddv x, double b) {
+  const double p = x.hi * b;
+  double e = fma(x.hi, b, -p);
+  e = fma(x.lo, b, e);
+  return dd_fast_two_sum(p, e);
+}
+#pragma clang fp contract(on)
+
+__device__ __forceinline__ ddv dd_load(const double2* a, int i) {
+  const double2 v = a[i];
+  return {v.x, v.y};
+}
+__device__ __forceinline__ void dd_store(double2* a, int i, ddv v) { a[i] = make_double2(v.hi, v.lo); }
+
+__global__ void __launch_bounds__(256) hgp_wx_kernel(const EriArgs args, const WClassDesc D,
+                                                     const uint32_t* __restrict__ gblob, double2* __restrict__ scratch,
+                                                     int64_t nq_launch) {
+  __shared__ double scl[4 * W_SCL];
+  double2* vrr = scratch + (size_t)blockIdx.x * (D.nvrr + D.nhrr);
+  double2* hrr = vrr + D.nvrr;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const uint4* gV = reinterpret_cast<const uint4*>(gblob + D.gv);
+  const uint4* gH = reinterpret_cast<const uint4*>(gblob + D.gh);
+  const uint2_t* gO = reinterpret_cast<const uint2_t*>(gblob + D.go);
+  const uint2_t* gA = reinterpret_cast<const uint2_t*>(gblob + D.ga);
+  const int na = ncart_dev(D.la), nb = ncart_dev(D.lb), nc = ncart_dev(D.lc), nd = ncart_dev(D.ld);
+  const int mtop = D.mmax - 1;
+  const double* btab = args.boys + (size_t)mtop * BOYS_NROW * 8;
+  int ti = 0;
+  for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
+    ti = advance_task(args, ti, q);
+    const TaskDesc T = args.tasks[ti];
+    QuartetCtx qc;
+    decode_index(T, q - T.blockStart, qc);
+    qc.G = 1;
+    qc.g = 0;
+    if (quartet_mode(args, qc) == 2) continue;  // screened (uniform per block); output pre-zeroed
+    const PairTopo P = args.topo[qc.bra];
+    const PairTopo Q = args.topo[qc.ket];
+    const double Ax = args.shell_xyz[3 * P.shA], Ay = args.shell_xyz[3 * P.shA + 1], Az = args.shell_xyz[3 * P.shA + 2];
+    const double Cx = args.shell_xyz[3 * Q.shA], Cy = args.shell_xyz[3 * Q.shA + 1], Cz = args.shell_xyz[3 * Q.shA + 2];
+    if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
+    if (tid < nb) scl[W_SCL + tid] = args.bf_scale[P.bfB + tid];
+    if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
+    if (tid < nd) scl[3 * W_SCL + tid] = args.bf_scale[Q.bfB + tid];
+    const int K = P.npp * Q.npp;
+    for (int t = 0; t < K; ++t) {
+      const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
+      const PrimPair bp = args.pp[P.ppoff + u];
+      const PrimPair kp = args.pp[Q.ppoff + v];
+      const double zeta = bp.zeta, eta = kp.zeta;
+      const double inv_s = 1.0 / (zeta + eta);
+      const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),
+                   PQz = (Az + bp.PAz) - (Cz + kp.PAz);
+      const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = bp.K * kp.K * sqrt(inv_s);
+      if (tid < D.vt0) {
+        const ddv b = dd_two_sum(0.0, boys_lane(btab, T2, mtop, tid));
+        dd_store(vrr, tid, dd_mul_d(b, pref));
+      }
+      __syncthreads();
+      const double oo2z = bp.oo2z, oo2e = kp.oo2z, oo2s = 0.5 * inv_s;
+      const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
+      const double WPx = -eta_s * PQx, WPy = -eta_s * PQy, WPz = -eta_s * PQz;
+      const double WQx = zeta_s * PQx, WQy = zeta_s * PQy, WQz = zeta_s * PQz;
+      for (int lv = 0; lv < D.nvlev; ++lv) {
+        const int i1 = D.vlev[lv + 1];
+        for (int it = D.vlev[lv] + tid; it < i1; it += nthr) {
+          const uint4 o = gV[it];
+          const int s1 = o.x, s2 = o.y, s3 = o.z;
+          const int dir = o.w & 3;
+          const double n1 = (double)((o.w >> 2) & 7), n2 = (double)((o.w >> 5) & 7);
+          const bool cb = (o.w >> 8) & 1;
+          const bool h2 = (o.w >> 9) & 1, h3 = (o.w >> 10) & 1;
+          const double X1 = cb ? sel3(dir, kp.PAx, kp.PAy, kp.PAz) : sel3(dir, bp.PAx, bp.PAy, bp.PAz);
+          const double X2 = cb ? sel3(dir, WQx, WQy, WQz) : sel3(dir, WPx, WPy, WPz);
+          ddv val = dd_add(dd_mul_d(dd_load(vrr, s1), X1), dd_mul_d(dd_load(vrr, s1 + 1), X2));
+          if (cb && h3) val = dd_add(val, dd_mul_d(dd_load(vrr, s3 + 1), n1 * oo2s));
+          if (h2) {
+            const double c2 = cb ? n2 * oo2e : n1 * oo2z;
+            const double r2 = cb ? zeta_s : eta_s;
+            const ddv d = dd_add(dd_load(vrr, s2), dd_mul_d(dd_load(vrr, s2 + 1), -r2));
+            val = dd_add(val, dd_mul_d(d, c2));
+          }
+          dd_store(vrr, D.vt0 + it, val);
+        }
+        __syncthreads();
+      }
+      // contraction: slot x is always added by the same thread
+      for (int x = tid; x < D.nacc; x += nthr) {
+        const ddv v0 = dd_load(vrr, gA[x].x);
+        dd_store(hrr, x, t == 0 ? v0 : dd_add(dd_load(hrr, x), v0));
+      }
+      __syncthreads();  // the next primitive overwrites vrr
+    }
+    if (args.mode == MI_MODE_SCHWARZ)
+      for (int x = tid; x < D.nacc; x += nthr)
+        if (gA[x].y) record_vmax(&args.geom[qc.bra], hrr[x].x + hrr[x].y);
+    const PairGeom PG = args.geom[qc.bra];
+    const PairGeom QG = args.geom[qc.ket];
+    for (int lv = 0; lv < D.nhlev; ++lv) {
+      const int i1 = D.hlev[lv + 1];
+      for (int it = D.hlev[lv] + tid; it < i1; it += nthr) {
+        const uint4 o = gH[it];
+        const int j = o.z & 3;
+        const bool ket = (o.z >> 2) & 1;
+        const double X = ket ? sel3(j, QG.ABx, QG.ABy, QG.ABz) : sel3(j, PG.ABx, PG.ABy, PG.ABz);
+        dd_store(hrr, D.nacc + it, dd_add(dd_load(hrr, o.x), dd_mul_d(dd_load(hrr, o.y), X)));
+      }
+      __syncthreads();
+    }
+    const ScatterCtx sc = make_scatter(args, qc, P, Q);
+    if (args.mode == MI_MODE_JK) {  // uniform per block: the scaled block staged in the VRR area
+      double* blk = reinterpret_cast<double*>(vrr);
+      for (int x = tid; x < D.nout; x += nthr) {
+        const uint2_t o = gO[x];
+        const int ma = o.y & 15, mb = (o.y >> 4) & 15, mc = (o.y >> 8) & 15, md = (o.y >> 12) & 15;
+        blk[x] = (hrr[o.x].x + hrr[o.x].y) * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]);
+      }
+      __syncthreads();
+      JkRun run{kNoRun, kNoRun, kNoRun, 0, 0, 0};
+      jk_block_lds(args, sc, blk, na, nb, nc, nd, tid, nthr, nullptr, run);
+      __syncthreads();
+      continue;
+    }
+    for (int x = tid; x < D.nout; x += nthr) {
+      const uint2_t o = gO[x];
+      const int ma = o.y & 15, mb = (o.y >> 4) & 15, mc = (o.y >> 8) & 15, md = (o.y >> 12) & 15;
+      scatter_elem(args, sc, qc.bra, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md,
+                   (hrr[o.x].x + hrr[o.x].y) * (scl[ma] * scl[W_SCL + mb]) * (scl[2 * W_SCL + mc] * scl[3 * W_SCL + md]));
+    }
+    __syncthreads();
+  }
+}
+
 // Zero a device array (a kernel, not a memset node, so captured graphs replay it safely).
 // Plain 8-byte stores, 4096 x 256 lanes: 5.8 TB/s on 40 GiB, as fast as
 // hipMemsetAsync (5.9); 16-byte, non-temporal and unrolled variants measured
@@ -782,6 +953,20 @@ hipError_t launch_fused_wave_global(int nw, const EriArgs& args, const FusedSegs
   const uint32_t* gb = t->gblob;
   void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<FusedSegs*>(&F), &gb, &scratch};
   return hipLaunchKernel((const void*)hgp_wgf_kernel, dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
+}
+
+size_t wx_scratch_doubles(int c, int nblocks) {
+  return 2 * (size_t)nblocks * (kWClassDesc[c].nvrr + kWClassDesc[c].nhrr);
+}
+
+hipError_t launch_class_extended(int c, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
+                                 double* scratch, hipStream_t stream) {
+  const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
+  const WClassDesc D = kWClassDesc[c];
+  const uint32_t* gb = t->gblob;
+  double2* sc = reinterpret_cast<double2*>(scratch);
+  void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<WClassDesc*>(&D), &gb, &sc, &nq_launch};
+  return hipLaunchKernel((const void*)hgp_wx_kernel, dim3(nblocks), dim3(256), kargs, 0, stream);
 }
 
 size_t wg_scratch_doubles(int c, int nblocks) { return wg_hrr_in_lds(c) ? 0 : (size_t)nblocks * kWClassDesc[c].nhrr; }

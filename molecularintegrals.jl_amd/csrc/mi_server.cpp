@@ -139,7 +139,11 @@ int launch(mi_server* s, const std::vector<int64_t>& ids, int slot) {
   Batch b;
   b.slot = slot;
   b.reqs = ids;
-  int rc = mi_plan_build(s->ctx, bases.data(), (int)bases.size(), &r0.o, MI_BUILD_DEFER_UPLOAD, &b.plan);
+  // No bounds pass at build time: that pass reads the bounds back to the host
+  // (a synchronisation), and step must not block.  Mixed and screened batches
+  // run their Schwarz pass inside the execute, on the batch stream.
+  int rc = mi_plan_build(s->ctx, bases.data(), (int)bases.size(), &r0.o, MI_BUILD_DEFER_UPLOAD | MI_BUILD_NO_BOUNDS,
+                         &b.plan);
   if (rc == MI_OK) rc = mi_plan_upload(b.plan, st);
   if (rc == MI_OK && r0.kind == MI_REQ_ERI) {
     const int64_t len = b.plan->n_unique;

@@ -159,8 +159,9 @@ class Plan:
     def launches(self) -> list:
         """Per class launch: ((la,lb,lc,ld), shell quartets, primitive quartets,
         blocks, kernel), kernel = "R" (generated register kernel), "W<nw>" (LDS
-        wave kernel, nw wavefronts per quartet) or "G<nw>" (global-table kernel,
-        nw wavefronts per block)."""
+        wave kernel, nw wavefronts per quartet), "G<nw>" (global-table kernel,
+        nw wavefronts per block), "T<nw>" (wave kernel, global op tables) or
+        "X4" (double-double table kernel)."""
         out = []
         n = self.ctx.L.mi_plan_nlaunch(self.h)
         for i in range(n):
@@ -168,7 +169,7 @@ class Plan:
             n3 = (ctypes.c_int64 * 3)()
             k2 = (ctypes.c_int32 * 2)()
             self.ctx.check(self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3, k2))
-            kern = "R" if k2[0] == 1 else {0: "W", 2: "G", 3: "T"}[k2[0]] + str(k2[1])
+            kern = "R" if k2[0] == 1 else {0: "W", 2: "G", 3: "T", 4: "X"}[k2[0]] + str(k2[1])
             out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), kern))
         return out
 

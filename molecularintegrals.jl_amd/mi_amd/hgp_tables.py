@@ -70,12 +70,28 @@ def shell_aos(L):
 # ---------------------------------------------------------------- classes
 
 
-def pair_class_index(la, lb):
-    return la * (la + 1) // 2 + lb
+def pair_class_index(la, lb, lmax=LMAX_SHELL):
+    """Index of the pair class (L_A, L_B) in pair_classes(lmax)."""
+    if la >= lb:
+        return la * (la + 1) // 2 + lb
+    return (lmax + 1) * (lmax + 2) // 2 + lb * (lb - 1) // 2 + la
 
 
 def pair_classes(lmax=LMAX_SHELL):
-    return [(la, lb) for la in range(lmax + 1) for lb in range(la + 1)]
+    """Pair classes (L_A, L_B): the canonical L_A >= L_B ones first (index
+    L_A (L_A+1)/2 + L_B, the reference's chrr shell order up to a swap), then
+    the swapped L_A < L_B ones.  A shell pair is swapped when its HRR would
+    otherwise cancel many digits: a diffuse high-L shell A far from a tight
+    low-L shell B puts P next to B, so the (e0| terms grow like |AB|^e while
+    the result does not (see mi_host.cpp, pair orientation)."""
+    normal = [(la, lb) for la in range(lmax + 1) for lb in range(la + 1)]
+    swapped = [(la, lb) for lb in range(1, lmax + 1) for la in range(lb)]
+    return normal + swapped
+
+
+def is_swapped_class(c):
+    """A quartet class with a swapped pair (L_A < L_B or L_C < L_D)."""
+    return c[0] < c[1] or c[2] < c[3]
 
 
 def quartet_classes(lmax=LMAX_SHELL):

@@ -11,6 +11,7 @@
 #include "hgp_oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -167,6 +168,11 @@ static long double boys_top_ld(int m, long double x) {
     const long double SMALL = 1e-18L;
     if (x < SMALL) return 1.0L / (2 * m + 1);
     long double a = (long double)m + 0.5L;
+    if (x >= 200) { /* Gamma(m+1/2) / (2 x^{m+1/2}); the dropped tail is < e^{-200} x^{m-1/2}/Gamma(m+1/2) */
+        long double g = sqrtl(3.141592653589793238462643383279502884L);
+        for (int k = 1; k <= m; ++k) g *= ((long double)k - 0.5L);
+        return g / (2 * powl(x, a));
+    }
     long double term = 1.0L / a;
     long double sum = term;
     for (int n = 1; n < 10000; ++n) {
@@ -226,6 +232,7 @@ static int is_key(const flat_basis *fb, int I, int J, int K, int Lsh) {
 
 
 /* The recurrences, instantiated in FP64 and in x87 extended precision. */
+#define BOYS_TOP_(m, T) ((REAL)boys_top_ld((m), (long double)(T)))
 #define REAL double
 #define SFX(name) name##_d
 #define RSQRT_(x) sqrt(x)
@@ -239,6 +246,8 @@ static int is_key(const flat_basis *fb, int I, int J, int K, int Lsh) {
 #undef REXP_
 #undef RPOW_
 #undef RPI
+#undef BOYS_TOP_
+#define BOYS_TOP_(m, T) ((REAL)boys_top_ld((m), (long double)(T)))
 #define REAL long double
 #define SFX(name) name##_x
 #define RSQRT_(x) sqrtl(x)
@@ -252,8 +261,54 @@ static int is_key(const flat_basis *fb, int I, int J, int K, int Lsh) {
 #undef REXP_
 #undef RPOW_
 #undef RPI
+#undef BOYS_TOP_
 
-void oracle_farray(int mmax, double T, double *Fms) { farray_d(mmax, T, Fms); }
+/* binary128 instantiation: the recurrences in quad precision.  Boys:
+ * F_m(T) = e^{-T}/2 sum_n T^n / prod_{k=0..n} (m+1/2+k) (the series of
+ * boys_top_ld, all terms positive) for T < 200; above, F_m(T) =
+ * Gamma(m+1/2)/(2 T^{m+1/2}) - Gamma(m+1/2,T)/(2 T^{m+1/2}), whose second term is
+ * below e^{-200} T^{m-1/2} / Gamma(m+1/2) relative and is dropped. */
+static __float128 boys_top_q(int m, __float128 x) {
+    if (x < 1e-18Q) return 1 / (__float128)(2 * m + 1);
+    const __float128 a = (__float128)m + 0.5Q;
+    if (x >= 200) {
+        /* Gamma(m+1/2) = (2m-1)!! sqrt(pi) / 2^m */
+        __float128 g = sqrtq(M_PIq);
+        for (int k = 1; k <= m; ++k) g *= ((__float128)k - 0.5Q);
+        return g / (2 * powq(x, a));
+    }
+    __float128 term = 1 / a, sum = term;
+    for (int n = 1; n < 100000; ++n) {
+        term *= x / (a + n);
+        sum += term;
+        if (term < sum * 1e-36Q) break;
+    }
+    return 0.5Q * expq(-x) * sum;
+}
+#define BOYS_TOP_(m, T) boys_top_q((m), (T))
+#define REAL __float128
+#define SFX(name) name##_q
+#define RSQRT_(x) sqrtq(x)
+#define REXP_(x) expq(x)
+#define RPOW_(x, y) powq(x, y)
+#define RPI M_PIq
+#include "hgp_core.inc"
+#undef REAL
+#undef SFX
+#undef RSQRT_
+#undef REXP_
+#undef RPOW_
+#undef RPI
+#undef BOYS_TOP_
+
+double oracle_Fm_quad(int m, double T, int accurate) {
+    __float128 F[64];
+    farray_q(m + 1, (__float128)T, F, !accurate);
+    return (double)F[m];
+}
+
+void oracle_farray(int mmax, double T, double *Fms) { farray_d(mmax, T, Fms, 1); }
+void oracle_farray_accurate(int mmax, double T, double *Fms) { farray_d(mmax, T, Fms, 0); }
 
 void oracle_vrr(double *vrrs, int amax, int cmax, double aexpn, double bexpn, double cexpn, double dexpn,
                 const double *A, const double *B, const double *C, const double *D, int literal) {
@@ -303,6 +358,7 @@ static int eri_unique_hgp_impl(int nshell, const double *shell_xyz, const int32_
                         q[0] = I; q[1] = J; q[2] = K; q[3] = Lsh;
                     }
     memset(out, 0, sizeof(double) * (size_t)oracle_n_unique(nbf));
+    if (extended == 2) literal |= ORACLE_LIT_D5;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
@@ -310,7 +366,14 @@ static int eri_unique_hgp_impl(int nshell, const double *shell_xyz, const int32_
     {
         size_t cvn, wn, hn;
         work_sizes(maxl, &cvn, &wn, &hn);
-        if (extended) {
+        if (extended >= 2) {
+            __float128 *buf = (__float128 *)malloc(sizeof(__float128) * (cvn + wn + hn));
+#pragma omp for schedule(dynamic, 4)
+            for (int64_t q = 0; q < nkeys; ++q)
+                process_key_q(&fb, keys[4 * q], keys[4 * q + 1], keys[4 * q + 2], keys[4 * q + 3], order, literal,
+                              buf, buf + cvn, buf + cvn + wn, out, NULL);
+            free(buf);
+        } else if (extended) {
             long double *buf = (long double *)malloc(sizeof(long double) * (cvn + wn + hn));
 #pragma omp for schedule(dynamic, 16)
             for (int64_t q = 0; q < nkeys; ++q)
@@ -345,6 +408,128 @@ int oracle_eri_unique_hgp_ext(int nshell, const double *shell_xyz, const int32_t
                               double *out) {
     return eri_unique_hgp_impl(nshell, shell_xyz, shell_l, shell_prim_off, prim_exp, prim_coef, prim_norm, nbf,
                                bf_scale, order, 0, nthreads, out, 1);
+}
+
+int oracle_eri_unique_hgp_quad(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                               const int32_t *shell_prim_off, const double *prim_exp, const double *prim_coef,
+                               const double *prim_norm, int nbf, const double *bf_scale, int order, int accurate,
+                               int nthreads, double *out) {
+    return eri_unique_hgp_impl(nshell, shell_xyz, shell_l, shell_prim_off, prim_exp, prim_coef, prim_norm, nbf,
+                               bf_scale, order, 0, nthreads, out, accurate ? 3 : 2);
+}
+
+/* Selected slots of the unique array in binary128 (arbiter of GPU parity
+ * tests where the FP64 oracle loses digits): each slot is decoded to its
+ * function quartet (i >= j, k >= l, tri(i,j) <= tri(k,l)), whose shell
+ * quartet block (cvrr + chrr, src/HGP.jl:50-116) is computed once per
+ * distinct shell quartet. */
+typedef struct { int64_t key, idx; } slot_job;
+static int cmp_job(const void *a, const void *b) {
+    int64_t x = ((const slot_job *)a)->key, y = ((const slot_job *)b)->key;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+static void slot_decode(int64_t nbf, int64_t slot, int order, int64_t f[4]) {
+    int64_t M = nbf * (nbf + 1) / 2, ij, kl;
+    if (order == ORACLE_ORDER_IINDEX) { /* slot + 1 = kl (kl - 1) / 2 + ij, ij <= kl */
+        kl = (int64_t)((sqrt(8.0 * (double)slot + 1.0) + 1.0) / 2.0);
+        while (kl * (kl - 1) / 2 > slot) --kl;
+        while ((kl + 1) * kl / 2 <= slot) ++kl;
+        ij = slot - kl * (kl - 1) / 2 + 1;
+    } else { /* slot = (ij - 1)(M + 1) - ij (ij - 1) / 2 + kl - ij */
+        int64_t lo = 1, hi = M;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) / 2;
+            if ((mid - 1) * (M + 1) - mid * (mid - 1) / 2 <= slot) lo = mid; else hi = mid - 1;
+        }
+        ij = lo;
+        kl = slot - ((ij - 1) * (M + 1) - ij * (ij - 1) / 2) + ij;
+    }
+    int64_t p[2] = {ij, kl};
+    for (int t = 0; t < 2; ++t) {
+        int64_t a = (int64_t)((sqrt(8.0 * (double)p[t]) + 1.0) / 2.0);
+        while (a * (a - 1) / 2 >= p[t]) --a;
+        while (a * (a + 1) / 2 < p[t]) ++a;
+        f[2 * t] = a;
+        f[2 * t + 1] = p[t] - a * (a - 1) / 2;
+    }
+}
+static int shell_of(const flat_basis *fb, int64_t f1) { /* 1-based function -> shell */
+    int lo = 0, hi = fb->nshell - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (fb->bfoff[mid] < f1) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+int oracle_eri_slots_quad(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                          const int32_t *shell_prim_off, const double *prim_exp, const double *prim_coef,
+                          const double *prim_norm, int nbf, const double *bf_scale, int order, int accurate,
+                          int64_t nslot, const int64_t *slots, int nthreads, double *out) {
+    flat_basis fb = {nshell, nbf, shell_xyz, shell_l, shell_prim_off, prim_exp, prim_coef, prim_norm, bf_scale, NULL};
+    fb.bfoff = (int *)malloc(sizeof(int) * (nshell + 1));
+    int rc = check_basis(&fb);
+    if (rc) { free(fb.bfoff); return rc; }
+    int maxl = 0;
+    for (int s2 = 0; s2 < nshell; ++s2) maxl = shell_l[s2] > maxl ? shell_l[s2] : maxl;
+    slot_job *jobs = (slot_job *)malloc(sizeof(slot_job) * (size_t)(nslot > 0 ? nslot : 1));
+    int64_t ns = nshell;
+    for (int64_t q = 0; q < nslot; ++q) {
+        int64_t f[4];
+        if (slots[q] < 0 || slots[q] >= oracle_n_unique(nbf)) { free(jobs); free(fb.bfoff); return 3; }
+        slot_decode(nbf, slots[q], order, f);
+        int64_t key = 0;
+        for (int t = 0; t < 4; ++t) key = key * ns + shell_of(&fb, f[t]);
+        jobs[q].key = key;
+        jobs[q].idx = q;
+    }
+    qsort(jobs, (size_t)nslot, sizeof(slot_job), cmp_job);
+    int64_t ngroup = 0;
+    int64_t *gstart = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nslot + 1));
+    for (int64_t q = 0; q < nslot; ++q)
+        if (q == 0 || jobs[q].key != jobs[q - 1].key) gstart[ngroup++] = q;
+    gstart[ngroup] = nslot;
+    const int literal = accurate ? 0 : ORACLE_LIT_D5;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        size_t cvn, wn, hn;
+        work_sizes(maxl, &cvn, &wn, &hn);
+        __float128 *buf = (__float128 *)malloc(sizeof(__float128) * (cvn + wn + hn));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t g = 0; g < ngroup; ++g) {
+            int64_t key = jobs[gstart[g]].key;
+            int sh[4];
+            for (int t = 3; t >= 0; --t) { sh[t] = (int)(key % ns); key /= ns; }
+            int la = shell_l[sh[0]], lb = shell_l[sh[1]], lc = shell_l[sh[2]], ld = shell_l[sh[3]];
+            cvrr_shells_q(buf, buf + cvn, &fb, sh[0], sh[1], sh[2], sh[3], literal);
+            __float128 A[3], B[3], C[3], D[3];
+            for (int k = 0; k < 3; ++k) {
+                A[k] = shell_xyz[3 * sh[0] + k];
+                B[k] = shell_xyz[3 * sh[1] + k];
+                C[k] = shell_xyz[3 * sh[2] + k];
+                D[k] = shell_xyz[3 * sh[3] + k];
+            }
+            __float128 *hr = buf + cvn + wn;
+            hrr_from_vrr_q(hr, buf, la, lb, lc, ld, A, B, C, D);
+            const int nb = oracle_nao(lb), nc = oracle_nao(lc + ld), nd = oracle_nao(ld);
+            for (int64_t q = gstart[g]; q < gstart[g + 1]; ++q) {
+                int64_t f[4];
+                slot_decode(nbf, slots[jobs[q].idx], order, f);
+                int h[4];
+                for (int t = 0; t < 4; ++t) h[t] = oracle_nao(shell_l[sh[t]] - 1) + (int)(f[t] - 1 - fb.bfoff[sh[t]]);
+                __float128 v = hr[(((size_t)h[0] * nb + h[1]) * nc + h[2]) * nd + h[3]];
+                v *= (__float128)bf_scale[f[0] - 1] * bf_scale[f[1] - 1] * bf_scale[f[2] - 1] * bf_scale[f[3] - 1];
+                out[jobs[q].idx] = (double)v;
+            }
+        }
+        free(buf);
+    }
+    free(gstart);
+    free(jobs);
+    free(fb.bfoff);
+    return 0;
 }
 
 int64_t oracle_eri_hgp_quartet_sample(int nshell, const double *shell_xyz, const int32_t *shell_l,

@@ -8,12 +8,17 @@
  * library (libmi_hgp.so) never links or calls anything declared here.
  *
  * Every function cites the reference file:line it restates.  "literal" flags
- * reproduce the three defects of the reference's all_twoe_ints_chrr bug for bug:
+ * reproduce the defects of the reference's all_twoe_ints_chrr bug for bug:
  *   ORACLE_LIT_D1  wrong [00|00] prefactor 2 pi^{5/2} (zeta+eta)^{-3/2}   src/HGP.jl:150-151
  *   ORACLE_LIT_D2  no normalisation (raw Shell.coefs, no CGBF norms)        src/HGP.jl:57-63
  *   ORACLE_LIT_D3  gather at hrrs[mshell...] instead of the AO offsets    src/HGP.jl:29-30
+ *   ORACLE_LIT_D5  farray's T >= 30 branch: asymptotic top order and no
+ *                  e^{-T} in the recursion (src/Boys.jl:21-26), off by up to
+ *                  1e-4 relative for F_12 at T = 30 and by up to 5e-10
+ *                  relative on f-shell integrals
  * With flags == 0 the oracle computes the package's tested value definition,
- * coulomb(bfs[i],bfs[j],bfs[k],bfs[l]) (src/ERI.jl:102), by HGP recurrences.
+ * coulomb(bfs[i],bfs[j],bfs[k],bfs[l]) (src/ERI.jl:102, whose Fgamma = Fm is
+ * the Boys function for every T), by HGP recurrences.
  *
  * Flattened basis format (shared with the product C ABI, see include/mi_hgp.h):
  *   shells in Basis.shells order; per shell xyz (bohr), L, primitive range;
@@ -34,6 +39,7 @@ extern "C" {
 #define ORACLE_LIT_D1 1
 #define ORACLE_LIT_D2 2
 #define ORACLE_LIT_D3 4
+#define ORACLE_LIT_D5 8
 
 #define ORACLE_ORDER_IINDEX 0 /* all_twoe_ints slot = iindex(i,j,k,l)  src/ERI.jl:109 */
 #define ORACLE_ORDER_CHRR 1   /* enumeration order of iiterator           src/Basis.jl:203 */
@@ -94,6 +100,30 @@ int oracle_eri_unique_hgp_ext(int nshell, const double *shell_xyz, const int32_t
                               const int32_t *shell_prim_off, const double *prim_exp,
                               const double *prim_coef, const double *prim_norm, int nbf,
                               const double *bf_scale, int order, int nthreads, double *out);
+
+/* The same algorithm in binary128 (__float128; inputs and output FP64): the
+ * recurrences' rounding is ~1e-34 relative, far below every parity bar, so
+ * this is the exact value of the reference algorithm.  accurate = 0: Boys by
+ * the reference's farray branches (T >= 30 asymptotic, e^-T dropped,
+ * src/Boys.jl:19-27); accurate = 1: the Boys function itself for every T, as
+ * coulomb()'s Fgamma (src/ERI.jl:41, src/Boys.jl:29-34). */
+int oracle_eri_unique_hgp_quad(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                               const int32_t *shell_prim_off, const double *prim_exp,
+                               const double *prim_coef, const double *prim_norm, int nbf,
+                               const double *bf_scale, int order, int accurate, int nthreads,
+                               double *out);
+/* farray with the Boys function exact for every T (no asymptotic branch). */
+void oracle_farray_accurate(int mmax, double T, double *Fms);
+
+/* The binary128 values of nslot selected 0-based slots (in `order`) of the
+ * unique array, without computing the rest. */
+int oracle_eri_slots_quad(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                          const int32_t *shell_prim_off, const double *prim_exp,
+                          const double *prim_coef, const double *prim_norm, int nbf,
+                          const double *bf_scale, int order, int accurate, int64_t nslot,
+                          const int64_t *slots, int nthreads, double *out);
+/* F_m(T) through the binary128 farray (accurate as above). */
+double oracle_Fm_quad(int m, double T, int accurate);
 
 /* Same contract as above but only for the listed shell quartets (sampled CPU
  * baseline); returns number of ERIs computed (written nowhere). */

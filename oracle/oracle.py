@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhgp_oracle.so")
 
-LIT_D1, LIT_D2, LIT_D3 = 1, 2, 4
+LIT_D1, LIT_D2, LIT_D3, LIT_D5 = 1, 2, 4, 8
 ORDER_IINDEX, ORDER_CHRR = 0, 1
 
 _lib = None
@@ -39,6 +39,7 @@ def lib():
         L.oracle_Fm.restype = ctypes.c_double
         L.oracle_Fm.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_farray.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+        L.oracle_farray_accurate.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
         L.oracle_nao.argtypes = [ctypes.c_int]
         L.oracle_n_unique.restype = ctypes.c_int64
         L.oracle_n_unique.argtypes = [ctypes.c_int64]
@@ -59,6 +60,11 @@ def lib():
         L.oracle_eri_unique_hgp.argtypes = basis_args + [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_eri_unique_tho.argtypes = basis_args + [ctypes.c_int, _dp]
         L.oracle_eri_unique_hgp_ext.argtypes = basis_args + [ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_eri_unique_hgp_quad.argtypes = basis_args + [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_eri_slots_quad.argtypes = basis_args + [ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_int, _dp]
+        L.oracle_Fm_quad.restype = ctypes.c_double
+        L.oracle_Fm_quad.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.oracle_eri_hgp_quartet_sample.restype = ctypes.c_int64
         L.oracle_eri_hgp_quartet_sample.argtypes = basis_args + [ctypes.c_int64, _ip, ctypes.c_int, _dp]
         L.oracle_jk_from_unique.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -86,6 +92,14 @@ def farray(mmax, T):
     """Boys.jl:19-27."""
     out = np.zeros(mmax)
     lib().oracle_farray(int(mmax), float(T), out.ctypes.data_as(_dp))
+    return out
+
+
+def farray_accurate(mmax, T):
+    """farray (Boys.jl:19-27) without its T >= 30 asymptotic branch: the Boys
+    function for every T (what the corrected oracle and the GPU use)."""
+    out = np.zeros(mmax)
+    lib().oracle_farray_accurate(int(mmax), float(T), out.ctypes.data_as(_dp))
     return out
 
 
@@ -147,7 +161,8 @@ def n_unique(nbf):
 
 
 def eri_unique_hgp(fb, order=ORDER_CHRR, literal=0, nthreads=0):
-    """all_twoe_ints_chrr (HGP.jl:22-34) with D1-D3 corrected unless literal."""
+    """all_twoe_ints_chrr (HGP.jl:22-34) with D1-D3 and D5 (the T >= 30 Boys
+    branch, Boys.jl:21-26) corrected unless literal."""
     keep, args = _basis_args(fb)
     out = np.zeros(n_unique(fb.nbf))
     rc = lib().oracle_eri_unique_hgp(*args, int(order), int(literal), int(nthreads), out.ctypes.data_as(_dp))
@@ -165,6 +180,39 @@ def eri_unique_hgp_ext(fb, order=ORDER_CHRR, nthreads=0):
     if rc:
         raise ValueError(f"oracle_eri_unique_hgp_ext rejected the basis (code {rc})")
     return out
+
+
+def eri_unique_hgp_quad(fb, order=ORDER_CHRR, accurate=False, nthreads=0):
+    """The same HGP algorithm in binary128 (HGP.jl:50-242 with its recurrence
+    rounding ~1e-34): the exact value of the reference algorithm.  accurate:
+    Boys function evaluated exactly for every T (coulomb's Fgamma, ERI.jl:41)
+    instead of farray's T >= 30 asymptotic branch (Boys.jl:21-26)."""
+    keep, args = _basis_args(fb)
+    out = np.zeros(n_unique(fb.nbf))
+    rc = lib().oracle_eri_unique_hgp_quad(*args, int(order), int(bool(accurate)), int(nthreads),
+                                          out.ctypes.data_as(_dp))
+    if rc:
+        raise ValueError(f"oracle_eri_unique_hgp_quad rejected the basis (code {rc})")
+    return out
+
+
+def eri_slots_quad(fb, slots, order=ORDER_CHRR, accurate=False, nthreads=0):
+    """binary128 values (as eri_unique_hgp_quad) of the given 0-based slots only."""
+    keep, args = _basis_args(fb)
+    sl = np.ascontiguousarray(slots, dtype=np.int64)
+    out = np.zeros(sl.size)
+    rc = lib().oracle_eri_slots_quad(*args, int(order), int(bool(accurate)), sl.size,
+                                     sl.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(nthreads),
+                                     out.ctypes.data_as(_dp))
+    if rc:
+        raise ValueError(f"oracle_eri_slots_quad rejected the input (code {rc})")
+    return out
+
+
+def Fm_quad(m, T, accurate=True):
+    """F_m(T) from the binary128 farray (Boys.jl:19-27); accurate: no T >= 30
+    branch (the Boys function itself, Boys.jl:29-34)."""
+    return lib().oracle_Fm_quad(int(m), float(T), int(bool(accurate)))
 
 
 def eri_unique_tho(fb, order=ORDER_IINDEX):

@@ -162,3 +162,48 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_python_api_4d_expansion_matches_the_reference_loop():
+    """MolecularIntegrals.all_twoe_ints_4d expands the iindex array with the
+    0-based index algebra of python/molecular_integrals.py:467-482; the
+    vectorised expansion equals that loop, element for element."""
+    from mi_amd.molecular_integrals import _expand_4d
+    n = 5
+    ints = np.random.Generator(np.random.PCG64(0)).normal(size=n * (n + 1) * (n * n + n + 2) // 8)
+
+    def triangle(i):
+        return i * (i + 1) // 2
+
+    def iindex(i, j, k, l):
+        ij = triangle(max(i, j)) + min(i, j)
+        kl = triangle(max(k, l)) + min(k, l)
+        return triangle(max(ij, kl)) + min(ij, kl)
+
+    ref = np.zeros((n, n, n, n))
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for l in range(n):
+                    ref[i, j, k, l] = ints[iindex(i, j, k, l)]
+    assert np.array_equal(_expand_4d(ints, n), ref)
+
+
+def test_python_api_names_and_errors():
+    """Atom / Basis / MolecularIntegrals of python/molecular_integrals.py:
+    Atom xyz in bohr, Basis(atoms, name) through build_basis (names matched
+    without case), unknown methods ValueError, 'rys' not built -- all before
+    any device call."""
+    from mi_amd.molecular_integrals import Atom, Basis, MolecularIntegrals
+    atoms = [Atom(1, (0.0, 0.0, 0.0)), Atom(1, (1.4, 0.0, 0.0))]
+    assert atoms[1].xyz == (1.4, 0.0, 0.0) and atoms[0].atno == 1
+    b = Basis(atoms, "STO3G")
+    assert len(b) == 2 and b.name == "sto3g"
+    assert len(Basis([Atom(6, (0, 0, 0))], "cc-pVDZ")) == 15
+    with pytest.raises(KeyError):
+        Basis(atoms, "nonsense")
+    mi = MolecularIntegrals()
+    with pytest.raises(ValueError):
+        mi.all_twoe_ints(b, method="bogus")
+    with pytest.raises(NotImplementedError):
+        mi.all_twoe_ints(b, method="rys")

@@ -83,6 +83,63 @@ def test_farray_branches():
     assert F[0] == pytest.approx(float(boys_mp(0, T)), rel=1e-14)
 
 
+@pytest.mark.parametrize("m", [0, 3, 6, 9, 12])
+def test_farray_asymptotic_branch_against_mpmath(m):
+    """farray's T >= 30 branch (Boys.jl:21-26) for the orders HGP reaches with
+    f shells (m <= 12): the reference drops e^{-T}, so it is exact for F_0 up to
+    erfc(sqrt T) and off by up to ~1e-4 relative for F_12 at T = 30 (the error
+    each order carries is pinned here, against the 40-digit Boys function).
+    The binary128 oracle reproduces the branch in its reference mode and is
+    exact in its accurate mode."""
+    for T in [30.0, 33.0, 45.0, 80.0, 250.0, 1e4]:
+        ex = float(boys_mp(m, T))
+        ref_branch = math.sqrt(math.pi / 2) * math.prod(range(2 * m - 1, 0, -2)) / (2 * T) ** (m + 0.5)
+        F = O.farray(m + 1, T)
+        assert F[m] == pytest.approx(ref_branch, rel=1e-15)
+        # the branch's own error: the dropped upper incomplete gamma tail
+        mpmath.mp.dps = 40
+        tail = float(mpmath.gammainc(m + 0.5, T) / (2 * mpmath.mpf(T) ** (m + 0.5)))
+        assert F[m] == pytest.approx(ex + tail, rel=4e-16)
+        assert O.Fm_quad(m, T, accurate=True) == pytest.approx(ex, rel=1e-15)
+        assert O.Fm_quad(m, T, accurate=False) == pytest.approx(ref_branch, rel=1e-15)
+    assert abs(O.farray(13, 30.0)[12] / float(boys_mp(12, 30.0)) - 1) > 1e-5
+
+
+def test_quad_oracle_against_mpmath_tho_fixture():
+    """The binary128 oracle against the committed mpmath THO fixture
+    (tests/golden/cluster40_outliers.npz, written by make_cluster_outliers.py):
+    450 integrals of the 920-function cluster.  Accurate mode: the same value
+    to ~1e-16 relative (an independent algorithm, THO instead of HGP, at 40
+    digits).  The fixture's hard kinds are where the cheaper restatements miss
+    the parity bar, which pins the regime where only the binary128 oracle is
+    right: FP64 HGP (kind 0, HRR cancellation), x87 80-bit HGP (kind 1), and
+    the reference's own T >= 30 Boys branch (kind 2, literal D5)."""
+    import os
+    from mi_amd import configs, sub_basis
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cluster40_outliers.npz"))
+    wl, _ = configs.cluster_40(0)
+    fb = wl.flat()
+    qa, qr, d64, d80, idx = [], [], [], [], []
+    for k, sh in enumerate(g["shells"]):
+        sb = sub_basis(fb, sh)
+        sel_i = np.nonzero(g["set_id"] == k)[0]
+        sel = g["slot"][sel_i]
+        idx.append(sel_i)
+        qa.append(O.eri_slots_quad(sb, sel, accurate=True, nthreads=8))
+        qr.append(O.eri_slots_quad(sb, sel, accurate=False, nthreads=8))
+        d64.append(O.eri_unique_hgp(sb, order=O.ORDER_CHRR, nthreads=8)[sel])
+        d80.append(O.eri_unique_hgp_ext(sb, order=O.ORDER_CHRR, nthreads=8)[sel])
+    order = np.concatenate(idx)
+    ex, kind = g["exact"][order], g["kind"][order]
+    qa, qr, d64, d80 = (np.concatenate(x) for x in (qa, qr, d64, d80))
+    assert np.all(np.abs(qa - ex) <= 1e-15 * np.abs(ex) + 1e-25)
+    bar = lambda x: (np.abs(x - ex) <= 1e-12) | (np.abs(x - ex) <= 1e-10 * np.abs(ex))
+    assert not bar(d64)[kind == 0].any()      # FP64 HGP misses every kind-0 integral
+    assert (~bar(d80)[kind == 1]).sum() >= 10  # x87 80-bit HGP misses kind-1 integrals
+    assert (~bar(qr)[kind == 2]).sum() >= 1    # the reference's T >= 30 branch misses the bar
+    assert bar(d64)[kind == 3].all()          # controls: FP64 HGP is fine there
+
+
 # ------------------------------------------------------------------ AO tables
 def test_ao_table_kats():
     assert [O.nao(l) for l in range(3)] == [1, 4, 10]
