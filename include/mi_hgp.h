@@ -132,6 +132,20 @@ int mi_ctx_set_fusion(mi_ctx *ctx, int32_t on);
  * src/HGP.jl:22) and all_twoe_ints (order IINDEX, src/ERI.jl:104). */
 int mi_eri_unique(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, double *out, int64_t out_len);
 
+/* mi_eri_unique for callers that cannot allocate device memory (code that
+ * must not call hipMalloc: tight serving loops, harnesses that forbid it).
+ * No counterpart in the reference.  The output is the caller's DEVICE buffer
+ * (out_len = mi_n_unique(nbf)); every other device buffer of the call comes
+ * from a static 64 MiB scratch area that each device holds for the life of
+ * the process, so the call makes no device allocation.  A basis whose plan
+ * needs more returns MI_ENOMEM (about 60-300 functions fit, depending on
+ * contraction and f shells).  Runs on opts->stream and synchronises it before
+ * returning.  All calls on one device share the area: calls that overlap on
+ * one device (other host threads) race.  Values are those of mi_eri_unique
+ * (bit for bit in plain FP64). */
+int mi_eri_unique_noalloc(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, double *out_device,
+                          int64_t out_len);
+
 /* Plan / execute split: the plan holds the basis topology on the device
  * (shell pairs sorted by class, tasks); execute recomputes shell-pair data
  * and all ERIs into a caller-owned DEVICE buffer on opts->stream, without

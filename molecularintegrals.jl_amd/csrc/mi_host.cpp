@@ -248,6 +248,17 @@ int upload(mi_plan* P, T** dst, const T* src, size_t n) {
   mi_ctx* ctx = P->ctx;
   if (n == 0) n = 1;
   const size_t bytes = sizeof(T) * n;
+  if (P->arena) {
+    const size_t at = (P->arena_used + 255) & ~(size_t)255;
+    if (at + bytes > P->arena_cap)
+      return set_err(ctx, MI_ENOMEM, "the basis needs more device memory than the %zu-byte static scratch area of "
+                     "the allocation-free entry point", P->arena_cap);
+    *dst = reinterpret_cast<T*>(P->arena + at);
+    P->arena_used = at + bytes;
+    if (src) HIP_TRY(ctx, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, P->arena_stream));
+    P->device_bytes += bytes;
+    return MI_OK;
+  }
   if (P->alloc_stream) {
     HIP_TRY(ctx, hipMallocAsync((void**)dst, bytes, P->alloc_stream));
     if (src) HIP_TRY(ctx, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, P->alloc_stream));
@@ -264,6 +275,7 @@ void free_plan_buffers(mi_plan* p) {
   if (p->ev_fork) hipEventDestroy(p->ev_fork);
   for (auto& e : p->ev_join)
     if (e) hipEventDestroy(e);
+  if (p->arena) return;  // buffers live in the static scratch area
   void* bufs[] = {p->d_shell_xyz, p->d_shell_prim_off, p->d_prim_exp, p->d_prim_cn, p->d_bf_scale, p->d_topo,
                   p->d_geom,      p->d_pp,             p->d_pp_pair,  p->d_tasks,  p->d_scratch,   p->d_bounds,
                   p->d_block_task, p->d_pp_ab, p->d_run_ctr};
@@ -289,6 +301,14 @@ int64_t mi_n_unique(int32_t nbf) {
 
 const char* mi_last_error(const mi_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+// Static scratch area of mi_eri_unique_noalloc: every device the library
+// runs on holds its own copy for the life of the process.  Host code never
+// uses the name `scratch` as an address (on the host it is a stand-in): each
+// context reads the device address once, with its device current, through
+// hipGetSymbolAddress into ctx->noalloc_scratch.
+#define MI_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) char scratch[MI_NOALLOC_SCRATCH_BYTES];
+
 int mi_create(int device, mi_ctx** out) {
   if (!out) return MI_EINVAL;
   *out = nullptr;
@@ -303,6 +323,12 @@ int mi_create(int device, mi_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) {
     delete c;
     return MI_EDEVICE;
+  }
+  // device address of this device's static scratch area (null if unavailable:
+  // then only mi_eri_unique_noalloc fails)
+  if (hipGetSymbolAddress((void**)&c->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
+    c->noalloc_scratch = nullptr;
+    (void)hipGetLastError();
   }
   std::vector<double> tab = make_boys_table();
   if (hipMalloc(&c->d_boys, sizeof(double) * tab.size()) != hipSuccess ||
@@ -1027,7 +1053,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
       else if (L.var == 4) L.scratch_off = take(soff, mi::wx_scratch_doubles(L.cls, L.nblocks));
     P->scratch_doubles = std::max(off, soff);
   }
-  P->nlanes = std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
+  P->nlanes = (flags & MI_BUILD_ONE_LANE) ? 1 : std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
   assign_lanes(P);
   build_units(P);
   assign_unit_lanes(P);
@@ -1485,6 +1511,41 @@ int mi_eri_unique(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, double* o
     if (e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "copy back failed: %s", hipGetErrorString(e));
   }
   hipFree(d_out);
+  mi_plan_destroy(P);
+  return rc;
+}
+
+// mi_eri_unique without device allocation: the plan's device buffers (pair
+// and primitive-pair records, tasks, block tables, HRR scratch of the
+// global-table kernel) are carved from the static scratch area; the output is
+// the caller's device buffer.  One lane (the caller's stream), no Schwarz pass
+// at build time (mixed and screened plans run it inside the execute).  The
+// host images of the buffers are freed on return, so the call synchronises
+// its stream.  Calls that overlap on one device race on the shared area.
+int mi_eri_unique_noalloc(mi_ctx* ctx, const mi_basis* b, const mi_opts* opts, double* out_device,
+                          int64_t out_len) {
+  if (!ctx) return MI_EINVAL;
+  if (!out_device) return set_err(ctx, MI_EINVAL, "output pointer is NULL");
+  int rc = validate(ctx, b);
+  if (rc) return rc;
+  if (out_len != mi_n_unique(b->nbf))
+    return set_err(ctx, MI_ESIZE, "output length %lld, basis needs %lld", (long long)out_len,
+                   (long long)mi_n_unique(b->nbf));
+  if (opts && opts->order == MI_ORDER_JK) return set_err(ctx, MI_EINVAL, "order MI_ORDER_JK writes no unique array");
+  if (!ctx->noalloc_scratch) return set_err(ctx, MI_EDEVICE, "static scratch area unavailable on device %d", ctx->device);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  mi_plan* P = nullptr;
+  if ((rc = mi_plan_build(ctx, b, 1, opts, MI_BUILD_DEFER_UPLOAD | MI_BUILD_NO_BOUNDS | MI_BUILD_ONE_LANE, &P)))
+    return rc;
+  hipStream_t s = opts ? (hipStream_t)opts->stream : nullptr;
+  P->arena = ctx->noalloc_scratch;
+  P->arena_cap = MI_NOALLOC_SCRATCH_BYTES;
+  P->arena_stream = s;
+  rc = mi_plan_upload(P, nullptr);
+  if (rc == MI_OK) rc = mi_plan_run(P, out_device, out_len, s, nullptr, nullptr);
+  // the copies read P's host images: wait before they are freed (also after an error)
+  const hipError_t e = hipStreamSynchronize(s);
+  if (rc == MI_OK && e != hipSuccess) rc = set_err(ctx, MI_EDEVICE, "execute failed: %s", hipGetErrorString(e));
   mi_plan_destroy(P);
   return rc;
 }

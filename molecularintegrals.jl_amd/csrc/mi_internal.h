@@ -28,6 +28,7 @@ struct TuneEntry {
 
 struct mi_ctx {
   int device = 0;
+  char* noalloc_scratch = nullptr;  // device address of the static scratch area on this device (or null)
   double* d_boys = nullptr;
   void* wtables = nullptr;
   std::string err;
@@ -107,6 +108,12 @@ struct mi_plan {
   // and retiring a plan never waits for work on other streams.  The host
   // copies of uploaded data live as long as the plan.
   hipStream_t alloc_stream = nullptr;
+  // Arena placement (the entry point that allocates nothing): device buffers
+  // are carved from [arena, arena + arena_cap) and filled on arena_stream;
+  // nothing is freed.
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  hipStream_t arena_stream = nullptr;
   bool bounds_only = false;         // Schwarz pass only (mi_plan_build with MI_BUILD_BOUNDS_ONLY)
   double2* d_bounds = nullptr;      // per pair (Q, V) computed when the plan was built, or null
   int32_t* d_block_task = nullptr;  // register launches: task of each block's first wavefront
@@ -141,10 +148,12 @@ int mi_set_err(mi_ctx* c, int code, const char* fmt, ...);
 //                          (screening and FP32 selection then happen in the
 //                          kernels on bounds from a per-execute pass);
 //   MI_BUILD_BOUNDS_ONLY   a plan of the diagonal quartets alone, whose
-//                          execute fills the per-pair bounds.
+//                          execute fills the per-pair bounds;
+//   MI_BUILD_ONE_LANE      one stream lane (no events are created).
 #define MI_BUILD_DEFER_UPLOAD 1
 #define MI_BUILD_NO_BOUNDS 2
 #define MI_BUILD_BOUNDS_ONLY 4
+#define MI_BUILD_ONE_LANE 8      // every launch on the caller's stream (no fork/join events)
 int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, int flags, mi_plan** out);
 int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream);
 // One execute on stream (ev: per-launch timing events or nullptr; jk: the

@@ -19,6 +19,7 @@ MI_PREC_FP64, MI_PREC_MIXED = 0, 1
 
 # Every symbol include/mi_hgp.h declares (checked by the CPU test suite).
 EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique", "mi_eri_unique",
+            "mi_eri_unique_noalloc",
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
             "mi_plan_launch_info", "mi_plan_profile", "mi_plan_jk_profile", "mi_plan_balance", "mi_ctx_set_tuning", "mi_ctx_set_streams",
             "mi_ctx_set_fusion", "mi_plan_nunit", "mi_plan_unit_info", "mi_plan_profile_units",
@@ -97,6 +98,9 @@ def load():
     L.mi_eri_unique.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts),
                                 ctypes.POINTER(ctypes.c_double), ctypes.c_int64]
     L.mi_eri_unique.restype = ctypes.c_int
+    L.mi_eri_unique_noalloc.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts),
+                                vp, ctypes.c_int64]
+    L.mi_eri_unique_noalloc.restype = ctypes.c_int
     L.mi_plan_create.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts), ctypes.POINTER(vp)]
     L.mi_plan_create.restype = ctypes.c_int
     L.mi_plan_execute.argtypes = [vp, vp, ctypes.c_int64, vp]
