@@ -115,21 +115,24 @@ static double hrr_amplification(const double* xyz, const std::vector<int32_t>& L
   return amp;
 }
 
+// Largest HRR amplification a pair with L_B transferred units may keep in
+// FP64: 100 for d and f transfers (pairs of amplification 290-490 already
+// miss the 1e-12/1e-10 bar), 1e4 for a p transfer, whose single HRR level
+// stays below the bar up to that point on every sampled workload (the seeded
+// 920-function cluster's 100,100 benchmark quartets, 165,000 quartets of the
+// 512-molecule set).
+static double amp_limit(int lb) { return lb == 1 ? 1.0e4 : 100.0; }
+
 // Orientation (A, B) of the shell pair {I, J}: the canonical L_A >= L_B, with
 // the tighter shell on A between shells of equal L (the smaller HRR
 // amplification; plain FP64 HGP otherwise loses up to 1e-8 absolute on
 // far-apart tight/diffuse pairs of f shells), and swapped to L_A < L_B when
-// the canonical orientation amplifies more than MI_SWAP_AMP and the swapped
-// one less.  Swapped pairs form their own pair classes and run in the
-// table-driven kernels; on the seeded workloads 0.1-1% of pairs swap.
-#define MI_SWAP_AMP 100.0
-// Pairs whose best orientation still amplifies more than this (contracted
-// shells mixing tight and diffuse primitives several bohr apart: 0.2% of the
-// pairs of the seeded 920-function cluster) put their quartets on the
-// double-double table kernel (mi_kernels.hip, hgp_wx_kernel), MI_WX_BLOCKS
-// persistent blocks per class launch.
-#define MI_DD_AMP 100.0
-#define MI_WX_BLOCKS 128
+// the canonical orientation amplifies beyond amp_limit and the swapped one
+// less.  Swapped pairs form their own pair classes and run in the
+// table-driven kernels; on the seeded cluster 0.1% of pairs swap.
+// Double-double launches: one wavefront per block, as many persistent blocks
+// as MI_WX_SCRATCH bytes of work areas allow (16 bytes per VRR/HRR entry).
+#define MI_WX_SCRATCH ((size_t)16 << 20)
 static void orient_pair(const double* xyz, const std::vector<int32_t>& L, const std::vector<int32_t>& poff,
                         const std::vector<double>& pexp, int I, int J, int& A, int& B) {
   A = I;
@@ -137,7 +140,7 @@ static void orient_pair(const double* xyz, const std::vector<int32_t>& L, const 
   if (L[J] > L[I]) std::swap(A, B);
   if (I == J) return;
   const double keep = hrr_amplification(xyz, L, poff, pexp, A, B);
-  if (L[A] > L[B] && keep <= MI_SWAP_AMP) return;
+  if (L[A] > L[B] && keep <= amp_limit(L[B])) return;
   if (hrr_amplification(xyz, L, poff, pexp, B, A) < keep) std::swap(A, B);
 }
 int quartet_class(int X, int Y) { return X >= Y ? X * (X + 1) / 2 + Y : Y * (Y + 1) / 2 + X; }
@@ -431,10 +434,10 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
     // vk 0: register kernel (each task its own G); 1-3: wave kernel with
     // 1, 2, 4 wavefronts; 4-7: global-table kernel with 1, 2, 4, 8
     // wavefronts; 8-10: wave kernel with global op tables, 1, 2, 4 wavefronts;
-    // 11: double-double table kernel (4 wavefronts)
+    // 11: double-double table kernel (1 wavefront)
     for (int vk = 0; vk < 12; ++vk) {
       const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : (vk < 11 ? 3 : 4)));
-      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (vk < 8 ? (1 << (vk - 4)) : (vk < 11 ? (1 << (vk - 8)) : 4)));
+      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (vk < 8 ? (1 << (vk - 4)) : (vk < 11 ? (1 << (vk - 8)) : 1)));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
         if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
@@ -473,7 +476,10 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
         blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw, var == 3));
       // double-double kernel: a few persistent blocks (its quartets are rare,
       // and each block holds a 16-byte-per-entry work area in the scratch)
-      if (var == 4) blocks = std::min<int64_t>(qs, MI_WX_BLOCKS);
+      if (var == 4) {
+        const int64_t per = (int64_t)(mi::wx_scratch_doubles(c, 1) * sizeof(double));
+        blocks = std::min<int64_t>(qs, std::max<int64_t>(64, std::min<int64_t>(16384, (int64_t)MI_WX_SCRATCH / per)));
+      }
       L.tl = false;
       if (var == 2) {
         // tables in LDS only where that costs no resident blocks: a launch
@@ -742,7 +748,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     int mol, pc, bucket, pidx, A, B, K;
     double Q, V;
     int64_t ab0;  // first kept (ia, ib) code in ab_codes
-    int ext;      // HRR amplification above MI_DD_AMP: quartets run in double-double
+    int ext;      // HRR amplification above amp_limit: quartets run in double-double
   };
   std::vector<PairKey> keys;
   std::vector<int32_t> ab_codes;
@@ -771,7 +777,12 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
           }
         const int K = (int)((int64_t)ab_codes.size() - ab0);
         const int i = I - mol_sh0[m], j = J - mol_sh0[m];
-        // amplification of the kept primitive pairs in the chosen orientation
+        // amplification of the kept primitive pairs in the chosen orientation;
+        // above amp_limit (contracted shells mixing tight and diffuse
+        // primitives several bohr apart, whichever way they are oriented) the
+        // pair's quartets run in the double-double table kernel (hgp_wx_kernel):
+        // 2.5% of the quartets of random contracted-f test bases, none of the
+        // seeded cluster or 512-molecule set
         double amp = 1.0;
         if (L[B] > 0) {
           const double r = std::sqrt(r2);
@@ -782,7 +793,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
           }
         }
         keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0, ab0,
-                        amp > MI_DD_AMP ? 1 : 0});
+                        amp > amp_limit(L[B]) ? 1 : 0});
       }
   // J/K layout of a plan without a cut: within a (class, bucket) list, pairs
   // of equal first shell A are contiguous, so along a task row (one bra
@@ -966,7 +977,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
       const int bb = t.G, kb = t.variant;
       if (kext[c][k]) {  // a pair of the quartet cancels beyond FP64: the double-double kernel
         t.variant = 4;
-        t.G = 4;
+        t.G = 1;
         continue;
       }
       if ((rc = choose_kernel(ctx, t, bb, kb, kavgs[c][k], cls_nq[c] < MI_SMALL_LAUNCH ? 0 : 1,
@@ -1010,7 +1021,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         const int bk = keys[li.start].bucket;
         if (li.ext) {
           t.variant = 4;
-          t.G = 4;
+          t.G = 1;
         } else if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n), 0,
                                        false))) {
           delete P;

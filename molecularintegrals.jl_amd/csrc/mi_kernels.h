@@ -38,8 +38,8 @@ int wg_kernel_blocks_per_cu(int c, int nw, bool tl);
 size_t wg_scratch_doubles(int c, int nblocks);
 hipError_t launch_class_wave_global(int c, int nw, bool tl, const EriArgs& args, int nblocks, int64_t nq_launch,
                                     void* wtables, double* scratch, hipStream_t stream);
-// extended-precision (double-double) table kernel, variant 4: 256 threads per
-// block, one quartet per block at a time; scratch = wx_scratch_doubles(c,
+// extended-precision (double-double) table kernel, variant 4: one wavefront
+// per block, one quartet per block at a time; scratch = wx_scratch_doubles(c,
 // nblocks) doubles, 16-byte aligned
 size_t wx_scratch_doubles(int c, int nblocks);
 hipError_t launch_class_extended(int c, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,

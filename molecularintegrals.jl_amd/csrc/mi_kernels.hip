@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(512) hgp_wgf_kernel(const EriArgs args, const 
 // ---------------------------------------------------------------------------
 // Extended-precision table kernel (tuning variant 4): the global-table
 // kernel's op tables evaluated in double-double arithmetic for the quartets
-// of shell pairs whose HRR would cancel more than MI_DD_AMP of FP64's digits
+// of shell pairs whose HRR would amplify rounding beyond amp_limit (mi_host.cpp)
 // in either orientation (contracted pairs mixing tight and diffuse primitives,
 // several bohr apart: mi_host.cpp, pair orientation).  Inputs of the
 // recurrences (PA, WP, QC, WQ, the n/2zeta factors, AB, CD, the [0|0]^(m)
@@ -551,9 +551,10 @@ __global__ void __launch_bounds__(512) hgp_wgf_kernel(const EriArgs args, const 
 // cancellation happens on numbers that are exact to far below the bar.
 // Double-double arithmetic: Knuth's TwoSum, Dekker's FastTwoSum and the
 // fma-based TwoProd (exact product), the usual "sloppy" pair addition plus
-// one renormalisation.  One quartet per block at a time (persistent blocks),
-// VRR and HRR areas in a global slice of the plan scratch per block (16 bytes
-// per entry: a (ff|ff) VRR area is 276 KB, beyond LDS).
+// one renormalisation.  One wavefront per block, one quartet per block at a
+// time (persistent blocks), VRR and HRR areas in a global slice of the plan
+// scratch per block (16 bytes per entry: a (ff|ff) VRR area is 276 KB,
+// beyond LDS).
 // ---------------------------------------------------------------------------
 #pragma clang fp contract(off)
 struct ddv {
@@ -587,7 +588,7 @@ __device__ __forceinline__ ddv dd_load(const double2* a, int i) {
 }
 __device__ __forceinline__ void dd_store(double2* a, int i, ddv v) { a[i] = make_double2(v.hi, v.lo); }
 
-__global__ void __launch_bounds__(256) hgp_wx_kernel(const EriArgs args, const WClassDesc D,
+__global__ void __launch_bounds__(64) hgp_wx_kernel(const EriArgs args, const WClassDesc D,
                                                      const uint32_t* __restrict__ gblob, double2* __restrict__ scratch,
                                                      int64_t nq_launch) {
   __shared__ double scl[4 * W_SCL];
@@ -966,7 +967,7 @@ hipError_t launch_class_extended(int c, const EriArgs& args, int nblocks, int64_
   const uint32_t* gb = t->gblob;
   double2* sc = reinterpret_cast<double2*>(scratch);
   void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<WClassDesc*>(&D), &gb, &sc, &nq_launch};
-  return hipLaunchKernel((const void*)hgp_wx_kernel, dim3(nblocks), dim3(256), kargs, 0, stream);
+  return hipLaunchKernel((const void*)hgp_wx_kernel, dim3(nblocks), dim3(64), kargs, 0, stream);
 }
 
 size_t wg_scratch_doubles(int c, int nblocks) { return wg_hrr_in_lds(c) ? 0 : (size_t)nblocks * kWClassDesc[c].nhrr; }

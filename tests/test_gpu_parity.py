@@ -773,3 +773,32 @@ def test_swapped_pair_classes(kernels):
     Jr, Kr = O.jk_from_unique(fb.nbf, O.eri_unique_hgp_quad(fb, order=O.ORDER_IINDEX, nthreads=8), D)
     assert np.abs(J - Jr).max() <= 1e-10 and np.abs(K - Kr).max() <= 1e-10
     c.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_double_double_pairs(ctx, mode):
+    """Contracted f shells mixing tight and diffuse primitives a few bohr
+    apart cancel in the HRR whichever way the pair is oriented (amplification
+    290-490): those pairs' quartets run in the double-double table kernel
+    (launches "X").  Every integral at the parity bar (binary128 arbitration),
+    the Schwarz pass of the mixed form through the same kernel, and J/K of the
+    same basis against the oracle algebra on the exact array."""
+    from mi_amd import flat_from_shells
+    fb = flat_from_shells(random_spec(np.random.Generator(np.random.PCG64(952)), 8, lmax=3, kmax=3))
+    kw = {} if mode == "fp64" else dict(precision=1, tau=1e-8, schwarz_cut=1e-12)
+    plan = ctx.plan(fb, **kw)
+    assert any(l[4].startswith("X") for l in plan.launches()), plan.launches()
+    plan.close()
+    for order in (O.ORDER_CHRR, O.ORDER_IINDEX):
+        got = ctx.eri_unique(fb, order=order, **kw)
+        ref = O.eri_unique_hgp(fb, order=order, nthreads=8)
+        if mode == "fp64":
+            assert_eri_close(got, ref, f"double-double pairs order={order}", fb, order)
+        else:
+            exact = O.eri_unique_hgp_quad(fb, order=order, nthreads=8)
+            assert np.abs(got - exact).max() <= 1e-10
+    D = np.random.Generator(np.random.PCG64(6)).normal(size=(fb.nbf, fb.nbf))
+    D = (D + D.T) / fb.nbf
+    J, K = ctx.jk(fb, D, **kw)
+    Jr, Kr = O.jk_from_unique(fb.nbf, O.eri_unique_hgp_quad(fb, order=O.ORDER_IINDEX, nthreads=8), D)
+    assert np.abs(J - Jr).max() <= 1e-10 and np.abs(K - Kr).max() <= 1e-10
