@@ -276,8 +276,9 @@ int mi_server_release(mi_server *server, int64_t id);
 /* Run the engine until the queue is empty and nothing is in flight
  * (blocking); MI_EDEVICE if any request failed. */
 int mi_server_drain(mi_server *server);
-/* st[0..6] = queued, running, done, failed, batches launched, device bytes
- * in use, peak device bytes. */
+/* st[0..7] = queued, running (requests on the device), done, failed,
+ * batches launched, device bytes in use, peak device bytes, batches in
+ * flight. */
 int mi_server_stats(const mi_server *server, int64_t *stats, int32_t nstats);
 
 #ifdef __cplusplus

@@ -348,9 +348,9 @@ int mi_server_stats(const mi_server* s, int64_t* st, int32_t n) {
   if (!s || !st) return MI_EINVAL;
   int64_t running = 0;
   for (const Batch& b : s->inflight) running += (int64_t)b.reqs.size();
-  const int64_t v[7] = {(int64_t)s->queue.size(), running,          s->ndone, s->nfailed, s->nbatches,
-                        (int64_t)s->in_use,      (int64_t)s->peak};
-  for (int i = 0; i < n && i < 7; ++i) st[i] = v[i];
+  const int64_t v[8] = {(int64_t)s->queue.size(), running,          s->ndone, s->nfailed, s->nbatches,
+                        (int64_t)s->in_use,      (int64_t)s->peak, (int64_t)s->inflight.size()};
+  for (int i = 0; i < n && i < 8; ++i) st[i] = v[i];
   return MI_OK;
 }
 

@@ -63,9 +63,9 @@ class Server:
         self.check(self.L.mi_server_drain(self.h))
 
     def stats(self) -> dict:
-        st = (ctypes.c_int64 * 7)()
-        self.check(self.L.mi_server_stats(self.h, st, 7))
-        keys = ["queued", "running", "done", "failed", "batches", "bytes_in_use", "peak_bytes"]
+        st = (ctypes.c_int64 * 8)()
+        self.check(self.L.mi_server_stats(self.h, st, 8))
+        keys = ["queued", "running", "done", "failed", "batches", "bytes_in_use", "peak_bytes", "inflight_batches"]
         return dict(zip(keys, [int(x) for x in st]))
 
     def close(self):
@@ -87,6 +87,20 @@ class ServeResult:
     unique_eris: int = 0
     outputs: Dict[int, tuple] = field(default_factory=dict)     # rid -> device tensors
     stats: dict = field(default_factory=dict)
+    timeline: List[tuple] = field(default_factory=list)         # (t, batches in flight, requests running)
+
+    def overlap(self) -> dict:
+        """Batches in flight over the run: time-weighted mean, maximum, and the
+        fraction of the makespan with two or more batches on the device."""
+        if len(self.timeline) < 2:
+            return {"mean_batches_in_flight": 0.0, "max_batches_in_flight": 0, "frac_time_2plus_batches": 0.0}
+        t = np.array([x[0] for x in self.timeline])
+        b = np.array([x[1] for x in self.timeline], dtype=float)
+        dt = np.diff(t)
+        tot = float(dt.sum()) or 1.0
+        return {"mean_batches_in_flight": float((b[:-1] * dt).sum() / tot),
+                "max_batches_in_flight": int(b.max()),
+                "frac_time_2plus_batches": float(dt[b[:-1] >= 2].sum() / tot)}
 
     def p(self, q: float) -> float:
         return float(np.quantile(np.array(list(self.latency_s.values())), q))
@@ -142,6 +156,9 @@ def serve(requests: List[Request], mixed: bool = False, max_inflight: int = 4, m
             res.outputs[r.rid] = bufs
         srv.step()
         now = time.perf_counter() - t0
+        st = srv.stats()
+        if not res.timeline or res.timeline[-1][1:] != (st["inflight_batches"], st["running"]):
+            res.timeline.append((now, st["inflight_batches"], st["running"]))
         for rid in list(running):
             st = srv.poll(rid)
             if st == _lib.MI_REQ_DONE:
@@ -155,6 +172,7 @@ def serve(requests: List[Request], mixed: bool = False, max_inflight: int = 4, m
             if wait > 0:
                 time.sleep(min(wait, 1e-3))
     res.makespan_s = time.perf_counter() - t0
+    res.timeline.append((res.makespan_s, 0, 0))
     res.stats = srv.stats()
     srv.close()
     return res
