@@ -204,7 +204,8 @@ int mi_plan_nlaunch(const mi_plan *plan);
  * or MI_EINVAL on a bad index. */
 int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *counts3, int32_t *kernel2);
 /* One execute with HIP events between launches on `stream`; synchronises and
- * writes nlaunch+1 durations in ms (ms[0] = shell-pair kernel). */
+ * writes nlaunch+1 durations in ms (ms[0] = shell-pair kernel, with the
+ * Schwarz pass and the output zero fill of a screening plan). */
 int mi_plan_profile(mi_plan *plan, double *out_device, int64_t out_len, void *stream, float *ms);
 /* One profiled execute (output written as by mi_plan_execute), then the
  * launch units are reassigned to stream lanes by their measured durations
@@ -223,7 +224,8 @@ int mi_plan_nunit(const mi_plan *plan);
 int mi_plan_unit_info(const mi_plan *plan, int32_t u, int32_t *kind3, int32_t *members, int32_t cap);
 /* One execute issued as mi_plan_execute issues it, with HIP events between
  * the units on `stream`; synchronises and writes nunit+1 durations in ms
- * (ms[0] = shell-pair kernel). */
+ * (ms[0] = shell-pair kernel, with the Schwarz pass and the output zero fill
+ * of a screening plan). */
 int mi_plan_profile_units(mi_plan *plan, double *out_device, int64_t out_len, void *stream, float *ms);
 /* The same for one Coulomb/exchange execute (device arrays as in
  * mi_plan_jk_execute); ms[0] includes the Schwarz pass, ms[1] the zeroing

@@ -1203,7 +1203,7 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
       if (rc) return rc;
     }
   }
-  if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], s));
+  if (ev && jk) HIP_TRY(ctx, hipEventRecord(ev[1], s));
   a.tau2 = P->tau2;
   a.cut2 = P->cut2;
   if (jk) {
@@ -1222,6 +1222,8 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
     // screened quartets are never written: their integrals are zero
     HIP_TRY(ctx, mi::launch_zero(out, P->n_unique, s));
   }
+  // unique-array profiles: the zero fill is timed with the shell-pair kernel
+  if (ev && !jk) HIP_TRY(ctx, hipEventRecord(ev[1], s));
   // Class launches are independent (disjoint output slots): fork them over the
   // plan's stream lanes and join back into the caller's stream.
   // Unique-array executes issue the plan's units (fused launches included);

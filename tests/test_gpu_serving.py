@@ -140,3 +140,34 @@ def test_poisson_schedule_latency_and_mixed_error():
         err = max(err, max(np.abs(x - y).max() for x, y in zip(a[k:], b[k:])))
     print(f"mixed vs FP64 max |d| = {err:.2e}")
     assert err <= 1e-10
+
+
+def test_step_does_not_wait_for_a_mixed_batch():
+    """A mixed, screened J/K request (a 1,000-function cluster-recipe
+    molecule, seconds of device work) is admitted by one mi_server_step call
+    that returns while the batch still runs: the engine builds served plans
+    without the synchronous bounds pass (their Schwarz pass runs inside the
+    execute, on the batch stream)."""
+    import time
+    import torch
+    from mi_amd import _lib, configs
+    from mi_amd.serving import Server
+    rng = np.random.Generator(np.random.PCG64(12))
+    wl = configs.random_molecule(rng, 44, configs.CLUSTER_HEAVY, configs.CLUSTER_LIGHT)
+    fb = wl.flat()
+    n = fb.nbf
+    D = torch.from_numpy(0.5 * (lambda d: d + d.T)(rng.normal(size=(n, n)) / n)).cuda()
+    J = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    K = torch.empty_like(J)
+    srv = Server(0)
+    rid = srv.submit(fb, _lib.MI_REQ_JK, D_ptr=D.data_ptr(), J_ptr=J.data_ptr(), K_ptr=K.data_ptr(),
+                     precision=_lib.MI_PREC_MIXED, tau=1e-8, schwarz_cut=1e-12)
+    t = time.perf_counter()
+    srv.step()
+    dt = time.perf_counter() - t
+    st = srv.poll(rid)
+    print(f"{n} functions: step returned after {dt * 1e3:.1f} ms, state {st}")
+    assert st == _lib.MI_REQ_RUNNING
+    srv.drain()
+    assert srv.poll(rid) == _lib.MI_REQ_DONE
+    srv.close()
