@@ -515,7 +515,7 @@ def leg_served_batch(fbs, plain_ms):
             return self._fb
 
     reqs = [Request(i, 0.0, _W(fb), "eri") for i, fb in enumerate(fbs)]
-    r = serve(reqs, mixed=True, max_inflight=4, max_batch=64, hbm_budget=40 << 30)
+    r = serve(reqs, mixed=True, max_inflight=8, max_batch=64, hbm_budget=40 << 30)
     n = r.unique_eris
     out = {"requests": len(reqs), "value": r.throughput, "makespan_s": r.makespan_s, "batches": r.stats["batches"],
            "p50_ms": r.p(0.5) * 1e3, "p99_ms": r.p(0.99) * 1e3, "peak_device_gb": r.stats["peak_bytes"] / 1e9,

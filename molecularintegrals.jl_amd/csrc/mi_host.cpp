@@ -649,6 +649,20 @@ int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream) {
   return MI_OK;
 }
 
+int mi_plan_set_mol_outputs(mi_plan* P, double* const* outs) {
+  if (!P || !outs) return MI_EINVAL;
+  if (P->d_tasks) return set_err(P->ctx, MI_EINVAL, "plan already uploaded");
+  P->mol_out.assign(outs, outs + P->nmol);
+  const double* base = outs[0];
+  for (TaskDesc& t : P->h.tasks) {
+    // the molecule of a task: its output offset is that molecule's mol_off
+    const int m = (int)(std::upper_bound(P->mol_off.begin(), P->mol_off.end(), t.obase) - P->mol_off.begin()) - 1;
+    if (m < 0 || m >= P->nmol || P->mol_off[m] != t.obase) return set_err(P->ctx, MI_EINVAL, "task outside the plan");
+    t.obase = (int64_t)(outs[m] - base);
+  }
+  return MI_OK;
+}
+
 // Schwarz bounds of every shell pair, in the canonical pair order
 // (molecule, pair class, contraction bucket, [first shell: J/K layout,]
 // pair index) of a plan built with the same options: a bounds-only plan
@@ -1220,7 +1234,12 @@ int mi_plan_run(mi_plan* P, double* out, int64_t out_len, void* stream, hipEvent
       HIP_TRY(ctx, mi::launch_zero(reinterpret_cast<double*>(P->d_run_ctr), (int64_t)P->launches.size(), s));
   } else if (P->cut2 > 0.0 && !P->bounds_only) {
     // screened quartets are never written: their integrals are zero
-    HIP_TRY(ctx, mi::launch_zero(out, P->n_unique, s));
+    if (P->mol_out.empty()) {
+      HIP_TRY(ctx, mi::launch_zero(out, P->n_unique, s));
+    } else {
+      for (int m = 0; m < P->nmol; ++m)
+        HIP_TRY(ctx, mi::launch_zero(P->mol_out[m], P->mol_off[m + 1] - P->mol_off[m], s));
+    }
   }
   // unique-array profiles: the zero fill is timed with the shell-pair kernel
   if (ev && !jk) HIP_TRY(ctx, hipEventRecord(ev[1], s));

@@ -119,6 +119,9 @@ struct mi_plan {
   int32_t* d_block_task = nullptr;  // register launches: task of each block's first wavefront
   unsigned long long* d_run_ctr = nullptr;  // J/K: run counter of each launch (cleared every J/K execute)
   size_t device_bytes = 0;  // bytes of device memory the plan holds (or needs, before upload)
+  // Per-molecule output buffers (mi_plan_set_mol_outputs): molecule m writes
+  // its unique array at mol_out[m] instead of out + mol_off[m].
+  std::vector<double*> mol_out;
   struct HostData {         // host images of the uploaded buffers
     std::vector<double> xyz, pexp, pcn, bfs;
     std::vector<int32_t> poff, pp_pair, pp_ab;
@@ -156,6 +159,11 @@ int mi_set_err(mi_ctx* c, int code, const char* fmt, ...);
 #define MI_BUILD_ONE_LANE 8      // every launch on the caller's stream (no fork/join events)
 int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, int flags, mi_plan** out);
 int mi_plan_upload(mi_plan* P, hipStream_t alloc_stream);
+// Before the upload of a deferred build: molecule m's unique array goes to
+// outs[m] (caller-owned device buffers of mi_n_unique(nbf_m) doubles, any
+// addresses) instead of out + mol_off[m]; executes then take out = outs[0]
+// and zero-fill each buffer separately where the plan screens.
+int mi_plan_set_mol_outputs(mi_plan* P, double* const* outs);
 // One execute on stream (ev: per-launch timing events or nullptr; jk: the
 // Coulomb/exchange arrays or nullptr for the unique array).
 // unit_events: with ev, time the launch units (fused launches as executed)

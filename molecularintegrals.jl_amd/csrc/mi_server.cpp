@@ -3,29 +3,42 @@
 // reference, which evaluates one Basis per blocking call.
 //
 // Model.  Requests are queued in arrival order.  mi_server_step is the whole
-// engine, non-blocking, called by the host loop as often as it likes:
-//   1. retire: every batch whose completion event has fired releases its plan
-//      (stream-ordered frees) and its device bytes;
+// engine on the caller's thread, non-blocking, called as often as the host
+// loop likes:
+//   1. retire: every launched batch whose completion event has fired
+//      releases its plan (stream-ordered frees) and its device bytes;
 //   2. admit: while fewer than max_inflight batches are on the device, the
-//      head of the queue is admitted if its device bytes fit the budget (it
-//      waits otherwise: first come, first served).  ERI requests at the head
-//      with equal options are merged, up to max_batch and the budget, into one
-//      batched plan (one launch per quartet class over all of them), executed
-//      into an engine-owned buffer and copied into each caller's device
-//      buffer; a Coulomb/exchange request is a batch of its own.
+//      head of the queue (once sized) is admitted if its device bytes fit the
+//      budget (it waits otherwise: first come, first served).  ERI requests
+//      at the head with equal options are merged, up to max_batch and the
+//      budget, into one batched plan (one launch per quartet class over all
+//      of them) that writes each molecule straight into its caller's buffer;
+//      a Coulomb/exchange request is a batch of its own;
 //   3. each batch runs on its own stream, so batches admitted while others
 //      run overlap with them on the device.
-// Device memory of a batch (plan buffers, HRR scratch, merged output) is
-// allocated stream-ordered on the batch stream and freed the same way, so
-// neither admission nor retirement waits for other batches.  Shell-pair data
+// The host work of a request -- sizing its device footprint at submit, and
+// for an admitted batch the plan build (with the Schwarz bounds pass that
+// orders pairs by bound and leaves screened quartets out of the tasks), the
+// upload and the launch -- runs on a pool of worker threads, each with its
+// own context, so step never waits for it and plan builds of several batches
+// overlap each other and the device.  Device memory of a batch is allocated
+// stream-ordered on the batch stream and freed the same way.  Shell-pair data
 // stays resident for the life of its batch; the Boys grid and op tables are
-// resident for the life of the server (its context).
+// resident for the life of the server (its contexts).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -45,16 +58,56 @@ struct Request {
   const double* D = nullptr;
   double *out = nullptr, *J = nullptr, *K = nullptr;
   int64_t n_unique = 0;
-  size_t bytes = 0;  // device bytes when admitted alone (plan + output)
+  size_t bytes = 0;               // device bytes of its plan alone (set by a worker)
+  std::atomic<int> sized{0};      // 0: sizing pending, 1: sized, -1: sizing failed
+  std::string why;                // sizing error
 };
+
+// Batch states: a worker builds, uploads and launches it (PREPARING), then it
+// runs on its stream until its event fires (LAUNCHED), or its host work
+// failed (BROKEN).
+enum { B_PREPARING = 0, B_LAUNCHED = 1, B_BROKEN = 2 };
 
 struct Batch {
   mi_plan* plan = nullptr;
   int slot = -1;  // stream index
   hipEvent_t done = nullptr;
-  double* buf = nullptr;  // merged ERI output (engine-owned)
-  size_t bytes = 0;
+  size_t bytes = 0;  // reserved at admission
   std::vector<int64_t> reqs;
+  std::atomic<int> state{B_PREPARING};
+  std::string why;
+};
+
+// Worker pool: jobs run on threads that each own a context (plans are built
+// and launched there; a context is used by one thread at a time).
+struct Pool {
+  std::vector<std::thread> threads;
+  std::vector<mi_ctx*> ctxs;
+  std::deque<std::function<void(mi_ctx*)>> jobs;
+  std::mutex mu;
+  std::condition_variable cv;
+  bool stop = false;
+  void push(std::function<void(mi_ctx*)> f) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      jobs.push_back(std::move(f));
+    }
+    cv.notify_one();
+  }
+  void run(mi_ctx* c) {
+    hipSetDevice(c->device);
+    for (;;) {
+      std::function<void(mi_ctx*)> f;
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&] { return stop || !jobs.empty(); });
+        if (jobs.empty()) return;  // stop requested and nothing left
+        f = std::move(jobs.front());
+        jobs.pop_front();
+      }
+      f(c);
+    }
+  }
 };
 
 bool same_opts(const mi_opts& a, const mi_opts& b) {
@@ -63,14 +116,17 @@ bool same_opts(const mi_opts& a, const mi_opts& b) {
 
 }  // namespace
 
+#define MI_SERVER_WORKERS 8
+
 struct mi_server {
   mi_ctx* ctx = nullptr;
   mi_server_opts opts{};
   std::vector<hipStream_t> streams;
   std::vector<bool> slot_busy;
   std::deque<int64_t> queue;
-  std::unordered_map<int64_t, Request> reqs;
-  std::vector<Batch> inflight;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+  std::list<std::unique_ptr<Batch>> inflight;
+  Pool pool;
   int64_t next_id = 1;
   size_t in_use = 0, peak = 0;
   int64_t nbatches = 0, ndone = 0, nfailed = 0;
@@ -84,16 +140,20 @@ int serr(mi_server* s, int code, const std::string& msg) {
   return code;
 }
 
-// Device bytes a request needs alone: its plan (built on the host only) plus
-// the merged-output share.
-int request_bytes(mi_server* s, Request& r) {
+// Device bytes a request needs alone (its plan, built on the host only; the
+// outputs are the caller's buffers).  Runs on a worker.
+void size_request(mi_ctx* c, Request* r) {
   mi_plan* P = nullptr;
-  int rc = mi_plan_build(s->ctx, &r.b, 1, &r.o, MI_BUILD_DEFER_UPLOAD | MI_BUILD_NO_BOUNDS, &P);
-  if (rc) return serr(s, rc, s->ctx->err);
-  r.n_unique = P->n_unique;
-  r.bytes = P->device_bytes + (r.kind == MI_REQ_ERI ? sizeof(double) * (size_t)P->n_unique : 0);
+  const int rc = mi_plan_build(c, &r->b, 1, &r->o, MI_BUILD_DEFER_UPLOAD | MI_BUILD_NO_BOUNDS, &P);
+  if (rc) {
+    r->why = c->err;
+    r->sized.store(-1);
+    return;
+  }
+  r->n_unique = P->n_unique;
+  r->bytes = P->device_bytes;
   mi_plan_destroy(P);
-  return MI_OK;
+  r->sized.store(1);
 }
 
 void fail(mi_server* s, Request& r, const std::string& why) {
@@ -103,111 +163,117 @@ void fail(mi_server* s, Request& r, const std::string& why) {
 }
 
 void retire(mi_server* s) {
-  for (size_t i = 0; i < s->inflight.size();) {
-    Batch& b = s->inflight[i];
-    const hipError_t q = hipEventQuery(b.done);
-    if (q == hipErrorNotReady) {
-      ++i;
+  for (auto it = s->inflight.begin(); it != s->inflight.end();) {
+    Batch& b = **it;
+    const int st = b.state.load();
+    if (st == B_PREPARING) {
+      ++it;
       continue;
     }
+    hipError_t q = hipSuccess;
+    if (st == B_LAUNCHED) {
+      q = hipEventQuery(b.done);
+      if (q == hipErrorNotReady) {
+        ++it;
+        continue;
+      }
+    }
     for (int64_t id : b.reqs) {
-      Request& r = s->reqs[id];
-      if (q == hipSuccess) {
+      Request& r = *s->reqs[id];
+      if (st == B_LAUNCHED && q == hipSuccess) {
         r.state = MI_REQ_DONE;
         ++s->ndone;
       } else {
-        fail(s, r, std::string("batch failed on the device: ") + hipGetErrorString(q));
+        fail(s, r, st == B_BROKEN ? b.why : std::string("batch failed on the device: ") + hipGetErrorString(q));
       }
     }
-    hipStream_t st = s->streams[b.slot];
-    if (b.buf) hipFreeAsync(b.buf, st);
-    mi_plan_destroy(b.plan);  // stream-ordered frees on st
-    hipEventDestroy(b.done);
+    if (b.plan) mi_plan_destroy(b.plan);  // stream-ordered frees on the batch stream
+    if (b.done) hipEventDestroy(b.done);
     s->in_use -= b.bytes;
     s->slot_busy[b.slot] = false;
-    s->inflight.erase(s->inflight.begin() + i);
+    it = s->inflight.erase(it);
   }
 }
 
-// Launch one batch of requests ids (all ERI with equal options, or one J/K)
-// on stream slot.  Returns MI_OK or an error (requests then marked failed).
-int launch(mi_server* s, const std::vector<int64_t>& ids, int slot) {
-  hipStream_t st = s->streams[slot];
-  Request& r0 = s->reqs[ids[0]];
+// Build, upload and launch one batch (runs on a worker with context c): all
+// ERI requests with equal options, or one J/K request, on stream st.
+void launch_batch(mi_server* s, mi_ctx* c, Batch* b, std::vector<Request*> rs, hipStream_t st) {
   std::vector<mi_basis> bases;
-  for (int64_t id : ids) bases.push_back(s->reqs[id].b);
-  Batch b;
-  b.slot = slot;
-  b.reqs = ids;
-  // No bounds pass at build time: that pass reads the bounds back to the host
-  // (a synchronisation), and step must not block.  Mixed and screened batches
-  // run their Schwarz pass inside the execute, on the batch stream.
-  int rc = mi_plan_build(s->ctx, bases.data(), (int)bases.size(), &r0.o, MI_BUILD_DEFER_UPLOAD | MI_BUILD_NO_BOUNDS,
-                         &b.plan);
-  if (rc == MI_OK) rc = mi_plan_upload(b.plan, st);
+  for (Request* r : rs) bases.push_back(r->b);
+  const Request& r0 = *rs[0];
+  // a screening or mixed plan runs its bounds pass here (synchronous on this
+  // worker's bounds stream), so pairs are ordered by bound and screened
+  // quartets never enter a task
+  int rc = mi_plan_build(c, bases.data(), (int)bases.size(), &r0.o, MI_BUILD_DEFER_UPLOAD, &b->plan);
   if (rc == MI_OK && r0.kind == MI_REQ_ERI) {
-    const int64_t len = b.plan->n_unique;
-    if (hipMallocAsync((void**)&b.buf, sizeof(double) * (size_t)len, st) != hipSuccess) {
-      rc = mi_set_err(s->ctx, MI_ENOMEM, "batch output of %lld doubles", (long long)len);
+    std::vector<double*> outs;
+    for (Request* r : rs) outs.push_back(r->out);
+    rc = mi_plan_set_mol_outputs(b->plan, outs.data());
+  }
+  if (rc == MI_OK) rc = mi_plan_upload(b->plan, st);
+  if (rc == MI_OK) {
+    if (r0.kind == MI_REQ_ERI) {
+      rc = mi_plan_run(b->plan, rs[0]->out, b->plan->n_unique, st, nullptr, nullptr);
     } else {
-      rc = mi_plan_run(b.plan, b.buf, len, st, nullptr, nullptr);
-      for (size_t m = 0; rc == MI_OK && m < ids.size(); ++m) {
-        const Request& r = s->reqs[ids[m]];
-        const int64_t off = b.plan->mol_off[m], n = b.plan->mol_off[m + 1] - off;
-        if (hipMemcpyAsync(r.out, b.buf + off, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st) != hipSuccess)
-          rc = mi_set_err(s->ctx, MI_EDEVICE, "result copy failed");
-      }
+      JKArgs jk{r0.D, r0.J, r0.K};
+      rc = mi_plan_run(b->plan, nullptr, 0, st, nullptr, &jk);
     }
-  } else if (rc == MI_OK) {
-    JKArgs jk{r0.D, r0.J, r0.K};
-    rc = mi_plan_run(b.plan, nullptr, 0, st, nullptr, &jk);
   }
-  if (rc == MI_OK && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess)
-    rc = mi_set_err(s->ctx, MI_EDEVICE, "hipEventCreate failed");
-  if (rc == MI_OK && hipEventRecord(b.done, st) != hipSuccess) rc = mi_set_err(s->ctx, MI_EDEVICE, "event record");
+  if (rc == MI_OK && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
+    rc = mi_set_err(c, MI_EDEVICE, "hipEventCreate failed");
+  if (rc == MI_OK && hipEventRecord(b->done, st) != hipSuccess) rc = mi_set_err(c, MI_EDEVICE, "event record");
   if (rc != MI_OK) {
-    for (int64_t id : ids) fail(s, s->reqs[id], s->ctx->err);
-    if (b.buf) hipFreeAsync(b.buf, st);
-    if (b.plan) mi_plan_destroy(b.plan);
-    if (b.done) hipEventDestroy(b.done);
-    return rc;
+    b->why = c->err;
+    hipStreamSynchronize(st);  // nothing of this batch may still run when its plan goes
+    b->state.store(B_BROKEN);
+    return;
   }
-  b.bytes = b.plan->device_bytes + (b.buf ? sizeof(double) * (size_t)b.plan->n_unique : 0);
-  for (int64_t id : ids) s->reqs[id].state = MI_REQ_RUNNING;
-  s->in_use += b.bytes;
-  s->peak = std::max(s->peak, s->in_use);
-  s->slot_busy[slot] = true;
-  ++s->nbatches;
-  s->inflight.push_back(b);
-  return MI_OK;
+  b->state.store(B_LAUNCHED);
 }
 
 void admit(mi_server* s) {
   const size_t budget = (size_t)s->opts.hbm_budget;
   while (!s->queue.empty() && (int)s->inflight.size() < s->opts.max_inflight) {
     const int64_t head = s->queue.front();
-    Request& r = s->reqs[head];
-    if (r.bytes > budget) {  // can never fit
+    Request& r = *s->reqs[head];
+    const int sized = r.sized.load();
+    if (sized == 0) break;  // a worker is still sizing the head
+    if (sized < 0 || r.bytes > budget) {  // can never run
       s->queue.pop_front();
-      fail(s, r, "request needs more device memory than the server budget");
+      fail(s, r, sized < 0 ? r.why : "request needs more device memory than the server budget");
       continue;
     }
     if (s->in_use + r.bytes > budget) break;  // wait for memory
     int slot = 0;
     while (s->slot_busy[slot]) ++slot;
-    std::vector<int64_t> ids{head};
+    std::vector<Request*> rs{&r};
+    size_t bytes = r.bytes;
     s->queue.pop_front();
     if (r.kind == MI_REQ_ERI) {
-      size_t bytes = r.bytes;
-      while (!s->queue.empty() && (int)ids.size() < s->opts.max_batch) {
-        Request& n = s->reqs[s->queue.front()];
-        if (n.kind != MI_REQ_ERI || !same_opts(n.o, r.o) || s->in_use + bytes + n.bytes > budget) break;
+      while (!s->queue.empty() && (int)rs.size() < s->opts.max_batch) {
+        Request& n = *s->reqs[s->queue.front()];
+        if (n.sized.load() != 1 || n.kind != MI_REQ_ERI || !same_opts(n.o, r.o) || s->in_use + bytes + n.bytes > budget)
+          break;
         bytes += n.bytes;
-        ids.push_back(n.id);
+        rs.push_back(&n);
         s->queue.pop_front();
       }
     }
-    launch(s, ids, slot);
+    auto bp = std::make_unique<Batch>();
+    Batch* b = bp.get();
+    b->slot = slot;
+    b->bytes = bytes;
+    for (Request* q : rs) {
+      b->reqs.push_back(q->id);
+      q->state = MI_REQ_RUNNING;
+    }
+    s->in_use += bytes;
+    s->peak = std::max(s->peak, s->in_use);
+    s->slot_busy[slot] = true;
+    ++s->nbatches;
+    s->inflight.push_back(std::move(bp));
+    hipStream_t st = s->streams[slot];
+    s->pool.push([s, b, rs, st](mi_ctx* c) { launch_batch(s, c, b, rs, st); });
   }
 }
 
@@ -235,23 +301,44 @@ int mi_server_create(int32_t device, const mi_server_opts* opts, mi_server** out
   }
   s->streams.assign(s->opts.max_inflight, nullptr);
   s->slot_busy.assign(s->opts.max_inflight, false);
+  bool ok = true;
   for (auto& st : s->streams)
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-      for (auto& t : s->streams)
-        if (t) hipStreamDestroy(t);
-      mi_destroy(s->ctx);
-      delete s;
-      return MI_EDEVICE;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) ok = false;
+  // worker contexts (one per thread: plan building writes a context's error text)
+  for (int w = 0; ok && w < MI_SERVER_WORKERS; ++w) {
+    mi_ctx* c = nullptr;
+    if (mi_create(device, &c) != MI_OK) {
+      ok = false;
+      break;
     }
+    mi_ctx_set_streams(c, 1);
+    s->pool.ctxs.push_back(c);
+  }
+  if (!ok) {
+    for (auto& t : s->streams)
+      if (t) hipStreamDestroy(t);
+    for (mi_ctx* c : s->pool.ctxs) mi_destroy(c);
+    mi_destroy(s->ctx);
+    delete s;
+    return MI_EDEVICE;
+  }
+  for (mi_ctx* c : s->pool.ctxs) s->pool.threads.emplace_back([s, c] { s->pool.run(c); });
   *out = s;
   return MI_OK;
 }
 
 void mi_server_destroy(mi_server* s) {
   if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->pool.mu);
+    s->pool.stop = true;
+  }
+  s->pool.cv.notify_all();
+  for (auto& t : s->pool.threads) t.join();  // queued jobs finish first
   for (auto& st : s->streams) hipStreamSynchronize(st);
   retire(s);
   for (auto& st : s->streams) hipStreamDestroy(st);
+  for (mi_ctx* c : s->pool.ctxs) mi_destroy(c);
   mi_destroy(s->ctx);
   delete s;
 }
@@ -264,7 +351,8 @@ int mi_server_submit(mi_server* s, const mi_basis* basis, const mi_opts* opts, i
   if (kind != MI_REQ_ERI && kind != MI_REQ_JK) return serr(s, MI_EINVAL, "unknown request kind");
   if (kind == MI_REQ_ERI && !out) return serr(s, MI_EINVAL, "ERI request without an output buffer");
   if (kind == MI_REQ_JK && (!D || !J || !K)) return serr(s, MI_EINVAL, "J/K request without D, J or K");
-  Request r;
+  auto rp = std::make_unique<Request>();
+  Request& r = *rp;
   r.id = s->next_id++;
   r.kind = kind;
   const int ns = basis->nshell;
@@ -292,18 +380,10 @@ int mi_server_submit(mi_server* s, const mi_basis* basis, const mi_opts* opts, i
   r.out = out;
   r.J = J;
   r.K = K;
-  int rc = request_bytes(s, r);
-  if (rc) return rc;
   const int64_t rid = r.id;
-  s->reqs.emplace(rid, std::move(r));
-  Request& q = s->reqs[rid];  // the vectors moved; re-point the basis
-  q.b.shell_xyz = q.xyz.data();
-  q.b.shell_l = q.L.data();
-  q.b.shell_prim_off = q.poff.data();
-  q.b.prim_exp = q.pexp.data();
-  q.b.prim_coef = q.pcoef.data();
-  q.b.prim_norm = q.pnorm.data();
-  q.b.bf_scale = q.scale.data();
+  Request* rq = rp.get();
+  s->reqs.emplace(rid, std::move(rp));
+  s->pool.push([rq](mi_ctx* c) { size_request(c, rq); });  // the object stays put until released
   s->queue.push_back(rid);
   *id = rid;
   return MI_OK;
@@ -320,7 +400,7 @@ int mi_server_poll(mi_server* s, int64_t id, int32_t* state) {
   if (!s || !state) return MI_EINVAL;
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return serr(s, MI_EINVAL, "unknown request id");
-  *state = it->second.state;
+  *state = it->second->state;
   return MI_OK;
 }
 
@@ -328,7 +408,7 @@ int mi_server_release(mi_server* s, int64_t id) {
   if (!s) return MI_EINVAL;
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return serr(s, MI_EINVAL, "unknown request id");
-  if (it->second.state != MI_REQ_DONE && it->second.state != MI_REQ_FAILED)
+  if (it->second->state != MI_REQ_DONE && it->second->state != MI_REQ_FAILED)
     return serr(s, MI_EINVAL, "request still queued or running");
   s->reqs.erase(it);
   return MI_OK;
@@ -338,8 +418,15 @@ int mi_server_drain(mi_server* s) {
   if (!s) return MI_EINVAL;
   while (!s->queue.empty() || !s->inflight.empty()) {
     admit(s);
-    if (!s->inflight.empty()) hipEventSynchronize(s->inflight.front().done);
     retire(s);
+    bool launched = false;
+    for (auto& b : s->inflight)
+      if (b->state.load() == B_LAUNCHED) {
+        hipEventSynchronize(b->done);
+        launched = true;
+        break;
+      }
+    if (!launched) std::this_thread::sleep_for(std::chrono::microseconds(50));  // workers still preparing
   }
   return s->nfailed ? MI_EDEVICE : MI_OK;
 }
@@ -347,7 +434,7 @@ int mi_server_drain(mi_server* s) {
 int mi_server_stats(const mi_server* s, int64_t* st, int32_t n) {
   if (!s || !st) return MI_EINVAL;
   int64_t running = 0;
-  for (const Batch& b : s->inflight) running += (int64_t)b.reqs.size();
+  for (const auto& b : s->inflight) running += (int64_t)b->reqs.size();
   const int64_t v[8] = {(int64_t)s->queue.size(), running,          s->ndone, s->nfailed, s->nbatches,
                         (int64_t)s->in_use,      (int64_t)s->peak, (int64_t)s->inflight.size()};
   for (int i = 0; i < n && i < 8; ++i) st[i] = v[i];

@@ -85,24 +85,30 @@ def test_results_alone_equal_in_the_mix(ctx, max_inflight, max_batch):
 
 
 def test_admission_by_device_memory():
-    """A budget that holds one request at a time serialises the requests (all
-    complete, peak within the budget); a request beyond the budget fails
-    cleanly without blocking the queue."""
+    """Requests are admitted by the device bytes of their plans (the outputs
+    are the callers' buffers).  With a budget of 1.5 x the largest small
+    request, the small requests all complete without the engine ever holding
+    more than the budget, and a request beyond it fails cleanly without
+    blocking the queue."""
     import torch
     from mi_amd import _lib, configs
     from mi_amd.serving import Server
     rng = np.random.Generator(np.random.PCG64(3))
     wls = [configs.random_molecule(rng, 3, configs.SPLIT_HEAVY, configs.SPLIT_LIGHT) for _ in range(4)]
     big = configs.random_molecule(rng, 14, configs.SPLIT_HEAVY, configs.SPLIT_LIGHT)
-    need = max(w.nbf for w in wls)
-    budget = int(8 * (need * (need + 1) * (need * need + need + 2) // 8) * 1.5) + (4 << 20)
-    srv = Server(0, max_inflight=4, max_batch=8, hbm_budget=budget)
-    outs, ids = [], []
+    outs = []
     for w in wls + [big]:
         n = w.nbf
-        o = torch.empty(n * (n + 1) * (n * n + n + 2) // 8, dtype=torch.float64, device="cuda")
-        outs.append(o)
-        ids.append(srv.submit(w.flat(), _lib.MI_REQ_ERI, out_ptr=o.data_ptr()))
+        outs.append(torch.empty(n * (n + 1) * (n * n + n + 2) // 8, dtype=torch.float64, device="cuda"))
+    # the largest small request's bytes: the peak of a server running them one at a time
+    probe = Server(0, max_inflight=1, max_batch=1)
+    for w, o in zip(wls, outs):
+        probe.submit(w.flat(), _lib.MI_REQ_ERI, out_ptr=o.data_ptr())
+    probe.drain()
+    budget = int(1.5 * probe.stats()["peak_bytes"])
+    probe.close()
+    srv = Server(0, max_inflight=4, max_batch=8, hbm_budget=budget)
+    ids = [srv.submit(w.flat(), _lib.MI_REQ_ERI, out_ptr=o.data_ptr()) for w, o in zip(wls + [big], outs)]
     with pytest.raises(_lib.MiError):
         srv.drain()  # the big request fails: MI_EDEVICE from drain
     st = srv.stats()
@@ -144,10 +150,10 @@ def test_poisson_schedule_latency_and_mixed_error():
 
 def test_step_does_not_wait_for_a_mixed_batch():
     """A mixed, screened J/K request (a 1,000-function cluster-recipe
-    molecule, seconds of device work) is admitted by one mi_server_step call
-    that returns while the batch still runs: the engine builds served plans
-    without the synchronous bounds pass (their Schwarz pass runs inside the
-    execute, on the batch stream)."""
+    molecule, seconds of device work and a Schwarz bounds pass with a device
+    read-back at plan build): every mi_server_step call returns at once while
+    the request is sized, built and run by the engine's workers -- the
+    request is seen running, and no step call lasts more than 50 ms."""
     import time
     import torch
     from mi_amd import _lib, configs
@@ -162,12 +168,18 @@ def test_step_does_not_wait_for_a_mixed_batch():
     srv = Server(0)
     rid = srv.submit(fb, _lib.MI_REQ_JK, D_ptr=D.data_ptr(), J_ptr=J.data_ptr(), K_ptr=K.data_ptr(),
                      precision=_lib.MI_PREC_MIXED, tau=1e-8, schwarz_cut=1e-12)
-    t = time.perf_counter()
-    srv.step()
-    dt = time.perf_counter() - t
-    st = srv.poll(rid)
-    print(f"{n} functions: step returned after {dt * 1e3:.1f} ms, state {st}")
-    assert st == _lib.MI_REQ_RUNNING
-    srv.drain()
+    seen, worst, t_end = set(), 0.0, time.time() + 60
+    while time.time() < t_end:
+        t = time.perf_counter()
+        srv.step()
+        worst = max(worst, time.perf_counter() - t)
+        st = srv.poll(rid)
+        seen.add(st)
+        if st in (_lib.MI_REQ_DONE, _lib.MI_REQ_FAILED):
+            break
+        time.sleep(1e-3)
+    print(f"{n} functions: states seen {sorted(seen)}, longest step {worst * 1e3:.2f} ms")
     assert srv.poll(rid) == _lib.MI_REQ_DONE
+    assert _lib.MI_REQ_RUNNING in seen
+    assert worst < 0.05
     srv.close()
