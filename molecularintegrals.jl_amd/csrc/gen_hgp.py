@@ -149,11 +149,11 @@ def emit_rkernel(c):
     w(f"    const PrimPair kp = args.pp[Q.ppoff + v];")
     w(f"    v += dv; u += du; if (v >= nv) {{ v -= nv; ++u; }}")
     w(f"    const double zeta = bp.zeta, eta = kp.zeta;")
-    w(f"    const double inv_s = 1.0 / (zeta + eta);")
+    w(f"    const double rs = rsqrt_nr(zeta + eta), inv_s = rs * rs;")
     w(f"    const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),")
     w(f"                 PQz = (Az + bp.PAz) - (Cz + kp.PAz);")
     w(f"    const double T = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);")
-    w(f"    const double pref = bp.K * kp.K * sqrt(inv_s);")
+    w(f"    const double pref = bp.K * kp.K * rs;")
     w(f"    double F[{mmax}];")
     w(f"    boys_eval<{mmax}>(T, sboys, F);")
     if mmax > 1:

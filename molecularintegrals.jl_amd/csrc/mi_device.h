@@ -146,6 +146,21 @@ struct WClassDesc {
   int is_register;
 };
 
+// 1/sqrt(x) for a positive normal x: the hardware estimate (v_rsq_f64)
+// refined by two Newton steps y <- y + y (1/2 - (x/2) y^2), each of which at
+// least doubles the correct bits, so the result is within about one ulp.
+// Callers form 1/x as y*y (within about two ulp).  The arguments here
+// (exponent sums, Boys arguments T >= 30) are positive and normal, so this
+// replaces a correctly rounded division plus square root, whose scaling and
+// special-case handling cost about 25 instructions, with eight.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  const double h = 0.5 * x;
+  double y = __builtin_amdgcn_rsq(x);
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return y;
+}
+
 // ---------------------------------------------------------------------------
 // Boys function F_m(T), m = 0..MMAX-1 (reference: farray, src/Boys.jl:19-27).
 // T < 30: the top order from a 7-term Taylor expansion about the nearest grid
@@ -161,9 +176,9 @@ struct WClassDesc {
 // for T > m + 1/2.  The reference's T >= 30 branch (src/Boys.jl:21-26) keeps
 // only the first term and drops e^{-T}: 1e-4 relative on F_12 at T = 30 and
 // up to 5e-10 relative on f-shell integrals, so it is not reproduced.
-__device__ __forceinline__ double boys_large_T_f0(double T, double eT) {
-  const double rT = 1.0 / T;
-  return 0.88622692545275801365 * sqrt(rT) - 0.5 * eT * rT * fma(rT, fma(0.75, rT, -0.5), 1.0);
+// y = 1/sqrt(T), rT = y^2 = 1/T (rsqrt_nr below).
+__device__ __forceinline__ double boys_large_T_f0(double y, double rT, double eT) {
+  return 0.88622692545275801365 * y - 0.5 * eT * rT * fma(rT, fma(0.75, rT, -0.5), 1.0);
 }
 
 __device__ __forceinline__ double exp_small(double d) {
@@ -202,8 +217,9 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
     }
   } else {
     const double eT = exp(-T);
-    F[0] = boys_large_T_f0(T, eT);
-    const double o2T = 0.5 / T;
+    const double y = rsqrt_nr(T), rT = y * y;
+    F[0] = boys_large_T_f0(y, rT, eT);
+    const double o2T = 0.5 * rT;
 #pragma unroll
     for (int m = 1; m < MMAX; ++m) F[m] = fma((double)(2 * m - 1), F[m - 1], -eT) * o2T;
   }

@@ -119,8 +119,9 @@ __device__ __forceinline__ double boys_lane(const double* __restrict__ btab, dou
     for (int j = mtop - 1; j >= m; --j) fm = fma(t2, fm, eT) * kInvOdd[j];
   } else {
     const double eT = exp(-T2);
-    fm = boys_large_T_f0(T2, eT);
-    const double o2T = 0.5 / T2;
+    const double y = rsqrt_nr(T2), rT = y * y;
+    fm = boys_large_T_f0(y, rT, eT);
+    const double o2T = 0.5 * rT;
     for (int j = 1; j <= m; ++j) fm = fma((double)(2 * j - 1), fm, -eT) * o2T;
   }
   return fm;
@@ -215,11 +216,11 @@ __device__ __forceinline__ void w_body(const EriArgs& args, const WClassDesc& D,
       const PrimPair bp = args.pp[P.ppoff + u];
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
-      const double inv_s = 1.0 / (zeta + eta);
+      const double rs = rsqrt_nr(zeta + eta), inv_s = rs * rs;
       const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),
                    PQz = (Az + bp.PAz) - (Cz + kp.PAz);
       const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = bp.K * kp.K * sqrt(inv_s);
+      const double pref = bp.K * kp.K * rs;
       // Boys: lane m (< vt0) evaluates the top order and recurses down to m.
       if (lane < D.vt0) vrr[lane] = pref * boys_lane(btab, T2, mtop, lane);
       __syncwarp();
@@ -426,11 +427,11 @@ __device__ __forceinline__ void wg_body(const EriArgs& args, const WClassDesc& D
       const PrimPair bp = args.pp[P.ppoff + u];
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
-      const double inv_s = 1.0 / (zeta + eta);
+      const double rs = rsqrt_nr(zeta + eta), inv_s = rs * rs;
       const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),
                    PQz = (Az + bp.PAz) - (Cz + kp.PAz);
       const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = bp.K * kp.K * sqrt(inv_s);
+      const double pref = bp.K * kp.K * rs;
       if (tid < D.vt0) vrr[tid] = pref * boys_lane(btab, T2, mtop, tid);
       __syncthreads();
       const double oo2z = bp.oo2z, oo2e = kp.oo2z, oo2s = 0.5 * inv_s;
@@ -625,11 +626,11 @@ __global__ void __launch_bounds__(64) hgp_wx_kernel(const EriArgs args, const WC
       const PrimPair bp = args.pp[P.ppoff + u];
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
-      const double inv_s = 1.0 / (zeta + eta);
+      const double rs = rsqrt_nr(zeta + eta), inv_s = rs * rs;
       const double PQx = (Ax + bp.PAx) - (Cx + kp.PAx), PQy = (Ay + bp.PAy) - (Cy + kp.PAy),
                    PQz = (Az + bp.PAz) - (Cz + kp.PAz);
       const double T2 = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = bp.K * kp.K * sqrt(inv_s);
+      const double pref = bp.K * kp.K * rs;
       if (tid < D.vt0) {
         const ddv b = dd_two_sum(0.0, boys_lane(btab, T2, mtop, tid));
         dd_store(vrr, tid, dd_mul_d(b, pref));
