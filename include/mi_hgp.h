@@ -50,6 +50,12 @@ extern "C" {
 #define MI_PREC_FP64 0     /* everything in FP64                               */
 #define MI_PREC_MIXED 1    /* FP32 VRR for quartets whose VRR-output bound < tau */
 
+/* integral methods (mi_opts.method) */
+#define MI_METHOD_HGP 0    /* Head-Gordon/Pople recurrences (all_twoe_ints_chrr, src/HGP.jl:22) */
+#define MI_METHOD_RYS 1    /* Rys quadrature (all_twoe_ints_rys, src/Rys.jl:16): every
+                              quartet class through the Rys kernel, in FP64 (precision
+                              MI_PREC_MIXED screens and bounds as for HGP but keeps FP64) */
+
 typedef struct mi_ctx mi_ctx;
 typedef struct mi_plan mi_plan;
 
@@ -89,6 +95,7 @@ typedef struct {
                             cancels, so FP32 rounding stays below ~tau * 1e-7. */
     double schwarz_cut;  /* 0: compute every quartet (plain); >0: skip below   */
     void *stream;        /* hipStream_t for *_device calls; NULL = default     */
+    int32_t method;      /* MI_METHOD_* (0 = HGP)                               */
 } mi_opts;
 
 /* Context: device buffers, Boys grid and op tables for one GPU.
@@ -97,12 +104,20 @@ int mi_create(int device, mi_ctx **out);
 void mi_destroy(mi_ctx *ctx);
 const char *mi_last_error(const mi_ctx *ctx);
 int32_t mi_lmax(void);                   /* largest supported shell L      */
+/* The n-point Rys rule at T >= 0 as the Rys kernel evaluates it (host copy of
+ * its tables): roots x[r] = t_r^2 in (0,1) and weights w[r] with
+ * sum_r w_r x_r^k = F_k(T) for k < 2n.  Replaces Roots(n, X, roots, weights)
+ * (src/Rys.jl:161-174, whose roots are u = t^2/(1-t^2)).  n = 1..mi_rys_nmax(). */
+int mi_rys_rule(int32_t n, double T, double *x, double *w);
+int32_t mi_rys_nmax(void);
 int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
 
 /* Kernel selection (no counterpart in the reference).  rows: nrows x 6 int32
  * (quartet class, bra contraction bucket, ket contraction bucket, variant
  * [1 generated register kernel, 0 LDS wave kernel, 2 global-table kernel,
- * 3 LDS wave kernel reading its op tables from global memory],
+ * 3 LDS wave kernel reading its op tables from global memory, 5 Rys kernel
+ * (1, 2 or 4 wavefronts; used by MI_METHOD_RYS plans, and by HGP plans where
+ * the table names it: same integrals within rounding)],
  * G = lanes per quartet for the register kernel or wavefronts per quartet /
  * block for the wave kernels, regime [0: class launches of fewer than 65536
  * shell quartets, 1: larger ones, 2: launches of MI_ORDER_JK plans (where a
@@ -129,7 +144,10 @@ int mi_ctx_set_fusion(mi_ctx *ctx, int32_t on);
 
 /* One-shot drop-in: host basis in, host array out (caller-owned, out_len
  * must be mi_n_unique(nbf)).  Replaces all_twoe_ints_chrr (order CHRR,
- * src/HGP.jl:22) and all_twoe_ints (order IINDEX, src/ERI.jl:104). */
+ * src/HGP.jl:22) and all_twoe_ints (order IINDEX, src/ERI.jl:104); with
+ * method MI_METHOD_RYS and order IINDEX, all_twoe_ints_rys (src/Rys.jl:16-25,
+ * coulomb_rys :27-84, for every class up to (ff|ff), 7 roots; the reference's
+ * Roots stops at 5). */
 int mi_eri_unique(mi_ctx *ctx, const mi_basis *basis, const mi_opts *opts, double *out, int64_t out_len);
 
 /* mi_eri_unique for callers that cannot allocate device memory (code that
@@ -200,7 +218,8 @@ int mi_plan_nlaunch(const mi_plan *plan);
 /* cls4 = (la,lb,lc,ld); counts3 = (shell quartets, primitive quartets,
  * blocks); kernel2 = (variant, wavefronts per block): variant 1 = generated
  * register kernel (wavefronts 0), 0 = LDS wave kernel, 2 = global-table
- * kernel, 3 = LDS wave kernel with op tables in global memory.  Returns MI_OK
+ * kernel, 3 = LDS wave kernel with op tables in global memory, 4 =
+ * double-double table kernel, 5 = Rys kernel.  Returns MI_OK
  * or MI_EINVAL on a bad index. */
 int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *counts3, int32_t *kernel2);
 /* One execute with HIP events between launches on `stream`; synchronises and

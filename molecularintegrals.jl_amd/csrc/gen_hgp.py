@@ -316,6 +316,11 @@ def w_tables(c):
                 order.append(n)
     nhrr = len(acc_nodes) + len(order)
     accsrc = [base[(n[0], n[2])] | ((1 << 31) if n[0] == n[2] else 0) for n in acc_nodes]
+    # Rys kernel: Cartesian powers of e and f of each contracted (e0|f0), 3 bits each
+    rysexp = []
+    for n in acc_nodes:
+        (ex, ey, ez), (fx, fy, fz) = AO2M[n[0]], AO2M[n[2]]
+        rysexp.append(ex | ey << 3 | ez << 6 | fx << 9 | fy << 12 | fz << 15)
     hitems, hlev = [], []
     for lev in hlevels:
         hlev.append(len(hitems))
@@ -335,7 +340,7 @@ def w_tables(c):
     assert len(outs) <= nvrr  # the Coulomb/exchange path stages the block in the VRR area
     desc = dict(la=la, lb=lb, lc=lc, ld=ld, mmax=g.mmax, nvrr=nvrr, nhrr=nhrr, vt0=vt0,
                 vlev=vlev, hlev=hlev, nacc=len(accsrc), nout=len(outs))
-    return desc, vitems, accsrc, hitems, outs
+    return desc, vitems, accsrc, hitems, outs, rysexp
 
 
 # Per-primitive coefficient block of the LDS wave kernel (doubles, W_CF of
@@ -378,12 +383,13 @@ def pack16(vi, hi, ou, am):
     return words
 
 
-def pack32(vi, hi, ou, am):
+def pack32(vi, hi, ou, am, rx):
     """Global-kernel tables, 32-bit slot fields; returns (words, offsets).
       VRR  uint4 (s1, s2, s3, dir | n1 << 2 | n2 << 5 | kind << 8 | has2 << 9 | has3 << 10)
       HRR  uint4 (s1, s2, dir | ket << 2, 0)
       out  uint2 (slot, ma | mb << 4 | mc << 8 | md << 12)
-      acc  uint2 (VRR slot, diagonal flag)"""
+      acc  uint2 (VRR slot, diagonal flag)
+      rys  uint32 per contracted (e0|f0): ex | ey << 3 | ez << 6 | fx << 9 | fy << 12 | fz << 15"""
     w = []
     gv = 0
     for s1, s2, s3, dr, n1, n2, kind, h2, h3 in vi:
@@ -397,9 +403,11 @@ def pack32(vi, hi, ou, am):
     ga = len(w)
     for a_ in am:
         w += [a_ & 0x7FFFFFFF, a_ >> 31]
+    gr = len(w)
+    w += list(rx)
     while len(w) % 4:
         w.append(0)
-    return w, (gv, gh, go, ga)
+    return w, (gv, gh, go, ga, gr)
 
 
 # ---------------------------------------------------------------- driver
@@ -470,15 +478,15 @@ def main():
     # boff = -1 otherwise.  Global-kernel tables (pack32) exist for every class.
     descs, BLOB, GBLOB = [], [], []
     for c in classes:
-        d, vi, am, hi, ou = w_tables(c)
+        d, vi, am, hi, ou, rx = w_tables(c)
         if d["nvrr"] < 65536 and d["nhrr"] < 65536 and d["nacc"] <= 1024:
             words = pack16(vi, hi, ou, am)
             d["boff"], d["bwords"] = len(BLOB) // 4, len(words)
             BLOB += words
         else:
             d["boff"], d["bwords"] = -1, 0
-        words, offs = pack32(vi, hi, ou, am)
-        d["gv"], d["gh"], d["go"], d["ga"] = (len(GBLOB) + o for o in offs)
+        words, offs = pack32(vi, hi, ou, am, rx)
+        d["gv"], d["gh"], d["go"], d["ga"], d["gr"] = (len(GBLOB) + o for o in offs)
         GBLOB += words
         descs.append(d)
     maxlev = max(len(d["vlev"]) for d in descs)
@@ -496,7 +504,7 @@ def main():
                  f"{d['vt0']}, {len(d['vlev']) - 1}, {len(d['hlev']) - 1}, {d['nacc']}, {d['nout']}, "
                  f"{d['boff']}, {d['bwords']}, {d['gv']}, {d['gh']}, {d['go']}, {d['ga']}, "
                  f"{{{', '.join(map(str, vl))}}}, {{{', '.join(map(str, hl))}}}, "
-                 f"{1 if c in rcls else 0}}},  // {cname(c)}")
+                 f"{1 if c in rcls else 0}, {d['gr']}}},  // {cname(c)}")
     # host copy (launch set-up) and device copy (fused wave launches read the
     # descriptor of each segment's class)
     t.append("static const WClassDesc kWClassDesc[HGP_NCLASS] = {")

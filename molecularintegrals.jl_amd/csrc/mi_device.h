@@ -144,6 +144,7 @@ struct WClassDesc {
   int vlev[16];  // level starts (VRR items)
   int hlev[10];
   int is_register;
+  int gr;        // global tables: word offset of the Rys kernel's per-(e0|f0) Cartesian powers
 };
 
 // 1/sqrt(x) for a positive normal x: the hardware estimate (v_rsq_f64)

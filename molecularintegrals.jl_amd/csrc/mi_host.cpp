@@ -168,7 +168,8 @@ bool tuning_row_ok(const mi_ctx* c, int cls, int bb, int kb, int var, int G, int
   if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET) return false;
   if (regime < -1 || regime >= MI_NREGIME) return false;
   if (G < 1 || G > 64 || (G & (G - 1))) return false;
-  if (var < 0 || var > 3) return false;
+  if (var < 0 || var > 5 || var == 4) return false;
+  if (var == 5 && (G > 4 || !mi::ryclass_fits_nw(cls, G, c->lds_limit))) return false;
   if (var == 1 && !mi::has_register_kernel(cls)) return false;
   if (var == 0 && !mi::wclass_fits_nw(cls, G, c->lds_limit)) return false;
   if (var == 3 && !mi::wclass_fits_nw(cls, G, c->lds_limit, true)) return false;
@@ -297,6 +298,14 @@ extern "C" {
 
 int32_t mi_lmax(void) { return mi::lmax_shell(); }
 
+int mi_rys_rule(int32_t n, double T, double* x, double* w) {
+  if (n < 1 || n > mi::rys_nmax() || !(T >= 0.0) || !x || !w) return MI_EINVAL;
+  mi::rys_rule_host(n, T, x, w);
+  return MI_OK;
+}
+
+int32_t mi_rys_nmax(void) { return mi::rys_nmax(); }
+
 int64_t mi_n_unique(int32_t nbf) {
   const int64_t n = nbf;
   return n * (n + 1) * (n * n + n + 2) / 8;
@@ -394,11 +403,21 @@ void mi_destroy(mi_ctx* ctx) {
 // global-table kernel with 4 wavefronts per block.
 // regime: 0/1 by the size of the class launch; J/K plans use the J/K regime
 // (2) where it has an entry, else the size regime.
-static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, int regime, bool jk) {
+// rys: the plan's method is Rys quadrature -- every class in the Rys kernel
+// (variant 5) with 1, 2 or 4 wavefronts per block by the size of its
+// contracted block, unless the tuning table names a Rys choice for it.
+static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, int regime, bool jk, bool rys) {
   const TuneEntry* tj = jk ? &ctx->tune[tune_index(2, t.cls, bb, kb)] : nullptr;
   const TuneEntry& te = (tj && tj->variant >= 0) ? *tj : ctx->tune[tune_index(regime, t.cls, bb, kb)];
   int variant, G;
-  if (te.variant >= 0) {
+  if (rys) {
+    const int nacc = mi::wclass(t.cls).nacc;
+    variant = 5;
+    G = te.variant == 5 ? te.G : (nacc <= 256 ? 1 : (nacc <= 1024 ? 2 : 4));
+    while (G < 4 && !mi::ryclass_fits_nw(t.cls, G, ctx->lds_limit)) G <<= 1;
+    if (!mi::ryclass_fits_nw(t.cls, G, ctx->lds_limit))
+      return set_err(ctx, MI_EUNSUPPORTED, "class %d has no Rys kernel in this build", t.cls);
+  } else if (te.variant >= 0) {
     variant = te.variant;
     G = te.G;
   } else {
@@ -417,7 +436,8 @@ static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, 
   }
   if ((variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) ||
       (variant == 3 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit, true)) ||
-      (variant == 2 && !mi::wgclass_fits_nw(t.cls, G, ctx->lds_limit)))
+      (variant == 2 && !mi::wgclass_fits_nw(t.cls, G, ctx->lds_limit)) ||
+      (variant == 5 && !mi::ryclass_fits_nw(t.cls, G, ctx->lds_limit)))
     return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
   t.G = G;
   t.variant = variant;
@@ -434,10 +454,14 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
     // vk 0: register kernel (each task its own G); 1-3: wave kernel with
     // 1, 2, 4 wavefronts; 4-7: global-table kernel with 1, 2, 4, 8
     // wavefronts; 8-10: wave kernel with global op tables, 1, 2, 4 wavefronts;
-    // 11: double-double table kernel (1 wavefront)
-    for (int vk = 0; vk < 12; ++vk) {
-      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : (vk < 11 ? 3 : 4)));
-      const int nw = vk == 0 ? 0 : (vk < 4 ? (1 << (vk - 1)) : (vk < 8 ? (1 << (vk - 4)) : (vk < 11 ? (1 << (vk - 8)) : 1)));
+    // 11: double-double table kernel (1 wavefront); 12-14: Rys kernel with
+    // 1, 2, 4 wavefronts
+    for (int vk = 0; vk < 15; ++vk) {
+      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : (vk < 11 ? 3 : (vk < 12 ? 4 : 5))));
+      const int nw = vk == 0 ? 0
+                             : (vk < 4 ? (1 << (vk - 1))
+                                       : (vk < 8 ? (1 << (vk - 4))
+                                                 : (vk < 11 ? (1 << (vk - 8)) : (vk < 12 ? 1 : (1 << (vk - 12))))));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
         if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
@@ -474,6 +498,7 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
       }
       if (var == 0 || var == 3)
         blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw, var == 3));
+      if (var == 5) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::ry_kernel_blocks_per_cu(c, nw));
       // double-double kernel: a few persistent blocks (its quartets are rare,
       // and each block holds a 16-byte-per-entry work area in the scratch)
       if (var == 4) {
@@ -704,6 +729,9 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   const bool bounds_only = (flags & MI_BUILD_BOUNDS_ONLY) != 0;
   const double tau2 = (opts && opts->precision == MI_PREC_MIXED) ? opts->tau * opts->tau : 0.0;
   const double cut2 = (opts && opts->schwarz_cut > 0) ? opts->schwarz_cut * opts->schwarz_cut : 0.0;
+  const int method = opts ? opts->method : MI_METHOD_HGP;
+  if (method != MI_METHOD_HGP && method != MI_METHOD_RYS) return set_err(ctx, MI_EINVAL, "unknown method %d", method);
+  const bool rys = method == MI_METHOD_RYS;
   // Bounds first (a device pass) when the plan screens or mixes precision:
   // pairs are then ordered by bound and screened quartets never enter a task.
   std::vector<double> kQ, kV;
@@ -995,7 +1023,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         continue;
       }
       if ((rc = choose_kernel(ctx, t, bb, kb, kavgs[c][k], cls_nq[c] < MI_SMALL_LAUNCH ? 0 : 1,
-                              P->order == MI_ORDER_JK))) {
+                              P->order == MI_ORDER_JK, rys))) {
         delete P;
         return rc;
       }
@@ -1037,7 +1065,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
           t.variant = 4;
           t.G = 1;
         } else if ((rc = choose_kernel(ctx, t, bk, bk, ((double)li.Ksum / li.n) * ((double)li.Ksum / li.n), 0,
-                                       false))) {
+                                       false, rys))) {
           delete P;
           return rc;
         }
@@ -1070,12 +1098,16 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
       return at;
     };
     size_t off = 0, soff = 0;
+    auto need = [](const Launch& L) -> size_t {
+      if (L.var == 2) return mi::wg_scratch_doubles(L.cls, L.nblocks);
+      if (L.var == 4) return mi::wx_scratch_doubles(L.cls, L.nblocks);
+      if (L.var == 5) return mi::ry_scratch_doubles(L.cls, L.nw, L.nblocks);
+      return 0;
+    };
     for (Launch& L : P->launches)
-      if (L.var == 2) L.scratch_off = take(off, mi::wg_scratch_doubles(L.cls, L.nblocks));
-      else if (L.var == 4) L.scratch_off = take(off, mi::wx_scratch_doubles(L.cls, L.nblocks));
+      if (L.var == 2 || L.var == 4 || L.var == 5) L.scratch_off = take(off, need(L));
     for (Launch& L : P->slaunches)
-      if (L.var == 2) L.scratch_off = take(soff, mi::wg_scratch_doubles(L.cls, L.nblocks));
-      else if (L.var == 4) L.scratch_off = take(soff, mi::wx_scratch_doubles(L.cls, L.nblocks));
+      if (L.var == 2 || L.var == 4 || L.var == 5) L.scratch_off = take(soff, need(L));
     P->scratch_doubles = std::max(off, soff);
   }
   P->nlanes = (flags & MI_BUILD_ONE_LANE) ? 1 : std::max(1, std::min(ctx->nstreams, (int)P->launches.size()));
@@ -1173,6 +1205,9 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
   else if (L.var == 4)
     HIP_TRY(ctx, mi::launch_class_extended(L.cls, a, L.nblocks, L.nq_shell, ctx->wtables,
                                            P->d_scratch + L.scratch_off, ls));
+  else if (L.var == 5)
+    HIP_TRY(ctx, mi::launch_class_rys(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables,
+                                      P->d_scratch + L.scratch_off, ls));
   else
     HIP_TRY(ctx, mi::launch_class_wave_global(L.cls, L.nw, L.tl, a, L.nblocks, L.nq_shell, ctx->wtables,
                                               P->d_scratch + L.scratch_off, ls));

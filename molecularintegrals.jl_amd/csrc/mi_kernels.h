@@ -56,6 +56,17 @@ hipError_t launch_fused_wave(int acc, int nw, const EriArgs& args, const FusedSe
 // HRR area does not fit in LDS uses scratch + F.bt_off[s]
 hipError_t launch_fused_wave_global(int nw, const EriArgs& args, const FusedSegs& F, size_t lds, int nblocks,
                                     void* wtables, double* scratch, hipStream_t stream);
+// Rys quadrature kernel, variant 5 (mi_rys.hip): nw in {1, 2, 4} wavefronts
+// per block; scratch = ry_scratch_doubles(c, nw, nblocks) doubles (HRR areas
+// that do not fit in LDS)
+bool ryclass_fits_nw(int c, int nw, size_t lds_limit);
+size_t ry_scratch_doubles(int c, int nw, int nblocks);
+int ry_kernel_blocks_per_cu(int c, int nw);
+hipError_t launch_class_rys(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
+                            double* scratch, hipStream_t stream);
+// host evaluation of the kernels' Rys rule: n roots x (= t^2) and weights at T
+void rys_rule_host(int n, double T, double* x, double* w);
+int rys_nmax();
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream);
 hipError_t launch_symmetrize(double* J, double* K, int n, hipStream_t stream);
 

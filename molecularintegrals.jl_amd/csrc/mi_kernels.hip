@@ -984,6 +984,23 @@ hipError_t launch_class_wave_global(int c, int nw, bool tl, const EriArgs& args,
   return hipLaunchKernel(wg_kernel_fn(c, tl, a.jk_run > 0), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
+// Rys kernel (mi_rys.hip) on the class descriptors and global op tables
+bool ryclass_fits_nw(const WClassDesc& D, int nw, size_t lds_limit);
+size_t ry_scratch_doubles(const WClassDesc& D, int nw, int nblocks);
+int ry_kernel_blocks_per_cu(const WClassDesc& D, int nw);
+hipError_t launch_rys(const WClassDesc& D, int nw, const EriArgs& args, int nblocks, int64_t nq_launch,
+                      const uint32_t* gblob, double* scratch, hipStream_t stream);
+hipError_t set_rys_lds_limit(size_t bytes);
+
+bool ryclass_fits_nw(int c, int nw, size_t lds_limit) { return ryclass_fits_nw(kWClassDesc[c], nw, lds_limit); }
+size_t ry_scratch_doubles(int c, int nw, int nblocks) { return ry_scratch_doubles(kWClassDesc[c], nw, nblocks); }
+int ry_kernel_blocks_per_cu(int c, int nw) { return ry_kernel_blocks_per_cu(kWClassDesc[c], nw); }
+hipError_t launch_class_rys(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
+                            double* scratch, hipStream_t stream) {
+  const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
+  return launch_rys(kWClassDesc[c], nw, args, nblocks, nq_launch, t->gblob, scratch, stream);
+}
+
 hipError_t launch_zero(double* x, int64_t n, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
@@ -1022,7 +1039,7 @@ hipError_t set_wave_kernel_lds_limit(size_t bytes) {
     if (e != hipSuccess) return e;
   }
   g_lds_limit = bytes;
-  return hipSuccess;
+  return set_rys_lds_limit(bytes);
 }
 
 }  // namespace mi

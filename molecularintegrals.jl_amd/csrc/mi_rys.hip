@@ -1,0 +1,350 @@
+// mi_rys.hip -- Rys quadrature kernel (tuning variant 5): the two-electron
+// integrals of MolecularIntegrals.jl's Rys path (coulomb_rys,
+// src/Rys.jl:27-84; all_twoe_ints_rys, :16-25) for every quartet class.
+//
+// For a primitive quartet the integral over Cartesian Gaussians is
+//   [e|f] = pref * sum_r w_r Gx_r[ex][fx] Gy_r[ey][fy] Gz_r[ez][fz]
+// (ABD eq. 5 and 9; pref = K_ab K_cd / sqrt(zeta + eta), the prefactor the
+// HGP kernels give [0|0]^(0) / F_0), where (x_r, w_r) is the n-point Rys rule
+// at T = zeta eta |PQ|^2 / (zeta + eta), n = (L_a+L_b+L_c+L_d)/2 + 1, and the
+// 2D integrals G_r[a][c] follow the recurrence of Recur (src/Rys.jl:92-126,
+// factors RecurFactors :141-151 with the root as x = t^2 rather than
+// u = t^2 / (1 - t^2)):
+//   G[0][0] = 1, G[a+1][0] = C G[a][0] + a B10 G[a-1][0],
+//   G[a][c+1] = C' G[a][c] + c B01 G[a][c-1] + a B00 G[a-1][c],
+//   C = PA + x WP, C' = QC + x WQ, B00 = x / (2(zeta+eta)),
+//   B10 = (1 - x eta/(zeta+eta)) / (2 zeta), B01 = (1 - x zeta/(zeta+eta)) / (2 eta).
+// The kernel forms [e|f] for the e (L_a <= |e| <= L_a+L_b) and f
+// (L_c <= |f| <= L_c+L_d) of the class -- the contracted (e0|f0) block the
+// HGP kernels build by VRR -- sums it over the primitive quartets and
+// transfers it to (ab|cd) with the HGP horizontal recurrence (the same op
+// tables).  The reference transfers each primitive's 1D integrals instead
+// (Shift, src/Rys.jl:128-136, I(i,j,k,l) from I(i+j,0,k+l,0)); the transfer
+// is linear, so after the contraction it gives the same integrals with the
+// transfer done once per shell quartet.
+//
+// Rys rule: Gauss rule of e^{-T x} x^{-1/2}/2 on x in [0, 1] (moments F_k(T)),
+// from the tables of gen_rys.c: degree-11 polynomials on unit intervals of T
+// below T_max(n), the large-T limit above (rule error ~2e-16 relative).
+//
+// Shape: persistent blocks of nw wavefronts, one shell quartet at a time; the
+// primitive quartets go in batches of nbat: each (primitive, root, direction)
+// item builds its 2D table in LDS, then each thread forms the products of the
+// (e0|f0) elements it owns (ACC per thread, kept in registers across batches).
+#include <cmath>
+
+#include "mi_device.h"
+
+__device__ __forceinline__ int ry_ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+#define RYS_QUAL_SMALL static __constant__ const
+#define RYS_QUAL_BIG static __device__ const
+#define RYS_NAME(x) x##Dev
+#include "gen/rys_table.inc"
+#undef RYS_QUAL_SMALL
+#undef RYS_QUAL_BIG
+#undef RYS_NAME
+#define RYS_QUAL_SMALL static const
+#define RYS_QUAL_BIG static const
+#define RYS_NAME(x) x##Host
+#include "gen/rys_table.inc"
+#undef RYS_QUAL_SMALL
+#undef RYS_QUAL_BIG
+#undef RYS_NAME
+
+#define RY_SCL 16  // LDS stride of the per-shell scale rows
+
+namespace mi {
+
+// Root r (x = t^2) and weight of the n-point rule at T.
+__device__ __forceinline__ void rys_rule_dev(int n, int r, double T, double& x, double& w) {
+  if (T < kRysTmaxDev[n]) {
+    const int i = (int)T;  // interval [i, i+1), RYS_H = 1
+    const double y = 2.0 * (T - (double)i) - 1.0;
+    const double* c = kRysCoefDev + kRysOffDev[n] + (size_t)(i * 2 * n + r) * (RYS_DEG + 1);
+    const double* d = c + (size_t)n * (RYS_DEG + 1);
+    double px = c[RYS_DEG], pw = d[RYS_DEG];
+#pragma unroll
+    for (int k = RYS_DEG - 1; k >= 0; --k) {
+      px = fma(px, y, c[k]);
+      pw = fma(pw, y, d[k]);
+    }
+    x = px;
+    w = pw;
+  } else {
+    const double rs = rsqrt_nr(T);
+    x = kRysLimitDev[n][0][r] * (rs * rs);
+    w = kRysLimitDev[n][1][r] * rs;
+  }
+}
+
+// Host evaluation of the same tables (C ABI mi_rys_rule, CPU tests).
+void rys_rule_host(int n, double T, double* x, double* w) {
+  for (int r = 0; r < n; ++r) {
+    if (T < kRysTmaxHost[n]) {
+      const int i = (int)T;
+      const double y = 2.0 * (T - (double)i) - 1.0;
+      const double* c = kRysCoefHost + kRysOffHost[n] + (size_t)(i * 2 * n + r) * (RYS_DEG + 1);
+      const double* d = c + (size_t)n * (RYS_DEG + 1);
+      double px = c[RYS_DEG], pw = d[RYS_DEG];
+      for (int k = RYS_DEG - 1; k >= 0; --k) {
+        px = std::fma(px, y, c[k]);
+        pw = std::fma(pw, y, d[k]);
+      }
+      x[r] = px;
+      w[r] = pw;
+    } else {
+      x[r] = kRysLimitHost[n][0][r] / T;
+      w[r] = kRysLimitHost[n][1][r] / std::sqrt(T);
+    }
+  }
+}
+int rys_nmax() { return RYS_NMAX; }
+
+// LDS: [scales: 4*RY_SCL][work][hrr: nhrr if HL]
+// S = (amax+1)(cmax+1) entries per 2D table; work (>= nbat n 3 S doubles)
+// holds the batch's 2D tables, then (J/K build, work >= nout) the scaled block.
+template <int ACC, bool HL>
+__global__ void __launch_bounds__(256) hgp_ry_kernel(const EriArgs args, const WClassDesc D,
+                                                     const uint32_t* __restrict__ gblob, double* __restrict__ scratch,
+                                                     int64_t nq_launch, int nbat, int work) {
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int n = (D.mmax + 1) / 2;
+  const int amax = D.la + D.lb, cmax = D.lc + D.ld;
+  const int c1 = cmax + 1, S = (amax + 1) * c1;
+  double* scl = lds;
+  double* G = lds + 4 * RY_SCL;
+  double* hrr = HL ? G + work : scratch + (size_t)blockIdx.x * D.nhrr;
+  const uint4* gH = reinterpret_cast<const uint4*>(gblob + D.gh);
+  const uint2_t* gO = reinterpret_cast<const uint2_t*>(gblob + D.go);
+  const uint2_t* gA = reinterpret_cast<const uint2_t*>(gblob + D.ga);
+  const uint32_t* gR = gblob + D.gr;
+  const int na = ry_ncart(D.la), nb = ry_ncart(D.lb), nc = ry_ncart(D.lc), nd = ry_ncart(D.ld);
+  // offsets of the owned (e0|f0) elements in a (root) group of three 2D tables
+  int ox[ACC], oy[ACC], oz[ACC];
+#pragma unroll
+  for (int k = 0; k < ACC; ++k) {
+    const int x = tid + k * nthr;
+    const uint32_t e = x < D.nacc ? gR[x] : 0u;
+    ox[k] = (int)(e & 7) * c1 + (int)((e >> 9) & 7);
+    oy[k] = S + (int)((e >> 3) & 7) * c1 + (int)((e >> 12) & 7);
+    oz[k] = 2 * S + (int)((e >> 6) & 7) * c1 + (int)((e >> 15) & 7);
+  }
+  JkRun run{kNoRun, kNoRun, kNoRun, 0, 0, 0};
+  int ti = 0;
+  for (int64_t q = blockIdx.x; q < nq_launch; q += gridDim.x) {
+    ti = advance_task(args, ti, q);
+    const TaskDesc T = args.tasks[ti];
+    QuartetCtx qc;
+    decode_index(T, q - T.blockStart, qc);
+    qc.G = 1;
+    qc.g = 0;
+    if (quartet_mode(args, qc) == 2) continue;  // screened (uniform per block): output pre-zeroed
+    const PairTopo P = args.topo[qc.bra];
+    const PairTopo Q = args.topo[qc.ket];
+    if (tid < na) scl[tid] = args.bf_scale[P.bfA + tid];
+    if (tid < nb) scl[RY_SCL + tid] = args.bf_scale[P.bfB + tid];
+    if (tid < nc) scl[2 * RY_SCL + tid] = args.bf_scale[Q.bfA + tid];
+    if (tid < nd) scl[3 * RY_SCL + tid] = args.bf_scale[Q.bfB + tid];
+    double acc[ACC];
+#pragma unroll
+    for (int k = 0; k < ACC; ++k) acc[k] = 0.0;
+    const int K = P.npp * Q.npp;
+    for (int t0 = 0; t0 < K; t0 += nbat) {
+      const int nbc = K - t0 < nbat ? K - t0 : nbat;
+      // 2D tables of item (primitive p, root r, direction d)
+      for (int it = tid; it < nbc * n * 3; it += nthr) {
+        const int p = it / (3 * n), rd = it - p * 3 * n, r = rd / 3, d = rd - r * 3;
+        const int t = t0 + p, u = t / Q.npp, v = t - u * Q.npp;
+        const PrimPair bp = args.pp[P.ppoff + u];
+        const PrimPair kp = args.pp[Q.ppoff + v];
+        const double zeta = bp.zeta, eta = kp.zeta;
+        const double rs = rsqrt_nr(zeta + eta), inv_s = rs * rs;
+        const double* A = args.shell_xyz + 3 * P.shA;
+        const double* C = args.shell_xyz + 3 * Q.shA;
+        const double PQx = (A[0] + bp.PAx) - (C[0] + kp.PAx), PQy = (A[1] + bp.PAy) - (C[1] + kp.PAy),
+                     PQz = (A[2] + bp.PAz) - (C[2] + kp.PAz);
+        const double Targ = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);
+        double x, w;
+        rys_rule_dev(n, r, Targ, x, w);
+        const double PQd = d == 0 ? PQx : (d == 1 ? PQy : PQz);
+        const double PAd = d == 0 ? bp.PAx : (d == 1 ? bp.PAy : bp.PAz);
+        const double QCd = d == 0 ? kp.PAx : (d == 1 ? kp.PAy : kp.PAz);
+        const double eta_s = eta * inv_s, zeta_s = zeta * inv_s;
+        const double Cf = PAd - x * eta_s * PQd;   // PA + x WP
+        const double Cp = QCd + x * zeta_s * PQd;  // QC + x WQ
+        const double B00 = 0.5 * x * inv_s;
+        const double B10 = bp.oo2z * (1.0 - x * eta_s);
+        const double B01 = kp.oo2z * (1.0 - x * zeta_s);
+        double* g = G + (size_t)it * S;
+        g[0] = d == 0 ? bp.K * kp.K * rs * w : 1.0;
+        if (amax > 0) g[c1] = Cf * g[0];
+        for (int a = 1; a < amax; ++a) g[(a + 1) * c1] = Cf * g[a * c1] + (double)a * B10 * g[(a - 1) * c1];
+        for (int a = 0; a <= amax; ++a) {
+          double* ga = g + a * c1;
+          const double* gm = g + (a - 1) * c1;
+          const double aB = (double)a * B00;
+          if (cmax > 0) ga[1] = Cp * ga[0] + (a > 0 ? aB * gm[0] : 0.0);
+          for (int c = 1; c < cmax; ++c)
+            ga[c + 1] = Cp * ga[c] + (double)c * B01 * ga[c - 1] + (a > 0 ? aB * gm[c] : 0.0);
+        }
+      }
+      __syncthreads();
+      const int ng = nbc * n;
+#pragma unroll
+      for (int k = 0; k < ACC; ++k) {
+        if (tid + k * nthr < D.nacc) {
+          double s = 0.0;
+          for (int j = 0; j < ng; ++j) {
+            const double* g = G + (size_t)j * 3 * S;
+            s = fma(g[ox[k]] * g[oy[k]], g[oz[k]], s);
+          }
+          acc[k] += s;
+        }
+      }
+      __syncthreads();  // the next batch overwrites the tables
+    }
+#pragma unroll
+    for (int k = 0; k < ACC; ++k) {
+      const int x = tid + k * nthr;
+      if (x < D.nacc) {
+        hrr[x] = acc[k];
+        if (args.mode == MI_MODE_SCHWARZ && gA[x].y) record_vmax(&args.geom[qc.bra], acc[k]);
+      }
+    }
+    __syncthreads();
+    // HRR levels: item k writes slot nacc + k (src/HGP.jl:83-116)
+    const PairGeom PG = args.geom[qc.bra];
+    const PairGeom QG = args.geom[qc.ket];
+    for (int lv = 0; lv < D.nhlev; ++lv) {
+      const int i1 = D.hlev[lv + 1];
+      for (int it = D.hlev[lv] + tid; it < i1; it += nthr) {
+        const uint4 o = gH[it];
+        const int j = o.z & 3;
+        const bool ket = (o.z >> 2) & 1;
+        const double X = ket ? (j == 0 ? QG.ABx : (j == 1 ? QG.ABy : QG.ABz))
+                             : (j == 0 ? PG.ABx : (j == 1 ? PG.ABy : PG.ABz));
+        hrr[D.nacc + it] = hrr[o.x] + X * hrr[o.y];
+      }
+      __syncthreads();
+    }
+    const ScatterCtx sc = make_scatter(args, qc, P, Q);
+    if (args.mode == MI_MODE_JK) {  // uniform per block
+      for (int x = tid; x < D.nout; x += nthr) {
+        const uint2_t o = gO[x];
+        const int ma = o.y & 15, mb = (o.y >> 4) & 15, mc = (o.y >> 8) & 15, md = (o.y >> 12) & 15;
+        G[x] = hrr[o.x] * (scl[ma] * scl[RY_SCL + mb]) * (scl[2 * RY_SCL + mc] * scl[3 * RY_SCL + md]);
+      }
+      __syncthreads();
+      jk_block_lds(args, sc, G, na, nb, nc, nd, tid, nthr, nullptr, run);
+      __syncthreads();
+      continue;
+    }
+    for (int x = tid; x < D.nout; x += nthr) {
+      const uint2_t o = gO[x];
+      const int ma = o.y & 15, mb = (o.y >> 4) & 15, mc = (o.y >> 8) & 15, md = (o.y >> 12) & 15;
+      scatter_elem(args, sc, qc.bra, sc.fa0 + ma, sc.fb0 + mb, sc.fc0 + mc, sc.fd0 + md,
+                   hrr[o.x] * (scl[ma] * scl[RY_SCL + mb]) * (scl[2 * RY_SCL + mc] * scl[3 * RY_SCL + md]));
+    }
+    __syncthreads();
+  }
+}
+
+static const void* ry_kernel_fn(int acc, bool hl) {
+  switch (acc) {
+    case 1: return hl ? (const void*)hgp_ry_kernel<1, true> : (const void*)hgp_ry_kernel<1, false>;
+    case 2: return hl ? (const void*)hgp_ry_kernel<2, true> : (const void*)hgp_ry_kernel<2, false>;
+    case 4: return hl ? (const void*)hgp_ry_kernel<4, true> : (const void*)hgp_ry_kernel<4, false>;
+    case 8: return hl ? (const void*)hgp_ry_kernel<8, true> : (const void*)hgp_ry_kernel<8, false>;
+    default: return hl ? (const void*)hgp_ry_kernel<24, true> : (const void*)hgp_ry_kernel<24, false>;
+  }
+}
+
+// Accumulators per thread for nacc elements over nthr threads (0: too many).
+static int ry_acc(int nacc, int nthr) {
+  const int need = (nacc + nthr - 1) / nthr;
+  if (need <= 1) return 1;
+  if (need <= 2) return 2;
+  if (need <= 4) return 4;
+  if (need <= 8) return 8;
+  return need <= 24 ? 24 : 0;
+}
+
+static size_t g_ry_lds_limit = 65536;  // dynamic LDS a block may use (set_rys_lds_limit)
+#define RY_LDS_TARGET (64 * 1024)      // LDS per block the batch size and the HRR placement aim for
+
+// Launch geometry of class D with nw wavefronts per block: primitive quartets
+// per batch (enough (primitive, root, direction) items for about two rounds
+// of the block's threads, fewer to keep the block within RY_LDS_TARGET), the
+// work area (the batch's 2D tables; for the J/K build also the scaled block),
+// whether the HRR area fits in LDS beside it, and the dynamic LDS bytes.
+struct RyGeom {
+  int acc = 0, nbat = 1, work = 0;
+  bool hl = false;
+  size_t lds = 0;
+};
+static RyGeom ry_geom(const WClassDesc& D, int nw, bool jk) {
+  RyGeom g;
+  const int nthr = 64 * nw;
+  const int n = (D.mmax + 1) / 2;
+  const size_t S = (size_t)(D.la + D.lb + 1) * (D.lc + D.ld + 1);
+  g.acc = ry_acc(D.nacc, nthr);
+  g.nbat = std::max(1, (2 * nthr) / (3 * n));
+  auto work = [&](int nbat) { return std::max((size_t)nbat * n * 3 * S, jk ? (size_t)D.nout : (size_t)0); };
+  auto bytes = [&](int nbat, bool hl) {
+    return sizeof(double) * ((size_t)4 * RY_SCL + work(nbat) + (hl ? (size_t)D.nhrr : 0));
+  };
+  while (g.nbat > 1 && bytes(g.nbat, false) > RY_LDS_TARGET) --g.nbat;
+  g.hl = bytes(g.nbat, true) <= RY_LDS_TARGET;
+  g.work = (int)work(g.nbat);
+  g.lds = bytes(g.nbat, g.hl);
+  return g;
+}
+
+// Class D runs in the Rys kernel with nw wavefronts per block (unique array,
+// Schwarz pass and J/K build alike).
+bool ryclass_fits_nw(const WClassDesc& D, int nw, size_t lds_limit) {
+  if (nw != 1 && nw != 2 && nw != 4) return false;
+  if ((D.mmax + 1) / 2 > RYS_NMAX) return false;
+  const RyGeom g = ry_geom(D, nw, true);
+  return g.acc > 0 && g.lds <= std::min(lds_limit, g_ry_lds_limit);
+}
+
+size_t ry_scratch_doubles(const WClassDesc& D, int nw, int nblocks) {
+  // the J/K geometry keeps the HRR area in LDS only where the unique-array one does
+  return ry_geom(D, nw, false).hl && ry_geom(D, nw, true).hl ? 0 : (size_t)nblocks * D.nhrr;
+}
+
+int ry_kernel_blocks_per_cu(const WClassDesc& D, int nw) {
+  const RyGeom g = ry_geom(D, nw, false);
+  int nb = 0;
+  if (g.acc == 0 ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ry_kernel_fn(g.acc, g.hl), 64 * nw, g.lds) != hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_rys(const WClassDesc& D, int nw, const EriArgs& args, int nblocks, int64_t nq_launch,
+                      const uint32_t* gblob, double* scratch, hipStream_t stream) {
+  const RyGeom g = ry_geom(D, nw, args.mode == MI_MODE_JK);
+  if (g.acc == 0 || g.lds > g_ry_lds_limit) return hipErrorInvalidValue;
+  EriArgs a = args;
+  a.jk_run = 0;
+  WClassDesc d = D;
+  int nbat = g.nbat, work = g.work;
+  void* kargs[] = {&a, &d, const_cast<const uint32_t**>(&gblob), &scratch, &nq_launch, &nbat, &work};
+  return hipLaunchKernel(ry_kernel_fn(g.acc, g.hl), dim3(nblocks), dim3(64 * nw), kargs, g.lds, stream);
+}
+
+hipError_t set_rys_lds_limit(size_t bytes) {
+  for (int acc : {1, 2, 4, 8, 24})
+    for (bool hl : {false, true}) {
+      hipError_t e = hipFuncSetAttribute(ry_kernel_fn(acc, hl), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+      if (e != hipSuccess) return e;
+    }
+  g_ry_lds_limit = bytes;
+  return hipSuccess;
+}
+
+}  // namespace mi

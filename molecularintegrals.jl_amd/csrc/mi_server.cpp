@@ -111,7 +111,8 @@ struct Pool {
 };
 
 bool same_opts(const mi_opts& a, const mi_opts& b) {
-  return a.order == b.order && a.precision == b.precision && a.tau == b.tau && a.schwarz_cut == b.schwarz_cut;
+  return a.order == b.order && a.precision == b.precision && a.tau == b.tau && a.schwarz_cut == b.schwarz_cut &&
+         a.method == b.method;
 }
 
 }  // namespace

@@ -9,6 +9,8 @@ Drop-in GPU versions of MolecularIntegrals.jl's ERI drivers:
   `iiterator`, src/Basis.jl:203);
 * `all_twoe_ints_gpu(bfs)` -- replaces `all_twoe_ints(bfs)` (src/ERI.jl:104-112),
   `iindex` order;
+* `all_twoe_ints_rys_gpu(bfs)` -- replaces `all_twoe_ints_rys(bfs)`
+  (src/Rys.jl:16-25): Rys quadrature, `iindex` order;
 * `jk_gpu(bfs, D)` -- direct Coulomb/exchange (make2JmK's J and K, src/ERI.jl:114-128).
 
 Values are the normalised contracted `coulomb(bfs[i],bfs[j],bfs[k],bfs[l])`.
@@ -18,7 +20,7 @@ module MolecularIntegralsAMD
 
 using MolecularIntegrals
 
-export all_twoe_ints_chrr_gpu, all_twoe_ints_gpu, jk_gpu
+export all_twoe_ints_chrr_gpu, all_twoe_ints_gpu, all_twoe_ints_rys_gpu, jk_gpu
 
 const libmi = get(ENV, "MI_HGP_LIB", joinpath(@__DIR__, "..", "libmi_hgp.so"))
 
@@ -45,7 +47,10 @@ struct MiOpts
     tau::Float64
     schwarz_cut::Float64
     stream::Ptr{Cvoid}
+    method::Int32
 end
+const MI_METHOD_HGP = Int32(0)
+const MI_METHOD_RYS = Int32(1)
 
 mutable struct Ctx
     h::Ptr{Cvoid}
@@ -85,7 +90,7 @@ function flatten(bfs::MolecularIntegrals.Basis)
     return xyz, L, off, ex, co, pn, scale
 end
 
-function _eri(bfs, order)
+function _eri(bfs, order, method=MI_METHOD_HGP)
     c = context()
     xyz, L, off, ex, co, pn, scale = flatten(bfs)
     n = length(bfs.cgbfs)
@@ -93,7 +98,7 @@ function _eri(bfs, order)
     GC.@preserve xyz L off ex co pn scale out begin
         b = Ref(MiBasis(length(L), pointer(xyz), pointer(L), pointer(off), pointer(ex), pointer(co),
                         pointer(pn), n, pointer(scale)))
-        o = Ref(MiOpts(order, 0, 0.0, 0.0, C_NULL))
+        o = Ref(MiOpts(order, 0, 0.0, 0.0, C_NULL, method))
         rc = ccall((:mi_eri_unique, libmi), Cint, (Ptr{Cvoid}, Ref{MiBasis}, Ref{MiOpts}, Ptr{Float64}, Int64),
                    c.h, b, o, out, length(out))
         check(c, rc)
@@ -103,6 +108,8 @@ end
 
 all_twoe_ints_chrr_gpu(bfs) = _eri(bfs, MI_ORDER_CHRR)
 all_twoe_ints_gpu(bfs) = _eri(bfs, MI_ORDER_IINDEX)
+# all_twoe_ints_rys (src/Rys.jl:16-25): Rys quadrature on the GPU, iindex order
+all_twoe_ints_rys_gpu(bfs) = _eri(bfs, MI_ORDER_IINDEX, MI_METHOD_RYS)
 
 function jk_gpu(bfs, D::Matrix{Float64}; schwarz_cut=0.0)
     c = context()
@@ -114,7 +121,7 @@ function jk_gpu(bfs, D::Matrix{Float64}; schwarz_cut=0.0)
     GC.@preserve xyz L off ex co pn scale Dr J K begin
         b = Ref(MiBasis(length(L), pointer(xyz), pointer(L), pointer(off), pointer(ex), pointer(co),
                         pointer(pn), n, pointer(scale)))
-        o = Ref(MiOpts(MI_ORDER_IINDEX, 0, 0.0, schwarz_cut, C_NULL))
+        o = Ref(MiOpts(MI_ORDER_IINDEX, 0, 0.0, schwarz_cut, C_NULL, MI_METHOD_HGP))
         rc = ccall((:mi_jk, libmi), Cint,
                    (Ptr{Cvoid}, Ref{MiBasis}, Ref{MiOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
                    c.h, b, o, Dr, J, K)

@@ -16,9 +16,11 @@ LIB_PATH = os.environ.get("MI_HGP_LIB", os.path.join(PKG_ROOT, "libmi_hgp.so"))
 MI_OK, MI_EINVAL, MI_EUNSUPPORTED, MI_EDEVICE, MI_ENOMEM, MI_ESIZE = range(6)
 MI_ORDER_IINDEX, MI_ORDER_CHRR, MI_ORDER_JK = 0, 1, 2  # MI_ORDER_JK: plan for J/K builds only
 MI_PREC_FP64, MI_PREC_MIXED = 0, 1
+MI_METHOD_HGP, MI_METHOD_RYS = 0, 1
 
 # Every symbol include/mi_hgp.h declares (checked by the CPU test suite).
 EXPORTED = ["mi_create", "mi_destroy", "mi_last_error", "mi_lmax", "mi_n_unique", "mi_eri_unique",
+            "mi_rys_rule", "mi_rys_nmax",
             "mi_eri_unique_noalloc",
             "mi_plan_create", "mi_plan_execute", "mi_plan_destroy", "mi_plan_stats", "mi_plan_nlaunch",
             "mi_plan_launch_info", "mi_plan_profile", "mi_plan_jk_profile", "mi_plan_balance", "mi_ctx_set_tuning", "mi_ctx_set_streams",
@@ -49,7 +51,8 @@ class MiOpts(ctypes.Structure):
                 ("precision", ctypes.c_int32),
                 ("tau", ctypes.c_double),
                 ("schwarz_cut", ctypes.c_double),
-                ("stream", ctypes.c_void_p)]
+                ("stream", ctypes.c_void_p),
+                ("method", ctypes.c_int32)]
 
 
 class MiServerOpts(ctypes.Structure):
@@ -95,6 +98,11 @@ def load():
     L.mi_lmax.restype = ctypes.c_int32
     L.mi_n_unique.argtypes = [ctypes.c_int32]
     L.mi_n_unique.restype = ctypes.c_int64
+    L.mi_rys_rule.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_double)]
+    L.mi_rys_rule.restype = ctypes.c_int
+    L.mi_rys_nmax.argtypes = []
+    L.mi_rys_nmax.restype = ctypes.c_int32
     L.mi_eri_unique.argtypes = [vp, ctypes.POINTER(MiBasis), ctypes.POINTER(MiOpts),
                                 ctypes.POINTER(ctypes.c_double), ctypes.c_int64]
     L.mi_eri_unique.restype = ctypes.c_int
@@ -186,5 +194,19 @@ class BasisArgs:
                               dp(arrs[5]), fb.nbf, dp(arrs[6]))
 
 
-def opts(order=MI_ORDER_CHRR, precision=MI_PREC_FP64, tau=0.0, schwarz_cut=0.0, stream=None):
-    return MiOpts(order, precision, tau, schwarz_cut, stream)
+def opts(order=MI_ORDER_CHRR, precision=MI_PREC_FP64, tau=0.0, schwarz_cut=0.0, stream=None,
+         method=MI_METHOD_HGP):
+    return MiOpts(order, precision, tau, schwarz_cut, stream, method)
+
+
+def rys_rule(n: int, T: float):
+    """The n-point Rys rule at T as the Rys kernel evaluates it (host copy of
+    its tables): roots x = t^2 and weights (mi_rys_rule)."""
+    L = load()
+    x = np.empty(n)
+    w = np.empty(n)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.mi_rys_rule(int(n), float(T), x.ctypes.data_as(dp), w.ctypes.data_as(dp))
+    if rc != MI_OK:
+        raise MiError(rc, f"mi_rys_rule(n={n}, T={T})")
+    return x, w

@@ -49,16 +49,19 @@ class Context:
             raise _lib.MiError(rc, self.L.mi_last_error(self.h).decode())
 
     def eri_unique(self, fb: FlatBasis, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
-                   schwarz_cut=0.0) -> np.ndarray:
+                   schwarz_cut=0.0, method=_lib.MI_METHOD_HGP) -> np.ndarray:
+        """All unique ERIs (host array); method MI_METHOD_RYS computes them by
+        Rys quadrature (all_twoe_ints_rys, src/Rys.jl:16-25)."""
         args = _lib.BasisArgs(fb)
         n = int(self.L.mi_n_unique(fb.nbf))
         out = np.empty(n, dtype=np.float64)
-        o = _lib.opts(order, precision, tau, schwarz_cut)
+        o = _lib.opts(order, precision, tau, schwarz_cut, method=method)
         self.check(self.L.mi_eri_unique(self.h, ctypes.byref(args.struct), ctypes.byref(o),
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
         return out
 
-    def jk(self, fb: FlatBasis, D, precision=_lib.MI_PREC_FP64, tau=0.0, schwarz_cut=0.0):
+    def jk(self, fb: FlatBasis, D, precision=_lib.MI_PREC_FP64, tau=0.0, schwarz_cut=0.0,
+           method=_lib.MI_METHOD_HGP):
         """Direct Coulomb and exchange (host arrays): J_ij = sum (ij|kl) D_kl,
         K_ij = sum (ik|jl) D_kl; make2JmK's G is 2J - K (src/ERI.jl:114-128)."""
         args = _lib.BasisArgs(fb)
@@ -66,7 +69,7 @@ class Context:
         D = np.ascontiguousarray(D, dtype=np.float64).reshape(n, n)
         J = np.empty((n, n))
         K = np.empty((n, n))
-        o = _lib.opts(_lib.MI_ORDER_JK, precision, tau, schwarz_cut)
+        o = _lib.opts(_lib.MI_ORDER_JK, precision, tau, schwarz_cut, method=method)
         dp = ctypes.POINTER(ctypes.c_double)
         self.check(self.L.mi_jk(self.h, ctypes.byref(args.struct), ctypes.byref(o), D.ctypes.data_as(dp),
                                 J.ctypes.data_as(dp), K.ctypes.data_as(dp)))
@@ -88,13 +91,13 @@ class Context:
         self.check(self.L.mi_ctx_set_fusion(self.h, 1 if on else 0))
 
     def plan(self, fb: FlatBasis, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
-             schwarz_cut=0.0) -> "Plan":
-        return Plan(self, [fb], order, precision, tau, schwarz_cut)
+             schwarz_cut=0.0, method=_lib.MI_METHOD_HGP) -> "Plan":
+        return Plan(self, [fb], order, precision, tau, schwarz_cut, method)
 
     def batch_plan(self, fbs, order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0,
-                   schwarz_cut=0.0) -> "Plan":
+                   schwarz_cut=0.0, method=_lib.MI_METHOD_HGP) -> "Plan":
         """One plan over many molecules (outputs back to back, see Plan.mol_offsets)."""
-        return Plan(self, list(fbs), order, precision, tau, schwarz_cut)
+        return Plan(self, list(fbs), order, precision, tau, schwarz_cut, method)
 
 
 class Plan:
@@ -102,13 +105,13 @@ class Plan:
     writes all unique ERIs into a caller-owned device buffer (e.g. a torch
     tensor's ``data_ptr()``) on a given HIP stream."""
 
-    def __init__(self, ctx: Context, fbs, order, precision, tau, schwarz_cut):
+    def __init__(self, ctx: Context, fbs, order, precision, tau, schwarz_cut, method=_lib.MI_METHOD_HGP):
         self.ctx = ctx
         self.fbs = fbs
         self.fb = fbs[0]
         self._args = [_lib.BasisArgs(fb) for fb in fbs]
         arr = (_lib.MiBasis * len(fbs))(*[a.struct for a in self._args])
-        o = _lib.opts(order, precision, tau, schwarz_cut)
+        o = _lib.opts(order, precision, tau, schwarz_cut, method=method)
         h = ctypes.c_void_p()
         if len(fbs) == 1:
             ctx.check(ctx.L.mi_plan_create(ctx.h, arr, ctypes.byref(o), ctypes.byref(h)))
@@ -161,7 +164,8 @@ class Plan:
         blocks, kernel), kernel = "R" (generated register kernel), "W<nw>" (LDS
         wave kernel, nw wavefronts per quartet), "G<nw>" (global-table kernel,
         nw wavefronts per block), "T<nw>" (wave kernel, global op tables) or
-        "X4" (double-double table kernel)."""
+        "X4" (double-double table kernel) or "Y<nw>" (Rys kernel, nw wavefronts
+        per block)."""
         out = []
         n = self.ctx.L.mi_plan_nlaunch(self.h)
         for i in range(n):
@@ -169,7 +173,7 @@ class Plan:
             n3 = (ctypes.c_int64 * 3)()
             k2 = (ctypes.c_int32 * 2)()
             self.ctx.check(self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3, k2))
-            kern = "R" if k2[0] == 1 else {0: "W", 2: "G", 3: "T", 4: "X"}[k2[0]] + str(k2[1])
+            kern = "R" if k2[0] == 1 else {0: "W", 2: "G", 3: "T", 4: "X", 5: "Y"}[k2[0]] + str(k2[1])
             out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), kern))
         return out
 

@@ -13,8 +13,9 @@ Names and argument meaning follow the reference: ``Atom(atno, xyz)`` with xyz
 in bohr (:258-297), ``Basis(atoms, name)`` over the named basis sets
 (:300-343, build_basis src/Basis.jl:122-139), ``MolecularIntegrals()`` with
 ``all_twoe_ints`` and ``all_twoe_ints_4d`` (:442-482).  Unknown methods raise
-ValueError as there; 'rys' (Rys quadrature, src/Rys.jl) is not built and
-raises NotImplementedError.
+ValueError as there; 'rys' runs Rys quadrature on the GPU (all_twoe_ints_rys,
+src/Rys.jl:16-25; C ABI method MI_METHOD_RYS), the same integrals by the
+other algorithm.
 """
 from __future__ import annotations
 
@@ -120,11 +121,9 @@ class MolecularIntegrals:
         (python/molecular_integrals.py:409-440; all_twoe_ints, src/ERI.jl:104-112)."""
         if method not in self._METHODS:
             raise ValueError(f"Unknown method: {method}")
-        if method == "rys":
-            raise NotImplementedError("method 'rys' (Rys quadrature, src/Rys.jl) is not built; "
-                                      "'default' and 'hgp' compute the same integrals on the GPU")
         b = basis._basis if isinstance(basis, Basis) else basis
-        return self._ctx().eri_unique(_flat(b), order=_lib.MI_ORDER_IINDEX)
+        m = _lib.MI_METHOD_RYS if method == "rys" else _lib.MI_METHOD_HGP
+        return self._ctx().eri_unique(_flat(b), order=_lib.MI_ORDER_IINDEX, method=m)
 
     def all_twoe_ints_4d(self, basis, method: str = "default") -> np.ndarray:
         """All (ij|kl) as an (n, n, n, n) array (python/molecular_integrals.py:442-482)."""

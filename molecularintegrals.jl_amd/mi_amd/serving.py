@@ -38,9 +38,10 @@ class Server:
             raise _lib.MiError(rc, self.L.mi_server_last_error(self.h).decode())
 
     def submit(self, fb, kind: int, out_ptr: int = 0, D_ptr: int = 0, J_ptr: int = 0, K_ptr: int = 0,
-               order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0, schwarz_cut=0.0) -> int:
+               order=_lib.MI_ORDER_CHRR, precision=_lib.MI_PREC_FP64, tau=0.0, schwarz_cut=0.0,
+               method=_lib.MI_METHOD_HGP) -> int:
         args = _lib.BasisArgs(fb)
-        o = _lib.opts(order, precision, tau, schwarz_cut)
+        o = _lib.opts(order, precision, tau, schwarz_cut, method=method)
         rid = ctypes.c_int64()
         vp = ctypes.c_void_p
         self.check(self.L.mi_server_submit(self.h, ctypes.byref(args.struct), ctypes.byref(o), int(kind),

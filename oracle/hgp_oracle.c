@@ -719,6 +719,224 @@ int oracle_eri_unique_tho(int nshell, const double *shell_xyz, const int32_t *sh
 }
 
 /* ------------------------------------------------------------------------ */
+/* Rys quadrature -- src/Rys.jl.                                              */
+/* ------------------------------------------------------------------------ */
+/* The n-point Rys rule at X (Roots(n, X, roots, weights), src/Rys.jl:161-174).
+ * The reference evaluates fitted polynomials (Root1..Root5, :176-1480); here
+ * the rule is computed from its definition, the Gauss rule of the weight
+ * e^{-X x} x^{-1/2}/2 on x = t^2 in [0, 1], whose moments are F_k(X): in
+ * binary128, moments by the accurate Boys function, the three-term
+ * recurrence by Chebyshev's algorithm, then the Golub-Welsch eigenproblem of
+ * the Jacobi matrix by cyclic Jacobi rotations (nodes = eigenvalues, weights
+ * = F_0(X) (first eigenvector component)^2).  Roots are returned as the
+ * reference's u = x / (1 - x) (RecurFactors, :141-151, takes u). */
+#define ORACLE_RYS_NMAX 10
+static void rys_rule_q(int n, __float128 X, __float128 *x, __float128 *w) {
+    __float128 mu[2 * ORACLE_RYS_NMAX];
+    farray_q(2 * n, X, mu, 0); /* F_0 .. F_{2n-1}, accurate for every X */
+    __float128 alpha[ORACLE_RYS_NMAX], beta[ORACLE_RYS_NMAX];
+    __float128 sm[2 * ORACLE_RYS_NMAX], s0[2 * ORACLE_RYS_NMAX], s1[2 * ORACLE_RYS_NMAX];
+    for (int l = 0; l < 2 * n; ++l) { sm[l] = 0; s0[l] = mu[l]; }
+    alpha[0] = mu[1] / mu[0];
+    beta[0] = mu[0];
+    for (int k = 1; k < n; ++k) {
+        for (int l = k; l < 2 * n - k; ++l) s1[l] = s0[l + 1] - alpha[k - 1] * s0[l] - beta[k - 1] * sm[l];
+        alpha[k] = s1[k + 1] / s1[k] - s0[k] / s0[k - 1];
+        beta[k] = s1[k] / s0[k - 1];
+        for (int l = 0; l < 2 * n; ++l) { sm[l] = s0[l]; s0[l] = s1[l]; }
+    }
+    /* symmetric Jacobi matrix J, eigenvectors V (columns) by cyclic Jacobi */
+    __float128 A[ORACLE_RYS_NMAX][ORACLE_RYS_NMAX], V[ORACLE_RYS_NMAX][ORACLE_RYS_NMAX];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            A[i][j] = i == j ? alpha[i] : (j == i + 1 ? sqrtq(beta[j]) : (i == j + 1 ? sqrtq(beta[i]) : 0));
+            V[i][j] = i == j;
+        }
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        __float128 off = 0, diag = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                if (i != j) off += A[i][j] * A[i][j];
+                else diag += A[i][j] * A[i][j];
+            }
+        if (off <= 1e-66Q * diag) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                if (A[p][q] == 0) continue;
+                const __float128 th = (A[q][q] - A[p][p]) / (2 * A[p][q]);
+                const __float128 t = (th >= 0 ? 1 : -1) / (fabsq(th) + sqrtq(th * th + 1));
+                const __float128 c = 1 / sqrtq(t * t + 1), sn = t * c;
+                for (int k = 0; k < n; ++k) { /* A <- A R (columns p, q) */
+                    const __float128 akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - sn * akq;
+                    A[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) { /* A <- R^T A (rows p, q) */
+                    const __float128 apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - sn * aqk;
+                    A[q][k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const __float128 vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - sn * vkq;
+                    V[k][q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    /* ascending nodes */
+    int idx[ORACLE_RYS_NMAX];
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (A[idx[j]][idx[j]] < A[idx[i]][idx[i]]) { int t = idx[i]; idx[i] = idx[j]; idx[j] = t; }
+    for (int r = 0; r < n; ++r) {
+        x[r] = A[idx[r]][idx[r]];
+        w[r] = mu[0] * V[0][idx[r]] * V[0][idx[r]];
+    }
+}
+
+int oracle_rys_roots(int n, double X, double *roots, double *weights) {
+    if (n < 1 || n > ORACLE_RYS_NMAX || !(X >= 0)) return 1;
+    __float128 x[ORACLE_RYS_NMAX], w[ORACLE_RYS_NMAX];
+    rys_rule_q(n, (__float128)X, x, w);
+    for (int r = 0; r < n; ++r) {
+        roots[r] = (double)(x[r] / (1 - x[r]));
+        weights[r] = (double)w[r];
+    }
+    return 0;
+}
+
+/* RecurFactors (src/Rys.jl:141-151), t = u. */
+static void rys_recur_factors(double t, double A, double B, double Px, double Qx, double xi, double xk,
+                              double *C, double *Cp, double *B0, double *B1, double *B1p) {
+    const double ooopt = 1 / (1 + t);
+    const double fact = t * ooopt / (A + B);
+    *B0 = 0.5 * fact;
+    *B1 = 0.5 * ooopt / A + 0.5 * fact;
+    *B1p = 0.5 * ooopt / B + 0.5 * fact;
+    *C = (Px - xi) * ooopt + (B * (Qx - xi) + A * (Px - xi)) * fact;
+    *Cp = (Qx - xk) * ooopt + (B * (Qx - xk) + A * (Px - xk)) * fact;
+}
+
+#define RYS_GDIM 13 /* i+j, k+l <= 12 */
+/* Recur (src/Rys.jl:92-126): G[a][b] = I(a,0,b,0) for a <= i+j, b <= k+l. */
+static void rys_recur(double G[RYS_GDIM][RYS_GDIM], double t, int i, int j, int k, int l, double xi, double xj,
+                      double xk, double xl, double ai, double aj, double ak, double al) {
+    const int n = i + j, m = k + l;
+    const double A = ai + aj, B = ak + al;
+    const double Px = (ai * xi + aj * xj) / A, Qx = (ak * xk + al * xl) / B;
+    double C, Cp, B0, B1, B1p;
+    rys_recur_factors(t, A, B, Px, Qx, xi, xk, &C, &Cp, &B0, &B1, &B1p);
+    G[0][0] = M_PI * exp(-ai * aj * (xi - xj) * (xi - xj) / (ai + aj) - ak * al * (xk - xl) * (xk - xl) / (ak + al)) /
+              sqrt(A * B);
+    if (n > 0) G[1][0] = C * G[0][0];
+    if (m > 0) G[0][1] = Cp * G[0][0];
+    for (int a = 2; a <= n; ++a) G[a][0] = B1 * (a - 1) * G[a - 2][0] + C * G[a - 1][0];
+    for (int b = 2; b <= m; ++b) G[0][b] = B1p * (b - 1) * G[0][b - 2] + Cp * G[0][b - 1];
+    if (m == 0 || n == 0) return;
+    for (int a = 1; a <= n; ++a) {
+        G[a][1] = a * B0 * G[a - 1][0] + Cp * G[a][0];
+        for (int b = 2; b <= m; ++b) G[a][b] = B1p * (b - 1) * G[a][b - 2] + a * B0 * G[a - 1][b - 1] + Cp * G[a][b - 1];
+    }
+}
+
+/* Shift (src/Rys.jl:128-136): I(i,j,k,l) from G = I(i+j,0,k+l,0). */
+static double rys_shift(double G[RYS_GDIM][RYS_GDIM], int i, int j, int k, int l, double xij, double xkl) {
+    double ijkl = 0;
+    for (int m = 0; m <= l; ++m) {
+        double ijm0 = 0;
+        for (int n = 0; n <= j; ++n) ijm0 += binomial(j, n) * ipow(xij, j - n) * G[n + i][m + k];
+        ijkl += binomial(l, m) * ipow(xkl, l - m) * ijm0;
+    }
+    return ijkl;
+}
+
+/* coulomb_rys(a::PGBF, b, c, d) (src/Rys.jl:59-84) without the primitive
+ * norms (the caller applies them, as for oracle_coulomb_prim). */
+double oracle_coulomb_rys_prim(double aexpn, const double *A, int aI, int aJ, int aK, double bexpn, const double *B,
+                               int bI, int bJ, int bK, double cexpn, const double *C, int cI, int cJ, int cK,
+                               double dexpn, const double *D, int dI, int dJ, int dK) {
+    const int norder = (aI + aJ + aK + bI + bJ + bK + cI + cJ + cK + dI + dJ + dK) / 2 + 1;
+    const double Ae = aexpn + bexpn, Be = cexpn + dexpn;
+    const double rho = Ae * Be / (Ae + Be);
+    double P[3], Q[3], rpq2 = 0;
+    for (int x = 0; x < 3; ++x) {
+        P[x] = (aexpn * A[x] + bexpn * B[x]) / Ae;
+        Q[x] = (cexpn * C[x] + dexpn * D[x]) / Be;
+        rpq2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+    }
+    const double X = rpq2 * rho;
+    double roots[ORACLE_RYS_NMAX], weights[ORACLE_RYS_NMAX];
+    if (norder > ORACLE_RYS_NMAX || oracle_rys_roots(norder, X, roots, weights)) return NAN;
+    double G[RYS_GDIM][RYS_GDIM];
+    double ijkl = 0;
+    const int Ia[3] = {aI, aJ, aK}, Ib[3] = {bI, bJ, bK}, Ic[3] = {cI, cJ, cK}, Id[3] = {dI, dJ, dK};
+    for (int r = 0; r < norder; ++r) {
+        double I1[3];
+        for (int x = 0; x < 3; ++x) { /* Int1d (:86-90) */
+            rys_recur(G, roots[r], Ia[x], Ib[x], Ic[x], Id[x], A[x], B[x], C[x], D[x], aexpn, bexpn, cexpn, dexpn);
+            I1[x] = rys_shift(G, Ia[x], Ib[x], Ic[x], Id[x], A[x] - B[x], C[x] - D[x]);
+        }
+        ijkl += I1[0] * I1[1] * I1[2] * weights[r];
+    }
+    return 2 * sqrt(rho / M_PI) * ijkl;
+}
+
+/* all_twoe_ints_rys (src/Rys.jl:16-25): every unique quartet (iiterator,
+ * Utils.jl:17) by coulomb_rys on CGBFs (:27-44), contraction and norms as in
+ * oracle_eri_unique_tho. */
+int oracle_eri_unique_rys(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                          const int32_t *shell_prim_off, const double *prim_exp, const double *prim_coef,
+                          const double *prim_norm, int nbf, const double *bf_scale, int order, int nthreads,
+                          double *out) {
+    flat_basis fb = {nshell, nbf, shell_xyz, shell_l, shell_prim_off, prim_exp, prim_coef, prim_norm, bf_scale, NULL};
+    fb.bfoff = (int *)malloc(sizeof(int) * (nshell + 1));
+    int rc = check_basis(&fb);
+    if (rc) { free(fb.bfoff); return rc; }
+    int *bsh = (int *)malloc(sizeof(int) * nbf), *bm = (int *)malloc(sizeof(int) * nbf);
+    for (int s = 0; s < nshell; ++s)
+        for (int m = 0; m < ncart(shell_l[s]); ++m) {
+            bsh[fb.bfoff[s] + m] = s;
+            bm[fb.bfoff[s] + m] = oracle_nao(shell_l[s] - 1) + m;
+        }
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 1; i <= nbf; ++i)
+        for (int64_t j = 1; j <= i; ++j)
+            for (int64_t k = 1; k <= nbf; ++k)
+                for (int64_t l = 1; l <= k; ++l) {
+                    if (tri1(i, j) > tri1(k, l)) continue;
+                    int64_t f[4] = {i - 1, j - 1, k - 1, l - 1};
+                    int sh[4], I[4][3];
+                    for (int t = 0; t < 4; ++t) {
+                        sh[t] = bsh[f[t]];
+                        int32_t m[3];
+                        oracle_ao2m(bm[f[t]], m);
+                        I[t][0] = m[0]; I[t][1] = m[1]; I[t][2] = m[2];
+                    }
+                    double s = 0.0;
+                    for (int pa = shell_prim_off[sh[0]]; pa < shell_prim_off[sh[0] + 1]; ++pa)
+                        for (int pb = shell_prim_off[sh[1]]; pb < shell_prim_off[sh[1] + 1]; ++pb)
+                            for (int pc = shell_prim_off[sh[2]]; pc < shell_prim_off[sh[2] + 1]; ++pc)
+                                for (int pd = shell_prim_off[sh[3]]; pd < shell_prim_off[sh[3] + 1]; ++pd) {
+                                    double w = prim_coef[pa] * prim_coef[pb] * prim_coef[pc] * prim_coef[pd] *
+                                               prim_norm[pa] * prim_norm[pb] * prim_norm[pc] * prim_norm[pd];
+                                    s += w * oracle_coulomb_rys_prim(
+                                                 prim_exp[pa], shell_xyz + 3 * sh[0], I[0][0], I[0][1], I[0][2],
+                                                 prim_exp[pb], shell_xyz + 3 * sh[1], I[1][0], I[1][1], I[1][2],
+                                                 prim_exp[pc], shell_xyz + 3 * sh[2], I[2][0], I[2][1], I[2][2],
+                                                 prim_exp[pd], shell_xyz + 3 * sh[3], I[3][0], I[3][1], I[3][2]);
+                                }
+                    s *= bf_scale[f[0]] * bf_scale[f[1]] * bf_scale[f[2]] * bf_scale[f[3]];
+                    out[oracle_slot(nbf, i, j, k, l, order)] = s;
+                }
+    free(bsh);
+    free(bm);
+    free(fb.bfoff);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* J/K from the unique array -- index algebra of make2JmK, src/ERI.jl:114-128. */
 /* ------------------------------------------------------------------------ */
 void oracle_jk_from_unique(int nbf, const double *eri, const double *D, double *J, double *K) {

@@ -141,6 +141,23 @@ int oracle_eri_unique_tho(int nshell, const double *shell_xyz, const int32_t *sh
                           const double *prim_coef, const double *prim_norm,
                           int nbf, const double *bf_scale, int order, double *out);
 
+/* Rys quadrature (src/Rys.jl).  oracle_rys_roots: the n-point Rys rule at X
+ * (n <= 10) from its definition (binary128: moments F_k(X), Chebyshev's
+ * algorithm, Golub-Welsch by Jacobi rotations), roots as the reference's
+ * u = t^2/(1-t^2) (Roots, :161-174; the reference fits them for n <= 5).
+ * oracle_coulomb_rys_prim: coulomb_rys on primitives without norms
+ * (:59-84, Recur :92-126, Shift :128-136, RecurFactors :141-151), FP64.
+ * oracle_eri_unique_rys: all_twoe_ints_rys (:16-25) over a flattened basis. */
+int oracle_rys_roots(int n, double X, double *roots, double *weights);
+double oracle_coulomb_rys_prim(double aexpn, const double *A, int aI, int aJ, int aK,
+                               double bexpn, const double *B, int bI, int bJ, int bK,
+                               double cexpn, const double *C, int cI, int cJ, int cK,
+                               double dexpn, const double *D, int dI, int dJ, int dK);
+int oracle_eri_unique_rys(int nshell, const double *shell_xyz, const int32_t *shell_l,
+                          const int32_t *shell_prim_off, const double *prim_exp,
+                          const double *prim_coef, const double *prim_norm,
+                          int nbf, const double *bf_scale, int order, int nthreads, double *out);
+
 /* Coulomb and exchange from an iindex-ordered unique array, following the
  * index algebra of make2JmK (src/ERI.jl:114-128): J_ij = sum_kl D_kl (ij|kl),
  * K_ij = sum_kl D_kl (ik|jl).  D, J, K are n x n row-major. */

@@ -69,6 +69,10 @@ def lib():
         L.oracle_eri_hgp_quartet_sample.argtypes = basis_args + [ctypes.c_int64, _ip, ctypes.c_int, _dp]
         L.oracle_jk_from_unique.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.oracle_schwarz_pairs.argtypes = basis_args + [_dp]
+        L.oracle_rys_roots.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.oracle_coulomb_rys_prim.restype = ctypes.c_double
+        L.oracle_coulomb_rys_prim.argtypes = ([ctypes.c_double, _dp] + [ctypes.c_int] * 3) * 4
+        L.oracle_eri_unique_rys.argtypes = basis_args + [ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -222,6 +226,35 @@ def eri_unique_tho(fb, order=ORDER_IINDEX):
     rc = lib().oracle_eri_unique_tho(*args, int(order), out.ctypes.data_as(_dp))
     if rc:
         raise ValueError(f"oracle_eri_unique_tho rejected the basis (code {rc})")
+    return out
+
+
+def rys_roots(n, X):
+    """Roots(n, X) (Rys.jl:161-174): the n-point Rys rule from its definition
+    (binary128 Golub-Welsch on the Boys moments); roots as u = t^2/(1-t^2)."""
+    r = np.zeros(n)
+    w = np.zeros(n)
+    if lib().oracle_rys_roots(int(n), float(X), r.ctypes.data_as(_dp), w.ctypes.data_as(_dp)):
+        raise ValueError(f"oracle_rys_roots(n={n}, X={X})")
+    return r, w
+
+
+def coulomb_rys_prim(a, A, aIJK, b, B, bIJK, c, C, cIJK, d, D, dIJK):
+    """coulomb_rys on primitives (Rys.jl:59-84), without the primitive norms."""
+    keep = [_xyz(x) for x in (A, B, C, D)]
+    args = []
+    for e, k, ijk in zip((a, b, c, d), keep, (aIJK, bIJK, cIJK, dIJK)):
+        args += [float(e), k[1], *[int(x) for x in ijk]]
+    return lib().oracle_coulomb_rys_prim(*args)
+
+
+def eri_unique_rys(fb, order=ORDER_IINDEX, nthreads=0):
+    """all_twoe_ints_rys (Rys.jl:16-25): every unique quartet by Rys quadrature."""
+    keep, args = _basis_args(fb)
+    out = np.zeros(n_unique(fb.nbf))
+    rc = lib().oracle_eri_unique_rys(*args, int(order), int(nthreads), out.ctypes.data_as(_dp))
+    if rc:
+        raise ValueError(f"oracle_eri_unique_rys rejected the basis (code {rc})")
     return out
 
 

@@ -192,8 +192,7 @@ def test_python_api_4d_expansion_matches_the_reference_loop():
 def test_python_api_names_and_errors():
     """Atom / Basis / MolecularIntegrals of python/molecular_integrals.py:
     Atom xyz in bohr, Basis(atoms, name) through build_basis (names matched
-    without case), unknown methods ValueError, 'rys' not built -- all before
-    any device call."""
+    without case), unknown methods ValueError -- all before any device call."""
     from mi_amd.molecular_integrals import Atom, Basis, MolecularIntegrals
     atoms = [Atom(1, (0.0, 0.0, 0.0)), Atom(1, (1.4, 0.0, 0.0))]
     assert atoms[1].xyz == (1.4, 0.0, 0.0) and atoms[0].atno == 1
@@ -205,5 +204,39 @@ def test_python_api_names_and_errors():
     mi = MolecularIntegrals()
     with pytest.raises(ValueError):
         mi.all_twoe_ints(b, method="bogus")
-    with pytest.raises(NotImplementedError):
-        mi.all_twoe_ints(b, method="rys")
+
+
+def test_rys_kernel_rule_is_the_gauss_rule():
+    """The Rys kernel's rule (host copy of its tables, mi_rys_rule) is the
+    Gauss rule of e^{-T x} x^{-1/2}/2 on [0, 1]: sum_r w_r x_r^k = F_k(T) for
+    k < 2n (the moments determine the n-point rule uniquely), checked against
+    the oracle's binary128 Boys function on table intervals, interval edges
+    and the large-T limit; and it agrees root by root with the oracle's rule
+    (computed by a different method: Golub-Welsch with Jacobi rotations)."""
+    from mi_amd import _lib
+    from oracle import oracle as O
+    L = _lib.load()
+    nmax = int(L.mi_rys_nmax())
+    assert nmax >= 5  # the reference's Roots covers n <= 5 (src/Rys.jl:161-174)
+    rng = np.random.default_rng(5)
+    Ts = [0.0, 1e-12, 1e-3, 0.5, 1.0, 2.0 - 1e-13, 2.0, 9.99, 29.5, 30.0, 41.0, 47.7, 55.2, 68.0, 79.9, 80.0, 120.0,
+          1e3, 1e6] + list(rng.uniform(0, 90, 40))
+    worst = 0.0
+    for n in range(1, nmax + 1):
+        for T in Ts:
+            x, w = _lib.rys_rule(n, T)
+            assert np.all((x > 0) & (x < 1)) and np.all(w > 0) and np.all(np.diff(x) > 0), (n, T)
+            for k in range(2 * n):
+                Fk = O.Fm_quad(k, T, accurate=True)
+                rel = abs(float(np.sum(w * x ** k)) - Fk) / Fk
+                worst = max(worst, rel)
+                assert rel < 2e-14, (n, T, k, rel)
+            u, wo = O.rys_roots(n, T)
+            xo = u / (1 + u)
+            assert np.allclose(x, xo, rtol=5e-15, atol=0), (n, T, x, xo)
+            assert np.allclose(w, wo, rtol=5e-15, atol=0), (n, T, w, wo)
+    print(f"worst moment residual {worst:.2e}")
+    with pytest.raises(_lib.MiError):
+        _lib.rys_rule(nmax + 1, 1.0)
+    with pytest.raises(_lib.MiError):
+        _lib.rys_rule(1, -1.0)

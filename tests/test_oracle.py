@@ -22,6 +22,13 @@ What pins each oracle function (a plausible mistake must fail one of these):
   enumeration (test_indexing.py).
 * make2JmK index algebra (ERI.jl:114-128) -- brute force over the full 4-index
   tensor expanded from the unique array (test_jk_from_unique_brute_force).
+* Rys rule (Roots, Rys.jl:161-174)  -- the Gauss property sum_r w_r x_r^k =
+  F_k(X), k < 2n, against 40-digit Boys values (it fixes the n-point rule);
+  the reference fits Root1..Root5, whose own last digits stay unpinned here.
+* coulomb_rys / all_twoe_ints_rys (Rys.jl:16-151) -- the Rys KATs of
+  runtests.jl:130-148, and agreement with the exact HGP values (binary128) and
+  THO on random contracted bases up to f (a wrong Recur/Shift term, sign or
+  index fails them).
 """
 import math
 
@@ -376,3 +383,55 @@ def test_extended_oracle_arbitrates_cancellation():
     d = np.abs(e64 - ext)
     off = ~((d <= 1e-12) | (d <= 1e-10 * np.abs(ext)))
     assert off.sum() <= 4
+
+
+# ------------------------------------------------------------------ Rys (src/Rys.jl)
+def test_rys_rule_is_the_gauss_rule_of_the_boys_moments():
+    """oracle_rys_roots (Roots, Rys.jl:161-174): sum_r w_r x_r^k = F_k(X) for
+    k < 2n with x = u/(1+u), against the 40-digit Boys function; nodes in (0,1)."""
+    for n in range(1, 11):
+        for X in [0.0, 1e-9, 0.3, 5.0, 29.0, 31.0, 60.0, 150.0]:
+            u, w = O.rys_roots(n, X)
+            x = u / (1 + u)
+            assert np.all((x > 0) & (x < 1)) and np.all(np.diff(x) > 0)
+            for k in range(2 * n):
+                Fk = float(boys_mp(k, X))
+                assert float(np.sum(w * x ** k)) == pytest.approx(Fk, rel=5e-14 if n <= 7 else 1e-12), (n, X, k)
+
+
+def test_rys_reference_kats():
+    """test/runtests.jl:130-148 through the Rys restatement (all_twoe_ints_rys
+    on normalised primitives): (ss|ss) at one centre, (ss|px px), the
+    symmetry zero (ss|s px) == 0, and the distance scan."""
+    from mi_amd import flat_from_shells, iindex
+    for r, val in [(0.0, 1.1283791670951362), (1.0, 0.8427007900292186), (2.0, 0.49766113257563993),
+                   (3.0, 0.33332596983445223), (4.0, 0.2499999961456855), (5.0, 1 / 5), (6.0, 1 / 6),
+                   (7.0, 1 / 7), (8.0, 1 / 8), (9.0, 1 / 9)]:
+        fb = flat_from_shells([((0, 0, 0), 0, [1.0], [1.0]), ((0, 0, r), 0, [1.0], [1.0])])
+        e = O.eri_unique_rys(fb, order=O.ORDER_IINDEX)
+        assert e[iindex(1, 1, 2, 2) - 1] == pytest.approx(val, rel=1e-8)
+        assert e[0] == pytest.approx(1.128379167, rel=1e-9)  # coulomb_rys(c,c,c,c)
+    fb = flat_from_shells([((0, 0, 0), 0, [1.0], [1.0]), ((0, 0, 0), 1, [1.0], [1.0])])
+    e = O.eri_unique_rys(fb, order=O.ORDER_IINDEX)
+    assert e[iindex(1, 1, 2, 2) - 1] == pytest.approx(0.9403159725793302, rel=1e-12)
+    assert e[iindex(1, 1, 1, 2) - 1] == 0.0
+
+
+@pytest.mark.parametrize("lmax,seed", [(2, 3), (3, 4)])
+def test_rys_equals_hgp_and_tho(lmax, seed):
+    """Rys quadrature and HGP recurrences are different algorithms for the
+    same integrals: the Rys restatement (FP64) against the exact value of the
+    HGP algorithm (binary128, accurate Boys) on random contracted bases up to
+    f, and against THO (runtests.jl:60 compares all_twoe_ints with
+    all_twoe_ints_rys at rtol 1e-7)."""
+    from mi_amd import flat_from_shells
+    rng = np.random.default_rng(seed)
+    fb = flat_from_shells(random_spec(rng, 5, lmax=lmax, kmax=2))
+    rys = O.eri_unique_rys(fb, order=O.ORDER_CHRR)
+    ex = O.eri_unique_hgp_quad(fb, order=O.ORDER_CHRR, accurate=True)
+    d = np.abs(rys - ex)
+    assert np.all((d <= 1e-12) | (d <= 1e-10 * np.abs(ex))), float(d.max())
+    assert float(d.max()) <= 1e-13 * max(1.0, float(np.abs(ex).max()))
+    if lmax <= 2:
+        tho = O.eri_unique_tho(fb, order=O.ORDER_CHRR)
+        assert np.allclose(rys, tho, rtol=1e-10, atol=1e-12)
