@@ -30,8 +30,10 @@ The same line carries:
   cpu_baseline   the CPU oracle (C restatement of the reference HGP algorithm,
                  OpenMP over shell quartets) on a fixed leading subset of the
                  same molecules, extrapolated as a rate
-  legs           ethane_split_valence_60bf (the 60-function plain-FP64 step,
-                 the round-1 headline), jk_cluster_40atoms (direct J/K, 920
+  legs           ethane_split_valence_60bf (the 60-function plain-FP64 step),
+                 reference_timing_workloads (ethane STO-3G / 6-31G / cc-pVDZ,
+                 HGP and Rys quadrature, beside the reference's published
+                 single-thread times), jk_cluster_40atoms (direct J/K, 920
                  functions with f shells, with a 100,100-quartet ERI spot check
                  against the exact values), serving (Poisson requests of 2-60
                  atoms through the continuous-batching server, p50/p99) and
@@ -159,7 +161,7 @@ def plan_flops(plan):
 def symbol_of(launch):
     cname = "".join("spdf"[x] for x in launch[0])
     return {"R": "hgp_r_" + cname, "W": "hgp_w_kernel", "T": "hgp_w_kernel", "G": "hgp_wg_kernel",
-            "X": "hgp_wx_kernel"}[launch[4][0]]
+            "X": "hgp_wx_kernel", "Y": "hgp_ry_kernel"}[launch[4][0]]
 
 
 def roofline_from_profile(plan, ms, step_ms, n_unique):
@@ -354,6 +356,60 @@ def leg_60bf(args):
     log(f"[bench] 60-function step: {ms * 1e3:.1f} us, {n / (ms * 1e-3):.3e} unique ERIs/s; dominant {name} "
         f"{t * 1e3:.1f} us")
     return r
+
+
+# The reference's single-thread timings of its timing workloads (README.md:46-52,
+# test/timing.jl:81-89: ethane in three named basis sets), seconds per call.
+REFERENCE_TIMINGS_S = {"sto3g": {"readme_size": 16, "hgp": 0.041, "rys": 0.415},
+                       "6-31G": {"readme_size": 30, "hgp": 0.202, "rys": 1.550},
+                       "cc-pvdz": {"readme_size": 58, "hgp": 2.451, "rys": 25.94}}
+
+
+def leg_reference_workloads():
+    """Ethane in the reference's named basis sets through build_basis
+    (test/timing.jl:81-89), HGP (all_twoe_ints_chrr) and Rys quadrature
+    (all_twoe_ints_rys, iindex order) on the GPU: execute time with the basis
+    resident (plan built once) and the one-shot call with host arrays in and
+    out (mi_eri_unique, as the reference's call returns a host Vector), beside
+    the reference's published single-thread times.  Errors: Rys against HGP on
+    the GPU (different algorithms) and HGP against the FP64 oracle."""
+    import torch
+    from mi_amd import Context, build_basis, flatten
+    from mi_amd.molecules import ethane
+    from oracle import oracle as O
+
+    ctx = Context(0)
+    st = torch.cuda.current_stream()
+    sp = st.cuda_stream
+    res = {}
+    for name, ref in REFERENCE_TIMINGS_S.items():
+        fb = flatten(build_basis(ethane, name))
+        r = {"nbf": fb.nbf, "unique_eris": fb.nbf * (fb.nbf + 1) * (fb.nbf ** 2 + fb.nbf + 2) // 8,
+             "reference_single_thread_s": {"hgp": ref["hgp"], "rys": ref["rys"]}}
+        outs = {}
+        for meth, order, m in (("hgp", 1, 0), ("rys", 0, 1)):
+            plan = ctx.plan(fb, order=order, method=m)
+            out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+            ms = timed(lambda: plan.execute(out.data_ptr(), sp), 50, st)
+            outs[meth] = out.cpu().numpy()
+            plan.close()
+            t = time.perf_counter()
+            for _ in range(5):
+                ctx.eri_unique(fb, order=order, method=m)
+            one = (time.perf_counter() - t) / 5
+            r[meth] = {"execute_ms": ms, "one_shot_ms": one * 1e3, "value": r["unique_eris"] / (ms * 1e-3),
+                       "speedup_one_shot_vs_reference": ref[meth] / one}
+        hgp_iindex = ctx.eri_unique(fb, order=0)
+        r["max_abs_diff_rys_vs_hgp"] = float(np.abs(outs["rys"] - hgp_iindex).max())
+        r["max_abs_err_hgp_vs_fp64_oracle"] = float(
+            np.abs(outs["hgp"] - O.eri_unique_hgp(fb, order=1, nthreads=omp_threads())).max())
+        res[name] = r
+        log(f"[bench] ethane/{name} ({fb.nbf} functions): HGP {r['hgp']['execute_ms']:.3f} ms execute, "
+            f"{r['hgp']['one_shot_ms']:.2f} ms one-shot (reference {ref['hgp']} s); Rys "
+            f"{r['rys']['execute_ms']:.3f} ms execute, {r['rys']['one_shot_ms']:.2f} ms one-shot "
+            f"(reference {ref['rys']} s); Rys vs HGP max {r['max_abs_diff_rys_vs_hgp']:.1e}")
+    ctx.close()
+    return res
 
 
 def eri_spot_check(ctx, fb, nsets, seed=0):
@@ -617,6 +673,7 @@ def main():
         import torch
         legs = {}
         for name, fn in (("ethane_split_valence_60bf", lambda: leg_60bf(args)),
+                         ("reference_timing_workloads", leg_reference_workloads),
                          ("jk_cluster_40atoms", leg_jk),
                          ("serving", lambda: leg_serving(args.serve_requests, args.serve_rate,
                                                          args.serve_plain_requests)),

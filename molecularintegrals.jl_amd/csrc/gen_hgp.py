@@ -142,8 +142,10 @@ def emit_rkernel(c):
     w(f"  const double Cx = args.shell_xyz[3 * Q.shA], Cy = args.shell_xyz[3 * Q.shA + 1], "
       f"Cz = args.shell_xyz[3 * Q.shA + 2];")
     w(f"  const int G = qc.G;")
-    w(f"  int u = qc.g / Q.npp, v = qc.g - (qc.g / Q.npp) * Q.npp;")
-    w(f"  const int du = G / Q.npp, dv = G - du * Q.npp;")
+    w(f"  const double rnv = rcp_int(Q.npp);")
+    w(f"  int v, dv;")
+    w(f"  int u = udiv_rcp(qc.g, Q.npp, rnv, v);")
+    w(f"  const int du = udiv_rcp(G, Q.npp, rnv, dv);")
     w(f"  for (int t = qc.g; t < ntot; t += G) {{")
     w(f"    const PrimPair bp = args.pp[P.ppoff + u];")
     w(f"    const PrimPair kp = args.pp[Q.ppoff + v];")

@@ -60,6 +60,7 @@ struct TaskDesc {
   int64_t nprim;       // primitive quartets in the task
   int64_t obase;       // output offset of the task's molecule
   int64_t M;           // nbf(nbf+1)/2 of the task's molecule
+  double rnKet;        // 1 / nKet (the rectangle decode divides by nKet)
 };
 
 // What a launch does with each computed block element.
@@ -286,21 +287,49 @@ __device__ __forceinline__ int advance_task(const EriArgs& a, int ti, int64_t q)
   return lo;
 }
 
-__device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi, QuartetCtx& qc) {
+// floor(a / b) for 0 <= a < 2^31 and 1 <= b < 2^31, given rb ~ 1/b (double,
+// relative error below 2^-40): the product is within one of the quotient, so
+// one correction each way makes it exact; r = a - b floor(a / b).  About ten
+// instructions instead of an integer division's twenty-five or more.
+__device__ __forceinline__ int udiv_rcp(int a, int b, double rb, int& r) {
+  int q = (int)((double)a * rb);
+  r = a - q * b;
+  if (r < 0) {
+    --q;
+    r += b;
+  }
+  if (r >= b) {
+    ++q;
+    r -= b;
+  }
+  return q;
+}
+// 1/b for 1 <= b < 2^31: the hardware estimate plus one Newton step.
+__device__ __forceinline__ double rcp_int(int b) {
+  const double x = (double)b;
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+
+// qi: a quartet index within the task (tasks and launches hold fewer than
+// 2^31 quartets, which the host checks).
+__device__ __forceinline__ void decode_index(const TaskDesc& T, int64_t qi64, QuartetCtx& qc) {
+  const int qi = (int)qi64;
   if (T.same == 2) {  // Schwarz pass: the diagonal quartets (p|p)
-    qc.bra = qc.ket = T.braStart + (int)qi;
+    qc.bra = qc.ket = T.braStart + qi;
   } else if (T.same) {
-    int64_t q = (int64_t)((sqrt(8.0 * (double)qi + 1.0) - 1.0) * 0.5);
-    while (q * (q + 1) / 2 > qi) --q;
-    while ((q + 1) * (q + 2) / 2 <= qi) ++q;
-    const int64_t p = qi - q * (q + 1) / 2;
-    qc.bra = T.braStart + (int)p;
-    qc.ket = T.braStart + (int)q;
+    // largest q with q(q+1)/2 <= qi: the float square root is within one of it
+    int q = (int)((__builtin_sqrtf(8.0f * (float)qi + 1.0f) - 1.0f) * 0.5f);
+    while ((int64_t)q * (q + 1) / 2 > qi) --q;
+    while ((int64_t)(q + 1) * (q + 2) / 2 <= qi) ++q;
+    const int p = qi - (int)((int64_t)q * (q + 1) / 2);
+    qc.bra = T.braStart + p;
+    qc.ket = T.braStart + q;
   } else {
-    const int64_t p = qi / T.nKet;
-    const int64_t q = qi - p * T.nKet;
-    qc.bra = T.braStart + (int)p;
-    qc.ket = T.ketStart + (int)q;
+    int q;
+    const int p = udiv_rcp(qi, T.nKet, T.rnKet, q);
+    qc.bra = T.braStart + p;
+    qc.ket = T.ketStart + q;
   }
   qc.same_pair = (qc.bra == qc.ket);
   qc.bf0 = T.bf0;
@@ -321,8 +350,8 @@ __device__ __forceinline__ bool decode_quartet(const EriArgs& a, QuartetCtx& qc,
   // the task of the block's first wavefront (host table, else a block-uniform search)
   const int t0 = a.block_task ? a.block_task[bid] : find_task(a, w0);
   const TaskDesc T = a.tasks[advance_task(a, t0, w)];
-  const int G = T.G;
-  const int64_t qi = ((w - T.blockStart) * 64 + lane) / G;
+  const int G = T.G;  // a power of two
+  const int64_t qi = (int64_t)((((uint32_t)(w - T.blockStart)) << 6 | (uint32_t)lane) >> __builtin_ctz((unsigned)G));
   qc.G = G;
   qc.g = lane & (G - 1);
   // a lane past the task's last quartet gets the task's first quartet (valid

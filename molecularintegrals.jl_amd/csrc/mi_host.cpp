@@ -480,6 +480,7 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
         // register kernel: first wavefront of the task (a task takes
         // ceil(nq G / 64) whole wavefronts); wave kernels: first quartet
         t.blockStart = (int32_t)(reg ? waves : qs);
+        t.rnKet = 1.0 / (double)std::max(1, t.nKet);
         waves += reg ? (t.nq * t.G + 63) / 64 : 0;
         qs += t.nq;
         all_tasks.push_back(t);

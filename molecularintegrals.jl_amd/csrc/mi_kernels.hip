@@ -211,8 +211,10 @@ __device__ __forceinline__ void w_body(const EriArgs& args, const WClassDesc& D,
 #pragma unroll
     for (int k = 0; k < ACC; ++k) acc[k] = 0.0;
     const int K = P.npp * Q.npp;
+    const double rnv = rcp_int(Q.npp);
     for (int t = wv; t < K; t += nw) {
-      const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
+      int v;
+      const int u = udiv_rcp(t, Q.npp, rnv, v);
       const PrimPair bp = args.pp[P.ppoff + u];
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
@@ -422,8 +424,7 @@ __device__ __forceinline__ void wg_body(const EriArgs& args, const WClassDesc& D
     if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
     if (tid < nd) scl[3 * W_SCL + tid] = args.bf_scale[Q.bfB + tid];
     const int K = P.npp * Q.npp;
-    for (int t = 0; t < K; ++t) {
-      const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
+    for (int t = 0, u = 0, v = 0; t < K; ++t, v = v + 1 == Q.npp ? 0 : v + 1, u += v == 0 ? 1 : 0) {
       const PrimPair bp = args.pp[P.ppoff + u];
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;
@@ -621,8 +622,7 @@ __global__ void __launch_bounds__(64) hgp_wx_kernel(const EriArgs args, const WC
     if (tid < nc) scl[2 * W_SCL + tid] = args.bf_scale[Q.bfA + tid];
     if (tid < nd) scl[3 * W_SCL + tid] = args.bf_scale[Q.bfB + tid];
     const int K = P.npp * Q.npp;
-    for (int t = 0; t < K; ++t) {
-      const int u = t / Q.npp, v = t - (t / Q.npp) * Q.npp;
+    for (int t = 0, u = 0, v = 0; t < K; ++t, v = v + 1 == Q.npp ? 0 : v + 1, u += v == 0 ? 1 : 0) {
       const PrimPair bp = args.pp[P.ppoff + u];
       const PrimPair kp = args.pp[Q.ppoff + v];
       const double zeta = bp.zeta, eta = kp.zeta;

@@ -151,12 +151,14 @@ __global__ void __launch_bounds__(256) hgp_ry_kernel(const EriArgs args, const W
 #pragma unroll
     for (int k = 0; k < ACC; ++k) acc[k] = 0.0;
     const int K = P.npp * Q.npp;
+    const double rnv = rcp_int(Q.npp);
     for (int t0 = 0; t0 < K; t0 += nbat) {
       const int nbc = K - t0 < nbat ? K - t0 : nbat;
       // 2D tables of item (primitive p, root r, direction d)
       for (int it = tid; it < nbc * n * 3; it += nthr) {
         const int p = it / (3 * n), rd = it - p * 3 * n, r = rd / 3, d = rd - r * 3;
-        const int t = t0 + p, u = t / Q.npp, v = t - u * Q.npp;
+        int v;
+        const int u = udiv_rcp(t0 + p, Q.npp, rnv, v);
         const PrimPair bp = args.pp[P.ppoff + u];
         const PrimPair kp = args.pp[Q.ppoff + v];
         const double zeta = bp.zeta, eta = kp.zeta;
