@@ -142,6 +142,11 @@ def emit_rkernel(c):
     w(f"  const double Cx = args.shell_xyz[3 * Q.shA], Cy = args.shell_xyz[3 * Q.shA + 1], "
       f"Cz = args.shell_xyz[3 * Q.shA + 2];")
     w(f"  const int G = qc.G;")
+    if mmax > 1:
+        # FP32 VRR only where every live quartet of the wavefront qualifies:
+        # the two VRR paths of a mixed wavefront would run one after the other
+        # (non-packed FP32 issues at the FP64 rate, so FP32 saves nothing there)
+        w(f"  const bool use32 = __all(qmode == 1 || !live);")
     w(f"  const double rnv = rcp_int(Q.npp);")
     w(f"  int v, dv;")
     w(f"  int u = udiv_rcp(qc.g, Q.npp, rnv, v);")
@@ -159,7 +164,7 @@ def emit_rkernel(c):
     w(f"    double F[{mmax}];")
     w(f"    boys_eval<{mmax}>(T, sboys, F);")
     if mmax > 1:
-        w(f"    if (qmode == 1) {{  // FP32 VRR (Boys and prefactor stay FP64)")
+        w(f"    if (use32) {{  // FP32 VRR (Boys and prefactor stay FP64)")
         d, bdy, acc = vrr_block(g, "float", "      ")
         L.extend(d + bdy + acc)
         w(f"    }} else {{")

@@ -197,6 +197,20 @@ __device__ __forceinline__ double exp_small(double d) {
   return r;
 }
 
+// e^{-T} for T >= 30 where it enters F_0..F_{MMAX-1} only as a correction:
+// relative to the term it corrects (F_0's erfc tail, the upward recursion's
+// (2m-1) F_{m-1}), e^{-T} weighs at most 1.4e-10 for orders up to 3 (at T =
+// 30, less above), so the single-precision hardware exponential (relative
+// error ~2e-6 with the rounding of T) changes F_m by at most ~3e-16 relative
+// -- at FP64 rounding -- for about three instructions instead of the
+// double-precision exp's twenty-five.  Higher orders weigh e^{-T} up to 7e-5
+// (m = 12) and take the double-precision exp.
+template <int MMAX>
+__device__ __forceinline__ double exp_minus_large_T(double T) {
+  if (MMAX <= 4) return (double)__expf(-(float)T);
+  return exp(-T);
+}
+
 template <int MMAX>
 __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ tab, double (&F)[MMAX]) {
   if (T < BOYS_TMAX) {
@@ -218,7 +232,7 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
       for (int m = MMAX - 2; m >= 0; --m) F[m] = fma(t2, F[m + 1], eT) * (1.0 / (2 * m + 1));
     }
   } else {
-    const double eT = exp(-T);
+    const double eT = exp_minus_large_T<MMAX>(T);
     const double y = rsqrt_nr(T), rT = y * y;
     F[0] = boys_large_T_f0(y, rT, eT);
     const double o2T = 0.5 * rT;

@@ -41,7 +41,9 @@ NBUCKET = 5
 BUCKET_K = [1, 3, 7, 20, 48]          # representative pair contraction per bucket
 TAUS = [1, 2, 4, 8, 16, 32, 64, 1 << 30]
 # kernel choices: (0, nw) LDS wave kernel, (2, nw) global-table kernel, (3, nw)
-# LDS wave kernel with global op tables, (1, tau) register kernel
+# LDS wave kernel with global op tables, (1, tau) register kernel, (5, nw) Rys
+# quadrature kernel (the same integrals by Rys quadrature; HGP transfer)
+RYS_CANDS = [(5, 1), (5, 2), (5, 4)]
 CANDS = [(0, 1), (0, 2), (0, 4), (2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 2), (3, 4)] + [(1, t) for t in TAUS]
 
 
@@ -59,9 +61,12 @@ def fallback(c, w_ok, g_ok):
 
 
 def available(c, var, tau, has_reg, w_ok, g_ok, t_ok=None):
-    """Whether class c has kernel choice (var, tau)."""
+    """Whether class c has kernel choice (var, tau); t_ok carries the Rys
+    kernel's wavefront counts under key 5."""
     if var == 1:
         return has_reg[c]
+    if var == 5:
+        return t_ok is not None and t_ok[c].get((5, int(tau)), False)
     ok = {0: w_ok, 2: g_ok, 3: t_ok}.get(var)
     return ok is not None and ok[c].get(int(tau), False)
 
@@ -212,6 +217,12 @@ def probe_variants(ctx, nclass):
                     ok[nw] = True
                 except Exception:
                     ok[nw] = False
+        for nw in (1, 2, 4):  # Rys kernel
+            try:
+                ctx.set_tuning([(k, 0, 0, 5, nw)])
+                t_ok[k][(5, nw)] = True
+            except Exception:
+                t_ok[k][(5, nw)] = False
     return has_reg, w_ok, g_ok, t_ok
 
 
@@ -264,7 +275,8 @@ def step_time(ctx, wl, reps, n=40):
     return float(np.median(ts))
 
 
-def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015, verbose=True, checkpoint=None):
+def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015, verbose=True, checkpoint=None,
+                  cands=None, min_l=0):
     """Coordinate descent on one tuning regime over whole-execute times.
     Per-class launch times, measured one class at a time, miss that classes
     share the chip (fused small launches; stream lanes), and that the
@@ -273,11 +285,13 @@ def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015,
     (heaviest first) tries every kernel choice and keeps the one that lowers
     the sum over workloads of execute time / starting time by more than
     ``gain``.  workloads: (FlatBasis or list of them, plan options).  Returns
-    the updated rows (other regimes unchanged)."""
+    the updated rows (other regimes unchanged).  cands: the kernel choices
+    tried (default CANDS); min_l: only classes of total angular momentum at
+    least min_l."""
     classes = quartet_classes()
     ctx = Context(0)
     has_reg, w_ok, g_ok, t_ok = probe_variants(ctx, len(classes))
-    cands = CANDS
+    cands = CANDS if cands is None else cands
 
     def valid(k, cand):
         return available(k, cand[0], cand[1], has_reg, w_ok, g_ok, t_ok)
@@ -312,12 +326,12 @@ def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015,
             prof = plan.profile(out.data_ptr(), sp)
         for i, (cls, nqs, _, _, _) in enumerate(plan.launches()):
             # regime 2: every launch of a J/K plan; else small / large launches
-            if regime == 2 or (nqs < 65536) == (regime == 0):
+            if (regime == 2 or (nqs < 65536) == (regime == 0)) and sum(cls) >= min_l:
                 k = classes.index(cls)
                 weight[k] = weight.get(k, 0.0) + prof[i + 1]
         plan.close()
     order = sorted(weight, key=lambda k: -weight[k])
-    name = lambda v, tt: ({0: "W", 2: "G", 3: "T"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
+    name = lambda v, tt: ({0: "W", 2: "G", 3: "T", 5: "Y"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
     if verbose:
         print(f"[refine] regime {regime} start: " + ", ".join(f"{b:.3f} ms" for b in base) +
               f"; {len(order)} classes", flush=True)
@@ -359,6 +373,9 @@ def main():
                     help="refine regime 0 (small launches, fused), 1 (large launches) or 2 (J/K plans) of the "
                          "existing --out file on whole-execute times")
     ap.add_argument("--passes", type=int, default=2, help="refinement passes over the classes")
+    ap.add_argument("--rys-only", action="store_true",
+                    help="with --refine: try only the Rys kernel choices (classes with L >= --min-l)")
+    ap.add_argument("--min-l", type=int, default=0)
     args = ap.parse_args()
     if args.refine >= 0:
         rng = np.random.Generator(np.random.PCG64(9))
@@ -380,7 +397,8 @@ def main():
             # screened form, and a 16-atom cluster-recipe molecule (mixed)
             wls, n = [(batch, {}), (batch, mixed), (cluster, mixed)], 3
         rows = refine_regime(wls, read_rows(args.out), regime=args.refine, reps=args.reps, n=n,
-                             passes=args.passes, checkpoint=lambda r: write_rows(args.out, r))
+                             passes=args.passes, checkpoint=lambda r: write_rows(args.out, r),
+                             cands=RYS_CANDS if args.rys_only else None, min_l=args.min_l)
         write_rows(args.out, rows)
         return 0
     want = sorted({int(x) for x in args.regimes.split(",")})
