@@ -106,6 +106,18 @@ def vrr_block(g, ctype, indent):
     return defs, body, accs
 
 
+# Boys grid of a register class: staged into LDS per block (30.8 KB) for the
+# light classes (top order <= 2), whose primitive loops are short and read it
+# most; read through L1/L2 for the others, whose many VGPRs already limit the
+# blocks per CU and whose blocks would otherwise hold the LDS while their
+# slowest wavefront finishes.
+BOYS_LDS_MAX_MMAX = 3
+
+
+def boys_in_lds(c):
+    return VrrGraph(*c).mmax <= BOYS_LDS_MAX_MMAX
+
+
 def emit_rkernel(c):
     la, lb, lc, ld = c
     g = VrrGraph(*c)
@@ -120,7 +132,11 @@ def emit_rkernel(c):
     # the block's Boys slice area, so the single-class kernel hgp_r_<class> and
     # the fused multi-class kernel hgp_rf_kernel run the same code.
     w(f"__device__ __forceinline__ void hgp_r_body_{name}(const EriArgs& args, double* sboys, int bid) {{")
-    w(f"  stage_boys_slice(sboys, args.boys, {mmax - 1});")
+    if boys_in_lds(c):
+        w(f"  stage_boys_slice(sboys, args.boys, {mmax - 1});")
+        w(f"  const double* btab = sboys;")
+    else:
+        w(f"  const double* btab = args.boys + (size_t){mmax - 1} * BOYS_NROW * 8;  // read through L1/L2")
     w(f"  QuartetCtx qc;")
     # Lanes without a quartet (past the task's end, or screened) return, except
     # in the J/K build: there every lane joins the wavefront's run reduction
@@ -162,7 +178,7 @@ def emit_rkernel(c):
     w(f"    const double T = zeta * eta * inv_s * (PQx * PQx + PQy * PQy + PQz * PQz);")
     w(f"    const double pref = bp.K * kp.K * rs;")
     w(f"    double F[{mmax}];")
-    w(f"    boys_eval<{mmax}>(T, sboys, F);")
+    w(f"    boys_eval<{mmax}>(T, btab, F);")
     if mmax > 1:
         w(f"    if (use32) {{  // FP32 VRR (Boys and prefactor stay FP64)")
         d, bdy, acc = vrr_block(g, "float", "      ")
@@ -250,8 +266,11 @@ def emit_rkernel(c):
     w("}")
     w("#ifndef MI_R_BODIES_ONLY")
     w(f"__global__ void __launch_bounds__(256) hgp_r_{name}(const EriArgs args) {{")
-    w(f"  __shared__ double sboys[BOYS_NROW * 8];")
-    w(f"  hgp_r_body_{name}(args, sboys, blockIdx.x);")
+    if boys_in_lds(c):
+        w(f"  __shared__ double sboys[BOYS_NROW * 8];")
+        w(f"  hgp_r_body_{name}(args, sboys, blockIdx.x);")
+    else:
+        w(f"  hgp_r_body_{name}(args, nullptr, blockIdx.x);")
     w("}")
     w("#endif")
     return "\n".join(L), g, h
@@ -461,9 +480,10 @@ def main():
     fz = ["// Generated by gen_hgp.py -- do not edit.", "#pragma once", "",
           f"#define HGP_NRGROUP {NRGROUP}"]
     for gi in range(NRGROUP):
+        lds = any(boys_in_lds(c) for c in classes if c in rcls and register_group(c) == gi)
         fz += [f"#if defined(MI_FUSED_GROUP) && MI_FUSED_GROUP == {gi}",
                f"__global__ void __launch_bounds__(256) hgp_rf{gi}_kernel(const EriArgs args, const FusedSegs F) {{",
-               "  __shared__ double sboys[BOYS_NROW * 8];",
+               "  __shared__ double sboys[BOYS_NROW * 8];" if lds else "  double* sboys = nullptr;",
                "  const int s = fused_segment(F, (int)blockIdx.x);",
                "  EriArgs a = args;",
                "  a.tasks = args.tasks + F.task_off[s];",
@@ -503,6 +523,7 @@ def main():
     t.append(f"#define HGP_NCLASS {len(classes)}")
     t.append(f"#define HGP_MAX_VLEV {maxlev}")
     t.append(f"#define HGP_MAX_HLEV {maxhlev}")
+    t.append(f"#define HGP_NRGROUP_T {NRGROUP}")
     rows = []
     for c, d in zip(classes, descs):
         vl = list(d["vlev"]) + [d["vlev"][-1]] * (maxlev - len(d["vlev"]))
@@ -520,6 +541,13 @@ def main():
     t.append("__constant__ WClassDesc kWClassDescDev[HGP_NCLASS] = {")
     t += rows
     t.append("};")
+    t.append("// wavefronts per block of each class's register kernel: four where the")
+    t.append("// block stages the Boys grid in LDS, else one (each wavefront on its own)")
+    t.append("static const int kRegisterWavesPerBlock[HGP_NCLASS] = {"
+             + ", ".join(str((4 if boys_in_lds(c) else 1) if c in rcls else 0) for c in classes) + "};")
+    t.append("static const int kFusedWavesPerBlock[HGP_NRGROUP_T] = {"
+             + ", ".join(str(4 if any(boys_in_lds(c) for c in rcls if register_group(c) == gi) else 1)
+                         for gi in range(NRGROUP)) + "};")
     t.append("// fusion group of each class's register kernel (-1: none)")
     t.append("static const int kRegisterGroup[HGP_NCLASS] = {"
              + ", ".join(str(register_group(c) if c in rcls else -1) for c in classes) + "};")

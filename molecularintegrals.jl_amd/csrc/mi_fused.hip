@@ -21,9 +21,9 @@ namespace mi {
 
 // launch_fused_register_g<MI_FUSED_GROUP>
 hipError_t MI_CAT(launch_fused_register_g, MI_FUSED_GROUP)(const EriArgs& args, const FusedSegs& F, int nblocks,
-                                                          hipStream_t stream) {
+                                                          int wpb, hipStream_t stream) {
   void* kargs[] = {const_cast<EriArgs*>(&args), const_cast<FusedSegs*>(&F)};
-  return hipLaunchKernel(kFusedRegisterKernel, dim3(nblocks), dim3(256), kargs, 0, stream);
+  return hipLaunchKernel(kFusedRegisterKernel, dim3(nblocks), dim3(64 * wpb), kargs, 0, stream);
 }
 
 }  // namespace mi

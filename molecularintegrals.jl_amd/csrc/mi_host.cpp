@@ -488,11 +488,17 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
       if (reg) {
         // block -> task of its first wavefront (one load in the kernel
         // instead of a search)
-        blocks = (waves + 3) / 4;
+        // one-wavefront blocks where the class reads the Boys grid through
+        // L1/L2 (each wavefront frees its slot when it ends), except in J/K
+        // plans: their launches run beside other requests' in the serving
+        // engine, and four-wavefront blocks leave those room between blocks
+        const int wpb = P->order == MI_ORDER_JK ? 4 : mi::register_waves_per_block(c);
+        L.wpb = wpb;
+        blocks = (waves + wpb - 1) / wpb;
         L.bt_off = (int64_t)P->h.block_task.size();
         int ti = 0;
         for (int64_t b = 0; b < blocks; ++b) {
-          const int64_t w0 = 4 * b;
+          const int64_t w0 = wpb * b;
           while (ti + 1 < L.ntasks && all_tasks[L.task_off + ti + 1].blockStart <= w0) ++ti;
           P->h.block_task.push_back(ti);
         }
@@ -1200,7 +1206,7 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
   a.block_task = L.var == 1 ? P->d_block_task + L.bt_off : nullptr;
   a.run_ctr = (a.mode == MI_MODE_JK && P->d_run_ctr) ? P->d_run_ctr + (&L - P->launches.data()) : nullptr;
   if (L.var == 1)
-    HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, ls));
+    HIP_TRY(ctx, mi::launch_class_register(L.cls, a, L.nblocks, L.wpb, ls));
   else if (L.var == 0 || L.var == 3)
     HIP_TRY(ctx, mi::launch_class_wave(L.cls, L.nw, L.var == 3, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
   else if (L.var == 4)

@@ -47,6 +47,7 @@ struct Launch {
   int nw;   // wave kernels: wavefronts per block
   bool tl;  // global-table kernel: op tables staged in LDS
   size_t scratch_off;  // global-table kernel: offset of its slice of the plan scratch (doubles)
+  int wpb = 4;         // register kernel: wavefronts per block
   int64_t bt_off;      // register kernel: offset of its block -> task table in the plan's
   int task_off, ntasks, nblocks;
   int64_t nq_shell, nq_prim;

@@ -18,7 +18,11 @@ hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const int32_t* pp_
                              const double* shell_xyz,
                              const int32_t* shell_prim_off, const double* prim_exp, const double* prim_cn,
                              PrimPair* pp, PairGeom* geom, const double2* bounds, hipStream_t stream);
-hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStream_t stream);
+hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, int wpb, hipStream_t stream);
+// wavefronts per block of class c's register kernel in unique-array plans (1
+// or 4; J/K plans use 4) and of fusion group g's fused kernel
+int register_waves_per_block(int c);
+int fused_register_waves_per_block(int g);
 // LDS wave kernel; tg: op tables read from global memory (tuning variant 3)
 // instead of staged into LDS (variant 0)
 hipError_t launch_class_wave(int c, int nw, bool tg, const EriArgs& args, int nblocks, int64_t nq_launch,

@@ -781,10 +781,13 @@ hipError_t launch_pair_prims(int npp, const int32_t* pp_pair, const int32_t* pp_
   return hipGetLastError();
 }
 
-hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, hipStream_t stream) {
+int register_waves_per_block(int c) { return kRegisterWavesPerBlock[c]; }
+int fused_register_waves_per_block(int g) { return kFusedWavesPerBlock[g]; }
+
+hipError_t launch_class_register(int c, const EriArgs& args, int nblocks, int wpb, hipStream_t stream) {
   const void* fn = kRegisterKernels[c];
   void* kargs[] = {const_cast<EriArgs*>(&args)};
-  return hipLaunchKernel(fn, dim3(nblocks), dim3(256), kargs, 0, stream);
+  return hipLaunchKernel(fn, dim3(nblocks), dim3(64 * wpb), kargs, 0, stream);
 }
 
 size_t wclass_lds_bytes(int c, int nw, bool tg) {
@@ -867,21 +870,22 @@ hipError_t launch_class_wave(int c, int nw, bool tg, const EriArgs& args, int nb
   return hipLaunchKernel(w_kernel_fn(c, a.jk_run > 0, tg), dim3(nblocks), dim3(64 * nw), kargs, lds, stream);
 }
 
-hipError_t launch_fused_register_g0(const EriArgs&, const FusedSegs&, int, hipStream_t);
-hipError_t launch_fused_register_g1(const EriArgs&, const FusedSegs&, int, hipStream_t);
-hipError_t launch_fused_register_g2(const EriArgs&, const FusedSegs&, int, hipStream_t);
-hipError_t launch_fused_register_g3(const EriArgs&, const FusedSegs&, int, hipStream_t);
-hipError_t launch_fused_register_g4(const EriArgs&, const FusedSegs&, int, hipStream_t);
+hipError_t launch_fused_register_g0(const EriArgs&, const FusedSegs&, int, int, hipStream_t);
+hipError_t launch_fused_register_g1(const EriArgs&, const FusedSegs&, int, int, hipStream_t);
+hipError_t launch_fused_register_g2(const EriArgs&, const FusedSegs&, int, int, hipStream_t);
+hipError_t launch_fused_register_g3(const EriArgs&, const FusedSegs&, int, int, hipStream_t);
+hipError_t launch_fused_register_g4(const EriArgs&, const FusedSegs&, int, int, hipStream_t);
 static_assert(HGP_NRGROUP == 5, "one launch_fused_register_g<k> per fusion group");
 
 hipError_t launch_fused_register(int group, const EriArgs& args, const FusedSegs& F, int nblocks,
                                  hipStream_t stream) {
+  const int wpb = kFusedWavesPerBlock[group];
   switch (group) {
-    case 0: return launch_fused_register_g0(args, F, nblocks, stream);
-    case 1: return launch_fused_register_g1(args, F, nblocks, stream);
-    case 2: return launch_fused_register_g2(args, F, nblocks, stream);
-    case 3: return launch_fused_register_g3(args, F, nblocks, stream);
-    case 4: return launch_fused_register_g4(args, F, nblocks, stream);
+    case 0: return launch_fused_register_g0(args, F, nblocks, wpb, stream);
+    case 1: return launch_fused_register_g1(args, F, nblocks, wpb, stream);
+    case 2: return launch_fused_register_g2(args, F, nblocks, wpb, stream);
+    case 3: return launch_fused_register_g3(args, F, nblocks, wpb, stream);
+    case 4: return launch_fused_register_g4(args, F, nblocks, wpb, stream);
     default: return hipErrorInvalidValue;
   }
 }
