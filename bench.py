@@ -161,7 +161,7 @@ def plan_flops(plan):
 def symbol_of(launch):
     cname = "".join("spdf"[x] for x in launch[0])
     return {"R": "hgp_r_" + cname, "W": "hgp_w_kernel", "T": "hgp_w_kernel", "G": "hgp_wg_kernel",
-            "X": "hgp_wx_kernel", "Y": "hgp_ry_kernel"}[launch[4][0]]
+            "X": "hgp_wx_kernel", "Y": "hgp_ry_kernel", "Z": "hgp_ryg_kernel"}[launch[4][0]]
 
 
 def roofline_from_profile(plan, ms, step_ms, n_unique):

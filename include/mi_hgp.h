@@ -116,8 +116,9 @@ int64_t mi_n_unique(int32_t nbf);        /* n(n+1)(n^2+n+2)/8, src/ERI.jl:106 */
  * (quartet class, bra contraction bucket, ket contraction bucket, variant
  * [1 generated register kernel, 0 LDS wave kernel, 2 global-table kernel,
  * 3 LDS wave kernel reading its op tables from global memory, 5 Rys kernel
- * (1, 2 or 4 wavefronts; used by MI_METHOD_RYS plans, and by HGP plans where
- * the table names it: same integrals within rounding)],
+ * (1, 2 or 4 wavefronts), 6 Rys lane-group kernel (G = 16, 32 or 64 lanes
+ * per shell quartet); 5 and 6 serve MI_METHOD_RYS plans, and HGP plans where
+ * the table names them: the same integrals within rounding],
  * G = lanes per quartet for the register kernel or wavefronts per quartet /
  * block for the wave kernels, regime [0: class launches of fewer than 65536
  * shell quartets, 1: larger ones, 2: launches of MI_ORDER_JK plans (where a
@@ -219,7 +220,8 @@ int mi_plan_nlaunch(const mi_plan *plan);
  * blocks); kernel2 = (variant, wavefronts per block): variant 1 = generated
  * register kernel (wavefronts 0), 0 = LDS wave kernel, 2 = global-table
  * kernel, 3 = LDS wave kernel with op tables in global memory, 4 =
- * double-double table kernel, 5 = Rys kernel.  Returns MI_OK
+ * double-double table kernel, 5 = Rys kernel, 6 = Rys lane-group kernel
+ * (second entry: lanes per group).  Returns MI_OK
  * or MI_EINVAL on a bad index. */
 int mi_plan_launch_info(const mi_plan *plan, int32_t i, int32_t *cls4, int64_t *counts3, int32_t *kernel2);
 /* One execute with HIP events between launches on `stream`; synchronises and

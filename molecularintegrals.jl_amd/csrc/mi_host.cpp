@@ -168,8 +168,9 @@ bool tuning_row_ok(const mi_ctx* c, int cls, int bb, int kb, int var, int G, int
   if (cls < 0 || cls >= mi::nclass() || bb < 0 || bb >= MI_NBUCKET || kb < 0 || kb >= MI_NBUCKET) return false;
   if (regime < -1 || regime >= MI_NREGIME) return false;
   if (G < 1 || G > 64 || (G & (G - 1))) return false;
-  if (var < 0 || var > 5 || var == 4) return false;
+  if (var < 0 || var > 6 || var == 4) return false;
   if (var == 5 && (G > 4 || !mi::ryclass_fits_nw(cls, G, c->lds_limit))) return false;
+  if (var == 6 && !mi::rygclass_fits(cls, G)) return false;
   if (var == 1 && !mi::has_register_kernel(cls)) return false;
   if (var == 0 && !mi::wclass_fits_nw(cls, G, c->lds_limit)) return false;
   if (var == 3 && !mi::wclass_fits_nw(cls, G, c->lds_limit, true)) return false;
@@ -405,12 +406,16 @@ void mi_destroy(mi_ctx* ctx) {
 // (2) where it has an entry, else the size regime.
 // rys: the plan's method is Rys quadrature -- every class in the Rys kernel
 // (variant 5) with 1, 2 or 4 wavefronts per block by the size of its
-// contracted block, unless the tuning table names a Rys choice for it.
+// contracted block, unless the tuning table names a Rys choice for it
+// (variant 5 or the lane-group kernel, variant 6).
 static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, int regime, bool jk, bool rys) {
   const TuneEntry* tj = jk ? &ctx->tune[tune_index(2, t.cls, bb, kb)] : nullptr;
   const TuneEntry& te = (tj && tj->variant >= 0) ? *tj : ctx->tune[tune_index(regime, t.cls, bb, kb)];
   int variant, G;
-  if (rys) {
+  if (rys && te.variant == 6) {  // the tuning table's Rys lane-group choice
+    variant = 6;
+    G = te.G;
+  } else if (rys) {
     const int nacc = mi::wclass(t.cls).nacc;
     variant = 5;
     G = te.variant == 5 ? te.G : (nacc <= 256 ? 1 : (nacc <= 1024 ? 2 : 4));
@@ -437,7 +442,8 @@ static int choose_kernel(mi_ctx* ctx, TaskDesc& t, int bb, int kb, double kavg, 
   if ((variant == 0 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit)) ||
       (variant == 3 && !mi::wclass_fits_nw(t.cls, G, ctx->lds_limit, true)) ||
       (variant == 2 && !mi::wgclass_fits_nw(t.cls, G, ctx->lds_limit)) ||
-      (variant == 5 && !mi::ryclass_fits_nw(t.cls, G, ctx->lds_limit)))
+      (variant == 5 && !mi::ryclass_fits_nw(t.cls, G, ctx->lds_limit)) ||
+      (variant == 6 && !mi::rygclass_fits(t.cls, G)))
     return set_err(ctx, MI_EUNSUPPORTED, "class %d has no kernel in this build", t.cls);
   t.G = G;
   t.variant = variant;
@@ -455,13 +461,15 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
     // 1, 2, 4 wavefronts; 4-7: global-table kernel with 1, 2, 4, 8
     // wavefronts; 8-10: wave kernel with global op tables, 1, 2, 4 wavefronts;
     // 11: double-double table kernel (1 wavefront); 12-14: Rys kernel with
-    // 1, 2, 4 wavefronts
-    for (int vk = 0; vk < 15; ++vk) {
-      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : (vk < 11 ? 3 : (vk < 12 ? 4 : 5))));
+    // 1, 2, 4 wavefronts; 15-17: Rys lane-group kernel with 16, 32, 64 lanes
+    // per group
+    for (int vk = 0; vk < 18; ++vk) {
+      const int var = vk == 0 ? 1 : (vk < 4 ? 0 : (vk < 8 ? 2 : (vk < 11 ? 3 : (vk < 12 ? 4 : (vk < 15 ? 5 : 6)))));
       const int nw = vk == 0 ? 0
                              : (vk < 4 ? (1 << (vk - 1))
                                        : (vk < 8 ? (1 << (vk - 4))
-                                                 : (vk < 11 ? (1 << (vk - 8)) : (vk < 12 ? 1 : (1 << (vk - 12))))));
+                                                 : (vk < 11 ? (1 << (vk - 8))
+                                                            : (vk < 12 ? 1 : (vk < 15 ? (1 << (vk - 12)) : (16 << (vk - 15)))))));
       std::vector<TaskDesc> ts;
       for (const TaskDesc& t : per_class[c])
         if (t.variant == var && (var == 1 || t.G == nw)) ts.push_back(t);
@@ -506,6 +514,8 @@ static int group_launches(mi_ctx* ctx, mi_plan* P, std::vector<std::vector<TaskD
       if (var == 0 || var == 3)
         blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::wave_kernel_blocks_per_cu(c, nw, var == 3));
       if (var == 5) blocks = std::min<int64_t>(qs, (int64_t)ctx->ncu * mi::ry_kernel_blocks_per_cu(c, nw));
+      if (var == 6)  // nw = lanes per group; 64 / nw groups per one-wavefront block
+        blocks = std::min<int64_t>((qs * nw + 63) / 64, (int64_t)ctx->ncu * mi::ryg_kernel_blocks_per_cu(c, nw));
       // double-double kernel: a few persistent blocks (its quartets are rare,
       // and each block holds a 16-byte-per-entry work area in the scratch)
       if (var == 4) {
@@ -1215,6 +1225,8 @@ static int launch_one(mi_ctx* ctx, const Launch& L, EriArgs a, const mi_plan* P,
   else if (L.var == 5)
     HIP_TRY(ctx, mi::launch_class_rys(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables,
                                       P->d_scratch + L.scratch_off, ls));
+  else if (L.var == 6)
+    HIP_TRY(ctx, mi::launch_class_ryg(L.cls, L.nw, a, L.nblocks, L.nq_shell, ctx->wtables, ls));
   else
     HIP_TRY(ctx, mi::launch_class_wave_global(L.cls, L.nw, L.tl, a, L.nblocks, L.nq_shell, ctx->wtables,
                                               P->d_scratch + L.scratch_off, ls));

@@ -68,6 +68,12 @@ size_t ry_scratch_doubles(int c, int nw, int nblocks);
 int ry_kernel_blocks_per_cu(int c, int nw);
 hipError_t launch_class_rys(int c, int nw, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
                             double* scratch, hipStream_t stream);
+// Rys lane-group kernel, variant 6 (mi_rys.hip): one-wavefront blocks of
+// 64/lg groups of lg in {16, 32, 64} lanes, one shell quartet per group
+bool rygclass_fits(int c, int lg);
+int ryg_kernel_blocks_per_cu(int c, int lg);
+hipError_t launch_class_ryg(int c, int lg, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
+                            hipStream_t stream);
 // host evaluation of the kernels' Rys rule: n roots x (= t^2) and weights at T
 void rys_rule_host(int n, double T, double* x, double* w);
 int rys_nmax();

@@ -995,6 +995,18 @@ int ry_kernel_blocks_per_cu(const WClassDesc& D, int nw);
 hipError_t launch_rys(const WClassDesc& D, int nw, const EriArgs& args, int nblocks, int64_t nq_launch,
                       const uint32_t* gblob, double* scratch, hipStream_t stream);
 hipError_t set_rys_lds_limit(size_t bytes);
+bool rygclass_fits(const WClassDesc& D, int lg);
+int ryg_kernel_blocks_per_cu(const WClassDesc& D, int lg);
+hipError_t launch_ryg(const WClassDesc& D, int lg, const EriArgs& args, int nblocks, int64_t nq_launch,
+                      const uint32_t* gblob, hipStream_t stream);
+
+bool rygclass_fits(int c, int lg) { return rygclass_fits(kWClassDesc[c], lg); }
+int ryg_kernel_blocks_per_cu(int c, int lg) { return ryg_kernel_blocks_per_cu(kWClassDesc[c], lg); }
+hipError_t launch_class_ryg(int c, int lg, const EriArgs& args, int nblocks, int64_t nq_launch, void* wtables,
+                            hipStream_t stream) {
+  const WTablesHost* t = static_cast<const WTablesHost*>(wtables);
+  return launch_ryg(kWClassDesc[c], lg, args, nblocks, nq_launch, t->gblob, stream);
+}
 
 bool ryclass_fits_nw(int c, int nw, size_t lds_limit) { return ryclass_fits_nw(kWClassDesc[c], nw, lds_limit); }
 size_t ry_scratch_doubles(int c, int nw, int nblocks) { return ry_scratch_doubles(kWClassDesc[c], nw, nblocks); }

@@ -43,7 +43,7 @@ TAUS = [1, 2, 4, 8, 16, 32, 64, 1 << 30]
 # kernel choices: (0, nw) LDS wave kernel, (2, nw) global-table kernel, (3, nw)
 # LDS wave kernel with global op tables, (1, tau) register kernel, (5, nw) Rys
 # quadrature kernel (the same integrals by Rys quadrature; HGP transfer)
-RYS_CANDS = [(5, 1), (5, 2), (5, 4)]
+RYS_CANDS = [(5, 1), (5, 2), (5, 4), (6, 16), (6, 32), (6, 64)]
 CANDS = [(0, 1), (0, 2), (0, 4), (2, 1), (2, 2), (2, 4), (2, 8), (3, 1), (3, 2), (3, 4)] + [(1, t) for t in TAUS]
 
 
@@ -65,8 +65,8 @@ def available(c, var, tau, has_reg, w_ok, g_ok, t_ok=None):
     kernel's wavefront counts under key 5."""
     if var == 1:
         return has_reg[c]
-    if var == 5:
-        return t_ok is not None and t_ok[c].get((5, int(tau)), False)
+    if var in (5, 6):
+        return t_ok is not None and t_ok[c].get((var, int(tau)), False)
     ok = {0: w_ok, 2: g_ok, 3: t_ok}.get(var)
     return ok is not None and ok[c].get(int(tau), False)
 
@@ -217,12 +217,13 @@ def probe_variants(ctx, nclass):
                     ok[nw] = True
                 except Exception:
                     ok[nw] = False
-        for nw in (1, 2, 4):  # Rys kernel
-            try:
-                ctx.set_tuning([(k, 0, 0, 5, nw)])
-                t_ok[k][(5, nw)] = True
-            except Exception:
-                t_ok[k][(5, nw)] = False
+        for var, gs in ((5, (1, 2, 4)), (6, (16, 32, 64))):  # Rys kernels
+            for g in gs:
+                try:
+                    ctx.set_tuning([(k, 0, 0, var, g)])
+                    t_ok[k][(var, g)] = True
+                except Exception:
+                    t_ok[k][(var, g)] = False
     return has_reg, w_ok, g_ok, t_ok
 
 
@@ -331,7 +332,7 @@ def refine_regime(workloads, rows, regime=0, reps=5, n=40, passes=2, gain=0.015,
                 weight[k] = weight.get(k, 0.0) + prof[i + 1]
         plan.close()
     order = sorted(weight, key=lambda k: -weight[k])
-    name = lambda v, tt: ({0: "W", 2: "G", 3: "T", 5: "Y"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
+    name = lambda v, tt: ({0: "W", 2: "G", 3: "T", 5: "Y", 6: "Z"}.get(v, "R/") + str(tt if tt < 1e6 else "inf"))
     if verbose:
         print(f"[refine] regime {regime} start: " + ", ".join(f"{b:.3f} ms" for b in base) +
               f"; {len(order)} classes", flush=True)
@@ -434,7 +435,8 @@ def main():
 def write_rows(path, rows):
     with open(path, "w") as f:
         f.write("# HGP ERI kernel choice per (class, bra bucket, ket bucket, regime): variant (1 register "
-                "kernel, 0 LDS wave kernel, 2 global-table kernel) and G (lanes per quartet, or wavefronts);\n"
+                "kernel, 0 LDS wave kernel, 2 global-table kernel, 3 wave kernel with global tables, 5 Rys "
+                "kernel, 6 Rys lane-group kernel) and G (lanes per quartet, or wavefronts);\n"
                 "# regime 0 = class launches of fewer than 65536 shell quartets, 1 = larger ones, "
                 "2 = launches of J/K plans.\n"
                 "# Written by mi_amd/autotune.py on the GPU it describes; columns: class bra_bucket ket_bucket "

@@ -164,8 +164,8 @@ class Plan:
         blocks, kernel), kernel = "R" (generated register kernel), "W<nw>" (LDS
         wave kernel, nw wavefronts per quartet), "G<nw>" (global-table kernel,
         nw wavefronts per block), "T<nw>" (wave kernel, global op tables) or
-        "X4" (double-double table kernel) or "Y<nw>" (Rys kernel, nw wavefronts
-        per block)."""
+        "X4" (double-double table kernel), "Y<nw>" (Rys kernel, nw wavefronts
+        per block) or "Z<lg>" (Rys lane-group kernel, lg lanes per quartet)."""
         out = []
         n = self.ctx.L.mi_plan_nlaunch(self.h)
         for i in range(n):
@@ -173,7 +173,7 @@ class Plan:
             n3 = (ctypes.c_int64 * 3)()
             k2 = (ctypes.c_int32 * 2)()
             self.ctx.check(self.ctx.L.mi_plan_launch_info(self.h, i, c4, n3, k2))
-            kern = "R" if k2[0] == 1 else {0: "W", 2: "G", 3: "T", 4: "X", 5: "Y"}[k2[0]] + str(k2[1])
+            kern = "R" if k2[0] == 1 else {0: "W", 2: "G", 3: "T", 4: "X", 5: "Y", 6: "Z"}[k2[0]] + str(k2[1])
             out.append((tuple(c4), int(n3[0]), int(n3[1]), int(n3[2]), kern))
         return out
 

@@ -121,3 +121,59 @@ def test_rys_screened_batched_and_jk(ctx):
     J, K = ctx.jk(fb, D, method=RYS)
     Jr, Kr = O.jk_from_unique(n, O.eri_unique_hgp(fb, order=O.ORDER_IINDEX), D)
     assert np.allclose(J, Jr, rtol=1e-11, atol=1e-12) and np.allclose(K, Kr, rtol=1e-11, atol=1e-12)
+
+
+def _ryg_rows(ctx, lg):
+    """Tuning rows that put every class the Rys lane-group kernel (variant 6)
+    can take on it, with lg lanes per quartet, in every regime."""
+    from mi_amd.hgp_tables import quartet_classes
+    rows = []
+    for k in range(len(quartet_classes())):
+        try:
+            ctx.set_tuning([(k, 0, 0, 6, lg)])
+        except Exception:
+            continue
+        rows += [(k, bb, kb, 6, lg, -1) for bb in range(5) for kb in range(5)]
+    ctx.set_tuning([])
+    return rows
+
+
+@pytest.mark.parametrize("lg", [16, 32, 64])
+def test_rys_lane_group_kernel(lg):
+    """Variant 6 (groups of lg lanes per shell quartet, several quartets per
+    wavefront) on random contracted bases up to f: unique array in HGP and Rys
+    plans against the oracle, a screened batch, and J/K."""
+    import torch
+    from mi_amd import Context, flat_from_shells
+    c = Context(0)
+    rows = _ryg_rows(c, lg)
+    assert len(rows) > 25 * 40  # most classes up to f fit
+    c.set_tuning(rows)
+    rng = np.random.Generator(np.random.PCG64(800 + lg))
+    fb = flat_from_shells(random_spec(rng, 7, lmax=3, kmax=3))
+    ref = O.eri_unique_hgp(fb, order=O.ORDER_CHRR, nthreads=8)
+    for method in (0, RYS):
+        plan = c.plan(fb, order=O.ORDER_CHRR, method=method)
+        assert any(l[4].startswith("Z") for l in plan.launches())
+        plan.close()
+        got = c.eri_unique(fb, order=O.ORDER_CHRR, method=method)
+        assert_eri_close(got, ref, f"lane-group lg={lg} method={method}", fb, O.ORDER_CHRR)
+    fbs = [flat_from_shells(random_spec(rng, 5 + k, lmax=2, kmax=3)) for k in range(3)]
+    plan = c.batch_plan(fbs, order=O.ORDER_CHRR, schwarz_cut=1e-12)
+    out = torch.empty(plan.n_unique, dtype=torch.float64, device="cuda")
+    plan.execute(out.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    offs = plan.mol_offsets()
+    for m, f in enumerate(fbs):
+        r = O.eri_unique_hgp(f, order=O.ORDER_CHRR)
+        d = np.abs(got[offs[m]:offs[m + 1]] - r)
+        assert np.all((d <= 1e-12) | (d <= 1e-10 * np.abs(r)) | (d <= 1.001e-12)), (m, float(d.max()))
+    plan.close()
+    n = fb.nbf
+    D = rng.standard_normal((n, n))
+    D = (D + D.T) / (2 * n)
+    J, K = c.jk(fb, D)
+    Jr, Kr = O.jk_from_unique(n, O.eri_unique_hgp(fb, order=O.ORDER_IINDEX, nthreads=8), D)
+    assert np.allclose(J, Jr, rtol=1e-11, atol=1e-12) and np.allclose(K, Kr, rtol=1e-11, atol=1e-12)
+    c.close()
