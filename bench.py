@@ -240,7 +240,9 @@ def headline(args):
     off = mplan.mol_offsets()
     ms_stats = mplan.stats()
     log(f"[bench] headline: {nmol} molecules, {n:.3e} unique ERIs ({n * 8 / 1e9:.1f} GB), mixed plan {t_plan:.2f} s")
-    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    # NaN first: a slot the graph leaves unwritten fails the error checks
+    # below, which read the graph's output
+    out = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
     mplan.capture(out.data_ptr())
     for _ in range(args.warmup):
         mplan.replay(sp)
@@ -255,12 +257,10 @@ def headline(args):
     ms_mixed = e0.elapsed_time(e1) / args.steps
     value = n / (ms_mixed * 1e-3)
     log(f"[bench] batched mixed graph: {ms_mixed:.1f} ms/step, {value:.3e} unique ERIs/s")
-    prof = mplan.profile(out.data_ptr(), sp)
-    roof = roofline_from_profile(mplan, prof, ms_mixed, n)
-    roof["profile_sum_ms"] = float(sum(prof))
-    # errors of the mixed form: sampled slots against the exact values, then
-    # every integral against the FP64 per-molecule plans (scratch buffer)
-    err = {"mixed_sampled": sampled_exact_errors(fbs, off, lambda idx: out[torch.from_numpy(idx).cuda()].cpu().numpy())}
+    # errors of the mixed form (the graph's output): sampled slots against the
+    # exact values, then every integral against the FP64 per-molecule plans
+    err = {"unwritten_slots": sum(int(torch.isnan(out[i:i + (1 << 28)]).sum()) for i in range(0, n, 1 << 28)),
+           "mixed_sampled": sampled_exact_errors(fbs, off, lambda idx: out[torch.from_numpy(idx).cuda()].cpu().numpy())}
     plans = [ctx.plan(fb) for fb in fbs]
     scratch = torch.empty(int(max(off[m + 1] - off[m] for m in range(nmol))), dtype=torch.float64, device="cuda")
     worst = torch.zeros((), dtype=torch.float64, device="cuda")
@@ -270,6 +270,9 @@ def headline(args):
         worst = torch.maximum(worst, (out[int(off[m]):int(off[m]) + k] - scratch[:k]).abs().max())
     err["max_abs_err_mixed_vs_fp64_every_integral"] = float(worst)
     del scratch
+    prof = mplan.profile(out.data_ptr(), sp)
+    roof = roofline_from_profile(mplan, prof, ms_mixed, n)
+    roof["profile_sum_ms"] = float(sum(prof))
     # plain path: per-molecule FP64 plans one after the other, same output
     base = out.data_ptr()
 

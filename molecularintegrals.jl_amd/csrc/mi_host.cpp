@@ -153,6 +153,19 @@ int kbucket(int K) {
   return 4;
 }
 
+// Finer contraction buckets for the pair lists of unique-array plans, nested
+// in kbucket's (the tuning table stays indexed by kbucket): a task pairs two
+// lists, and the lanes of a register-kernel wavefront loop as long as the
+// deepest quartet among them, so lists of pairs of similar primitive-pair
+// count K keep the lanes busy (quartet K = K_bra K_ket varies by up to 3.5x
+// per pair within one kbucket).
+int kbucket_fine(int K) {
+  static const int ub[] = {1, 3, 4, 6, 9, 12, 18, 27, 36, 54, 81};
+  int f = 0;
+  while (f < (int)(sizeof(ub) / sizeof(ub[0])) && K > ub[f]) ++f;
+  return f;
+}
+
 int pow2floor(double x) {
   int g = 1;
   while (g * 2 <= x && g < 64) g *= 2;
@@ -706,7 +719,7 @@ int mi_plan_set_mol_outputs(mi_plan* P, double* const* outs) {
 }
 
 // Schwarz bounds of every shell pair, in the canonical pair order
-// (molecule, pair class, contraction bucket, [first shell: J/K layout,]
+// (molecule, pair class, list bucket, [first shell: J/K layout,]
 // pair index) of a plan built with the same options: a bounds-only plan
 // (diagonal quartets) runs on the context's bounds stream with stream-ordered
 // memory and is read back.  Q = max |(ab|ab)|, V = max |(e0|e0)| (VRR outputs).
@@ -805,6 +818,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   // resolution); the kept (ia, ib) of a pair are listed in pp_ab.
   struct PairKey {
     int mol, pc, bucket, pidx, A, B, K;
+    int list;     // list split key: kbucket_fine (unique-array plans) or kbucket (J/K plans)
     double Q, V;
     int64_t ab0;  // first kept (ia, ib) code in ab_codes
     int ext;      // HRR amplification above amp_limit: quartets run in double-double
@@ -851,7 +865,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
             amp = std::max(amp, std::pow((b / z * r + sw) / (a / z * r + sw), (double)L[B]));
           }
         }
-        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K, 0.0, 0.0, ab0,
+        keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K,
+                        P->order == MI_ORDER_JK ? kbucket(K) : kbucket_fine(K), 0.0, 0.0, ab0,
                         amp > amp_limit(L[B]) ? 1 : 0});
       }
   // J/K layout of a plan without a cut: within a (class, bucket) list, pairs
@@ -866,7 +881,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     if (x.mol != y.mol) return x.mol < y.mol;
     if (x.pc != y.pc) return x.pc < y.pc;
     if (x.ext != y.ext) return x.ext < y.ext;
-    if (x.bucket != y.bucket) return x.bucket < y.bucket;
+    if (x.list != y.list) return x.list < y.list;
     if (jk_layout && x.A != y.A) return x.A < y.A;
     return x.pidx < y.pidx;
   });
@@ -886,7 +901,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
         if (x.mol != y.mol) return x.mol < y.mol;
         if (x.pc != y.pc) return x.pc < y.pc;
         if (x.ext != y.ext) return x.ext < y.ext;
-        if (x.bucket != y.bucket) return x.bucket < y.bucket;
+        if (x.list != y.list) return x.list < y.list;
         return x.Q > y.Q;
       });
   }
@@ -911,7 +926,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   P->npp = (int)ppoff;
   P->pair_mol.resize(npair);
   for (int r = 0; r < npair; ++r) P->pair_mol[r] = keys[r].mol;
-  // Contiguous lists of equal (molecule, pc, ext, bucket).
+  // Contiguous lists of equal (molecule, pc, ext, list bucket).
   struct List {
     int start, n, pc, Ksum, Kmax, mol;
     double qmax;  // bound of the list's first pair (its largest, once ordered by bound)
@@ -923,7 +938,7 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     int e = r;
     int Ksum = 0, Kmax = 0;
     while (e < npair && keys[e].mol == keys[r].mol && keys[e].pc == keys[r].pc && keys[e].ext == keys[r].ext &&
-           keys[e].bucket == keys[r].bucket) {
+           keys[e].list == keys[r].list) {
       Ksum += keys[e].K;
       Kmax = std::max(Kmax, keys[e].K);
       ++e;
