@@ -65,12 +65,12 @@ MIXED = dict(precision=1, tau=1e-8, schwarz_cut=1e-12)
 # HBM bytes per launch of a kernel symbol, from two rocprofv3 --pmc runs of
 # `bench.py --profile-headline` (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000,
 # the factors a 256 MiB device copy gives): profiles/hbm_traffic_batch512_mixed.md.
-MEASURED_TRAFFIC_BYTES = {"hgp_wg_kernel": 4033344636, "hgp_r_ppps": 18523226619, "hgp_r_psps": 13415412831,
-                          "hgp_r_psss": 11043336357, "hgp_r_dsps": 11883731600, "hgp_r_dpps": 18688688897,
-                          "hgp_r_ppss": 6955630845, "hgp_r_dspp": 9471368023, "hgp_r_dsss": 5116464175,
-                          "hgp_r_dpss": 6025285195, "hgp_r_pppp": 8326991273, "hgp_r_dpds": 17598254771,
-                          "hgp_r_ssss": 2601571612, "hgp_r_ddps": 6254042044, "hgp_r_dsds": 3132094496,
-                          "hgp_r_ddss": 2275144748}
+MEASURED_TRAFFIC_BYTES = {"hgp_ryg_kernel": 4033468964, "hgp_r_dpps": 18787550477, "hgp_r_ppps": 18698113592,
+                          "hgp_r_dpds": 15029895188, "hgp_r_psps": 13537428267, "hgp_r_dsps": 12091566751,
+                          "hgp_r_psss": 11161761489, "hgp_r_dspp": 9588402761, "hgp_r_pppp": 8444821210,
+                          "hgp_r_ppss": 7123887096, "hgp_r_dpss": 6133335006, "hgp_r_ddps": 5576149708,
+                          "hgp_r_dsss": 5130852182, "hgp_r_dsds": 3087078774, "hgp_wg_kernel": 2762586398,
+                          "hgp_r_ssss": 2655345430, "hgp_r_ddss": 2274334824}
 
 
 def log(*a):
