@@ -32,6 +32,7 @@
 // item builds its 2D table in LDS, then each thread forms the products of the
 // (e0|f0) elements it owns (ACC per thread, kept in registers across batches).
 #include <cmath>
+#include <vector>
 
 #include "mi_device.h"
 
@@ -283,8 +284,11 @@ __global__ void __launch_bounds__(256) hgp_ry_kernel(const EriArgs args, const W
 // same number of quartet slots and, per slot, of primitive iterations (the
 // most any of its groups needs), so the barriers stay in uniform control flow.
 // LDS per group: 4*RY_SCL scales, then max(3n S + 2n, nhrr [+ nout, J/K]).
+// AM, CM, NR >= 0: instantiation for classes with L_a+L_b = AM, L_c+L_d = CM
+// and NR roots (recurrence loops and root sums unrolled at compile time);
+// -1: read from the class descriptor.
 // ---------------------------------------------------------------------------
-template <int LG, int ACC>
+template <int LG, int ACC, int AM = -1, int CM = -1, int NR = -1>
 __global__ void __launch_bounds__(64) hgp_ryg_kernel(const EriArgs args, const WClassDesc D,
                                                      const uint32_t* __restrict__ gblob, int64_t nq_launch,
                                                      int area) {
@@ -293,8 +297,8 @@ __global__ void __launch_bounds__(64) hgp_ryg_kernel(const EriArgs args, const W
   const int lane = threadIdx.x, grp = lane / LG, lg = lane - grp * LG;
   double* scl = lds + (size_t)grp * area;
   double* W = scl + 4 * RY_SCL;
-  const int n = (D.mmax + 1) / 2;
-  const int amax = D.la + D.lb, cmax = D.lc + D.ld;
+  const int n = NR > 0 ? NR : (D.mmax + 1) / 2;
+  const int amax = AM >= 0 ? AM : D.la + D.lb, cmax = CM >= 0 ? CM : D.lc + D.ld;
   const int c1 = cmax + 1, S = (amax + 1) * c1, tab = 3 * n * S;
   const uint4* gH = reinterpret_cast<const uint4*>(gblob + D.gh);
   const uint2_t* gO = reinterpret_cast<const uint2_t*>(gblob + D.go);
@@ -481,7 +485,19 @@ static const void* ryg_fn_lg(int acc) {
     default: return (const void*)hgp_ryg_kernel<LG, 16>;
   }
 }
-static const void* ryg_kernel_fn(int lg, int acc) {
+// Compile-time instantiations (L_a+L_b, L_c+L_d, roots, lanes per group,
+// accumulators per lane): the shapes the tuning table routes to this kernel.
+#define RYG_SPECS(X)                                                                                      \
+  X(3, 2, 3, 16, 16) X(3, 2, 3, 32, 8) X(3, 3, 4, 16, 8) X(3, 3, 4, 16, 16) X(3, 3, 4, 32, 8)             \
+  X(3, 3, 4, 64, 4) X(3, 4, 4, 64, 8) X(4, 2, 4, 16, 16) X(4, 2, 4, 32, 8) X(4, 2, 4, 64, 4)              \
+  X(4, 2, 4, 64, 8) X(4, 3, 4, 64, 4) X(4, 3, 4, 64, 8) X(5, 0, 3, 16, 4) X(5, 1, 4, 32, 8)               \
+  X(5, 1, 4, 64, 4) X(5, 2, 4, 64, 8) X(5, 3, 5, 64, 8) X(5, 3, 5, 64, 16)
+static const void* ryg_kernel_fn(const WClassDesc& D, int lg, int acc) {
+  const int am = D.la + D.lb, cm = D.lc + D.ld, n = (D.mmax + 1) / 2;
+#define RYG_PICK(A, C, N, L, K) \
+  if (am == A && cm == C && n == N && lg == L && acc == K) return (const void*)hgp_ryg_kernel<L, K, A, C, N>;
+  RYG_SPECS(RYG_PICK)
+#undef RYG_PICK
   return lg == 16 ? ryg_fn_lg<16>(acc) : (lg == 32 ? ryg_fn_lg<32>(acc) : ryg_fn_lg<64>(acc));
 }
 
@@ -516,7 +532,7 @@ int ryg_kernel_blocks_per_cu(const WClassDesc& D, int lg) {
   const RygGeom g = ryg_geom(D, lg, false);
   int nb = 0;
   if (g.acc == 0 ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ryg_kernel_fn(lg, g.acc), 64, g.lds) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ryg_kernel_fn(D, lg, g.acc), 64, g.lds) != hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;
 }
@@ -530,7 +546,7 @@ hipError_t launch_ryg(const WClassDesc& D, int lg, const EriArgs& args, int nblo
   WClassDesc d = D;
   int area = g.area;
   void* kargs[] = {&a, &d, const_cast<const uint32_t**>(&gblob), &nq_launch, &area};
-  return hipLaunchKernel(ryg_kernel_fn(lg, g.acc), dim3(nblocks), dim3(64), kargs, g.lds, stream);
+  return hipLaunchKernel(ryg_kernel_fn(D, lg, g.acc), dim3(nblocks), dim3(64), kargs, g.lds, stream);
 }
 
 static const void* ry_kernel_fn(int acc, bool hl) {
@@ -624,12 +640,20 @@ hipError_t set_rys_lds_limit(size_t bytes) {
       hipError_t e = hipFuncSetAttribute(ry_kernel_fn(acc, hl), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
       if (e != hipSuccess) return e;
     }
-  for (int lg : {16, 32, 64})
-    for (int acc : {2, 4, 8, 16}) {
-      hipError_t e = hipFuncSetAttribute(ryg_kernel_fn(lg, acc), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)std::min(bytes, (size_t)RYG_LDS_MAX));
-      if (e != hipSuccess) return e;
-    }
+  std::vector<const void*> fns;
+  for (int acc : {2, 4, 8, 16}) {
+    fns.push_back((const void*)ryg_fn_lg<16>(acc));
+    fns.push_back((const void*)ryg_fn_lg<32>(acc));
+    fns.push_back((const void*)ryg_fn_lg<64>(acc));
+  }
+#define RYG_FN(A, C, N, L, K) fns.push_back((const void*)hgp_ryg_kernel<L, K, A, C, N>);
+  RYG_SPECS(RYG_FN)
+#undef RYG_FN
+  for (const void* f : fns) {
+    hipError_t e =
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::min(bytes, (size_t)RYG_LDS_MAX));
+    if (e != hipSuccess) return e;
+  }
   g_ry_lds_limit = bytes;
   return hipSuccess;
 }
