@@ -486,24 +486,27 @@ static const void* ryg_fn_lg(int acc) {
   }
 }
 // Compile-time instantiations (L_a+L_b, L_c+L_d, roots, lanes per group,
-// accumulators per lane): every class up to d with 3 or more units of angular
-// momentum, at each lane count its accumulators allow, and the f-class shapes
-// of the tuning table.
+// accumulators per lane = ceil(nacc / lanes)): every class up to d with 3 or
+// more units of angular momentum at each lane count its accumulators allow,
+// and the f-class shapes of the tuning table.  Exact accumulator counts keep
+// the registers of unused accumulators and their table indices free.
 #define RYG_SPECS(X) \
-  X(2, 1, 2, 16, 2) X(2, 1, 2, 32, 2) X(2, 1, 2, 64, 2) X(2, 2, 3, 16, 4) X(2, 2, 3, 16, 8)    \
-  X(2, 2, 3, 32, 2) X(2, 2, 3, 32, 4) X(2, 2, 3, 64, 2) X(3, 0, 2, 16, 2) X(3, 0, 2, 32, 2)    \
-  X(3, 0, 2, 64, 2) X(3, 1, 3, 16, 4) X(3, 1, 3, 32, 2) X(3, 1, 3, 64, 2) X(3, 2, 3, 16, 8)    \
-  X(3, 2, 3, 16, 16) X(3, 2, 3, 32, 4) X(3, 2, 3, 32, 8) X(3, 2, 3, 64, 2) X(3, 2, 3, 64, 4)   \
-  X(3, 3, 4, 16, 8) X(3, 3, 4, 16, 16) X(3, 3, 4, 32, 8) X(3, 3, 4, 64, 4) X(3, 4, 4, 64, 8)   \
-  X(4, 0, 3, 16, 2) X(4, 0, 3, 32, 2) X(4, 0, 3, 64, 2) X(4, 1, 3, 16, 8) X(4, 1, 3, 32, 4)    \
-  X(4, 1, 3, 64, 2) X(4, 2, 4, 16, 16) X(4, 2, 4, 32, 8) X(4, 2, 4, 32, 16) X(4, 2, 4, 64, 4)  \
-  X(4, 2, 4, 64, 8) X(4, 3, 4, 32, 16) X(4, 3, 4, 64, 4) X(4, 3, 4, 64, 8) X(4, 4, 5, 64, 16)  \
-  X(5, 0, 3, 16, 4) X(5, 1, 4, 32, 8) X(5, 1, 4, 64, 4) X(5, 2, 4, 64, 8) X(5, 3, 5, 64, 8)    \
-  X(5, 3, 5, 64, 16)
+  X(2, 1, 2, 16, 2) X(2, 1, 2, 32, 1) X(2, 1, 2, 64, 1) X(2, 2, 3, 16, 3) X(2, 2, 3, 16, 4)    \
+  X(2, 2, 3, 16, 6) X(2, 2, 3, 32, 2) X(2, 2, 3, 32, 3) X(2, 2, 3, 64, 1) X(2, 2, 3, 64, 2)    \
+  X(3, 0, 2, 16, 1) X(3, 0, 2, 32, 1) X(3, 0, 2, 64, 1) X(3, 1, 3, 16, 3) X(3, 1, 3, 32, 2)    \
+  X(3, 1, 3, 64, 1) X(3, 2, 3, 16, 6) X(3, 2, 3, 16, 9) X(3, 2, 3, 32, 3) X(3, 2, 3, 32, 5)    \
+  X(3, 2, 3, 64, 2) X(3, 2, 3, 64, 3) X(3, 3, 4, 16, 7) X(3, 3, 4, 16, 10) X(3, 3, 4, 16, 16)  \
+  X(3, 3, 4, 32, 5) X(3, 3, 4, 32, 8) X(3, 3, 4, 64, 4) X(3, 4, 4, 64, 5) X(4, 0, 3, 16, 2)    \
+  X(4, 0, 3, 32, 1) X(4, 0, 3, 64, 1) X(4, 1, 3, 16, 6) X(4, 1, 3, 32, 3) X(4, 1, 3, 64, 2)    \
+  X(4, 2, 4, 16, 10) X(4, 2, 4, 16, 12) X(4, 2, 4, 32, 5) X(4, 2, 4, 32, 6) X(4, 2, 4, 32, 9)  \
+  X(4, 2, 4, 64, 3) X(4, 2, 4, 64, 4) X(4, 2, 4, 64, 5) X(4, 3, 4, 32, 16) X(4, 3, 4, 64, 4)   \
+  X(4, 3, 4, 64, 7) X(4, 3, 4, 64, 8) X(4, 4, 5, 64, 16) X(5, 0, 3, 16, 4) X(5, 1, 4, 32, 5)   \
+  X(5, 1, 4, 64, 3) X(5, 2, 4, 64, 5) X(5, 2, 4, 64, 8) X(5, 3, 5, 64, 8) X(5, 3, 5, 64, 9)    \
+  X(5, 3, 5, 64, 13)
 static const void* ryg_kernel_fn(const WClassDesc& D, int lg, int acc) {
-  const int am = D.la + D.lb, cm = D.lc + D.ld, n = (D.mmax + 1) / 2;
+  const int am = D.la + D.lb, cm = D.lc + D.ld, n = (D.mmax + 1) / 2, need = (D.nacc + lg - 1) / lg;
 #define RYG_PICK(A, C, N, L, K) \
-  if (am == A && cm == C && n == N && lg == L && acc == K) return (const void*)hgp_ryg_kernel<L, K, A, C, N>;
+  if (am == A && cm == C && n == N && lg == L && need == K) return (const void*)hgp_ryg_kernel<L, K, A, C, N>;
   RYG_SPECS(RYG_PICK)
 #undef RYG_PICK
   return lg == 16 ? ryg_fn_lg<16>(acc) : (lg == 32 ? ryg_fn_lg<32>(acc) : ryg_fn_lg<64>(acc));
