@@ -65,12 +65,12 @@ MIXED = dict(precision=1, tau=1e-8, schwarz_cut=1e-12)
 # HBM bytes per launch of a kernel symbol, from two rocprofv3 --pmc runs of
 # `bench.py --profile-headline` (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000,
 # the factors a 256 MiB device copy gives): profiles/hbm_traffic_batch512_mixed.md.
-MEASURED_TRAFFIC_BYTES = {"hgp_ryg_kernel": 4012379609, "hgp_r_ppps": 18687303514, "hgp_r_dpps": 18198234125,
-                          "hgp_r_dpds": 15019711261, "hgp_r_psps": 13539323367, "hgp_r_dsps": 12082408500,
-                          "hgp_r_psss": 11161891629, "hgp_r_dspp": 9336493813, "hgp_r_pppp": 8445986485,
-                          "hgp_r_ppss": 7123955283, "hgp_r_dpss": 6132121040, "hgp_r_ddps": 5577695924,
-                          "hgp_r_dsss": 5130417229, "hgp_r_dsds": 3092351410, "hgp_wg_kernel": 2763377759,
-                          "hgp_r_ssss": 2655604601, "hgp_r_ddss": 2276969846}
+MEASURED_TRAFFIC_BYTES = {"hgp_ryg_kernel": 4046039491, "hgp_r_ppps": 18689379887, "hgp_r_dpps": 18179650698,
+                          "hgp_r_dpds": 15032069152, "hgp_r_psps": 13542996320, "hgp_r_dsps": 12086832695,
+                          "hgp_r_psss": 11161218090, "hgp_r_dspp": 9326708062, "hgp_r_pppp": 8431444566,
+                          "hgp_r_ppss": 7119721388, "hgp_r_dpss": 6133995468, "hgp_r_ddps": 5588570871,
+                          "hgp_r_dsss": 5131085530, "hgp_r_dsds": 3098223339, "hgp_wg_kernel": 2762015519,
+                          "hgp_r_ssss": 2655462367, "hgp_r_ddss": 2273834727}
 
 
 def log(*a):
