@@ -1,6 +1,6 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle.
 
-Bar (north star): integral count and slot order exact; every integral within
+Bar: integral count and slot order exact; every integral within
 1e-12 absolute or 1e-10 relative of the FP64 oracle.  The oracle restates
 all_twoe_ints_chrr (src/HGP.jl:22-34) with the D1-D3 corrections; it is pinned
 against the reference's own known-answer tests in test_oracle.py.
@@ -642,7 +642,7 @@ def test_cluster_920_sampled_shell_quartets(ctx):
     integrals).  Plain FP64: every integral at the parity bar (binary128
     arbitration where the FP64 oracle cancels).  Mixed screened form (FP32 VRR
     below 1e-8, screening 1e-12, as the cluster's J/K build): every integral
-    within 1e-10 of the exact value (the north star's mixed bar)."""
+    within 1e-10 of the exact value (the bar of the mixed form)."""
     import torch
     from mi_amd import configs, sub_basis
     wl, _ = configs.cluster_40(0)
