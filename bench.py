@@ -132,9 +132,17 @@ def cpu_baseline_single(fb, n_unique, budget_s=10.0):
         O.eri_unique_hgp(fb, order=O.ORDER_CHRR, nthreads=threads)
         times.append(time.perf_counter() - t)
     med = float(np.median(times))
+    # one thread as well: median of 5 runs after the warm-up above
+    t1 = []
+    for _ in range(5):
+        t = time.perf_counter()
+        O.eri_unique_hgp(fb, order=O.ORDER_CHRR, nthreads=1)
+        t1.append(time.perf_counter() - t)
+    med1 = float(np.median(t1))
     return {"value": n_unique / med, "unit": "unique ERIs/s", "cores": threads, "kind": "port",
             "sample": f"whole 60-function array, {len(times)} runs, median {med * 1e3:.2f} ms",
-            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "one_thread": {"value": n_unique / med1, "ms": med1 * 1e3, "sample": "median of 5 runs"}}
 
 
 def timed(fn, reps, stream):
