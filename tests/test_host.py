@@ -240,3 +240,38 @@ def test_rys_kernel_rule_is_the_gauss_rule():
         _lib.rys_rule(nmax + 1, 1.0)
     with pytest.raises(_lib.MiError):
         _lib.rys_rule(1, -1.0)
+
+
+def test_tuning_file_rows_are_well_formed():
+    """hgp_tuning.txt (read by mi_create; written by mi_amd/autotune.py):
+    '#' comment lines, then rows 'class bra_bucket ket_bucket variant G
+    regime' with in-range fields, one row per key, register-kernel rows only
+    for classes that have a generated register kernel, lane and wavefront
+    counts each kernel family accepts, and every canonical class tuned for the
+    small- and large-launch regimes."""
+    import importlib.util
+    from mi_amd.hgp_tables import quartet_classes
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location(
+        "gen_hgp", os.path.join(root, "molecularintegrals.jl_amd", "csrc", "gen_hgp.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    classes = quartet_classes()
+    reg = {k for k, c in enumerate(classes) if gen.is_register_class(c)}
+    allowed = {0: {1, 2, 4}, 1: {1, 2, 4, 8, 16, 32, 64}, 2: {1, 2, 4, 8}, 3: {1, 2, 4}, 4: {1},
+               5: {1, 2, 4}, 6: {16, 32, 64}}
+    keys = set()
+    path = os.path.join(root, "molecularintegrals.jl_amd", "hgp_tuning.txt")
+    with open(path) as f:
+        for line in f:
+            if not line.strip() or line.startswith("#"):
+                continue
+            c, bb, kb, var, G, regime = (int(x) for x in line.split())
+            assert 0 <= c < len(classes) and 0 <= bb < 5 and 0 <= kb < 5 and regime in (0, 1, 2), line
+            assert var in allowed and G in allowed[var], line
+            assert var != 1 or c in reg, line
+            assert (c, bb, kb, regime) not in keys, line
+            keys.add((c, bb, kb, regime))
+    for regime in (0, 1):
+        for c in range(55):  # the canonical classes (L_A >= L_B, L_C >= L_D)
+            assert all((c, bb, kb, regime) in keys for bb in range(5) for kb in range(5)), (classes[c], regime)
