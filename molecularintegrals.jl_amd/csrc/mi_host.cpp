@@ -106,12 +106,14 @@ static double hrr_amplification(const double* xyz, const std::vector<int32_t>& L
   double r2 = 0.0;
   for (int k = 0; k < 3; ++k) r2 += (xyz[3 * A + k] - xyz[3 * B + k]) * (xyz[3 * A + k] - xyz[3 * B + k]);
   const double r = std::sqrt(r2);
-  double amp = 1.0;
+  double rmax = 1.0;  // x^L_B rises with x > 0: the largest ratio gives the largest power
   for (int pa = poff[A]; pa < poff[A + 1]; ++pa)
     for (int pb = poff[B]; pb < poff[B + 1]; ++pb) {
       const double a = pexp[pa], b = pexp[pb], z = a + b, s = 1.0 / std::sqrt(2.0 * z);
-      amp = std::max(amp, std::pow((b / z * r + s) / (a / z * r + s), (double)L[B]));
+      rmax = std::max(rmax, (b / z * r + s) / (a / z * r + s));
     }
+  double amp = rmax;
+  for (int e = 1; e < L[B]; ++e) amp *= rmax;
   return amp;
 }
 
@@ -862,7 +864,10 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
           for (int64_t t = ab0; t < (int64_t)ab_codes.size(); ++t) {
             const double a = pexp[poff[A] + (ab_codes[t] & 0xFFFF)], b = pexp[poff[B] + (ab_codes[t] >> 16)];
             const double z = a + b, sw = 1.0 / std::sqrt(2.0 * z);
-            amp = std::max(amp, std::pow((b / z * r + sw) / (a / z * r + sw), (double)L[B]));
+            const double ratio = (b / z * r + sw) / (a / z * r + sw);
+            double pw = ratio;
+            for (int e = 1; e < L[B]; ++e) pw *= ratio;
+            amp = std::max(amp, pw);
           }
         }
         keys.push_back({m, pair_class(L[A], L[B]), kbucket(K), i * (i + 1) / 2 + j, A, B, K,
@@ -877,14 +882,38 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   // splitting its lists by first shell as well was measured: about 6% less
   // J/K time on a 920-function cluster for 20x the plan-build time.
   const bool jk_layout = P->order == MI_ORDER_JK && cut2 == 0.0;
-  std::stable_sort(keys.begin(), keys.end(), [jk_layout](const PairKey& x, const PairKey& y) {
-    if (x.mol != y.mol) return x.mol < y.mol;
-    if (x.pc != y.pc) return x.pc < y.pc;
-    if (x.ext != y.ext) return x.ext < y.ext;
-    if (x.list != y.list) return x.list < y.list;
-    if (jk_layout && x.A != y.A) return x.A < y.A;
-    return x.pidx < y.pidx;
-  });
+  // Order (molecule, class, ext, list bucket, pidx): the keys come in
+  // molecule order with pidx increasing, so a stable counting sort over
+  // (class, ext, list) within each molecule gives it in linear time.
+  {
+    auto list_id = [](const PairKey& k) { return (k.pc * 2 + k.ext) * 16 + k.list; };  // list < 16
+    int ng = 0;
+    for (const PairKey& k : keys) ng = std::max(ng, list_id(k) + 1);
+    std::vector<PairKey> sorted(keys.size());
+    std::vector<int> cnt;
+    for (size_t s = 0, e; s < keys.size(); s = e) {
+      for (e = s; e < keys.size() && keys[e].mol == keys[s].mol;) ++e;
+      cnt.assign(ng + 1, 0);
+      for (size_t r = s; r < e; ++r) ++cnt[list_id(keys[r]) + 1];
+      for (int g = 0; g < ng; ++g) cnt[g + 1] += cnt[g];
+      for (size_t r = s; r < e; ++r) sorted[s + cnt[list_id(keys[r])]++] = keys[r];
+    }
+    keys.swap(sorted);
+  }
+  // fn over each (molecule, class, ext, list) run of keys
+  auto for_each_list = [&keys](auto fn) {
+    for (size_t s = 0, e; s < keys.size(); s = e) {
+      const PairKey& k = keys[s];
+      for (e = s + 1; e < keys.size() && keys[e].mol == k.mol && keys[e].pc == k.pc && keys[e].ext == k.ext &&
+                      keys[e].list == k.list;)
+        ++e;
+      fn(keys.begin() + s, keys.begin() + e);
+    }
+  };
+  if (jk_layout)  // then by first shell, pidx order kept within equal A
+    for_each_list([](std::vector<PairKey>::iterator b, std::vector<PairKey>::iterator e) {
+      std::stable_sort(b, e, [](const PairKey& x, const PairKey& y) { return x.A < y.A; });
+    });
   const bool has_bounds = !kQ.empty();
   if (has_bounds) {
     if (kQ.size() != keys.size()) {
@@ -897,12 +926,8 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     }
     // within each (molecule, class, bucket) list: largest bound first
     if (!jk_layout)
-      std::stable_sort(keys.begin(), keys.end(), [](const PairKey& x, const PairKey& y) {
-        if (x.mol != y.mol) return x.mol < y.mol;
-        if (x.pc != y.pc) return x.pc < y.pc;
-        if (x.ext != y.ext) return x.ext < y.ext;
-        if (x.list != y.list) return x.list < y.list;
-        return x.Q > y.Q;
+      for_each_list([](std::vector<PairKey>::iterator b, std::vector<PairKey>::iterator e) {
+        std::stable_sort(b, e, [](const PairKey& x, const PairKey& y) { return x.Q > y.Q; });
       });
   }
   const int npair = (int)keys.size();
@@ -1011,17 +1036,24 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
       if (!screen_tasks) {
         rc = emit(bra->start, bra->n, ket->start, ket->n, x == y);
       } else {
-        // n_i: ket columns with Q_i Q_j >= cut2 (kets ordered by Q, largest first)
+        // n_i: ket columns with Q_i Q_j >= cut2 (kets ordered by Q, largest
+        // first).  Called with i non-decreasing, so n_i only falls: a binary
+        // search for the first row, then a pointer moved down.
+        int ncur = -1;
         auto kept = [&](int i) -> int {
           const double qi = keys[bra->start + i].Q;
           if (!(qi > 0.0)) return 0;
           const double need = cut2 / qi;
-          int lo = 0, hi = ket->n;  // first j with Q_j < need
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (keys[ket->start + mid].Q >= need) lo = mid + 1; else hi = mid;
+          if (ncur < 0) {
+            int lo = 0, hi = ket->n;  // first j with Q_j < need
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (keys[ket->start + mid].Q >= need) lo = mid + 1; else hi = mid;
+            }
+            ncur = lo;
           }
-          return lo;
+          while (ncur > 0 && keys[ket->start + ncur - 1].Q < need) --ncur;
+          return ncur;
         };
         int row = 0;
         rc = MI_OK;
