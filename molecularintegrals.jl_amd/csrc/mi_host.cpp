@@ -725,10 +725,27 @@ int mi_plan_set_mol_outputs(mi_plan* P, double* const* outs) {
 // pair index) of a plan built with the same options: a bounds-only plan
 // (diagonal quartets) runs on the context's bounds stream with stream-ordered
 // memory and is read back.  Q = max |(ab|ab)|, V = max |(e0|e0)| (VRR outputs).
+// A shell pair of a plan being built, before its topology is laid out.
+struct PairKey {
+  int mol, pc, bucket, pidx, A, B, K;
+  int list;     // list split key: kbucket_fine (unique-array plans) or kbucket (J/K plans)
+  double Q, V;
+  int64_t ab0;  // first kept (ia, ib) code in ab_codes
+  int ext;      // HRR amplification above amp_limit: quartets run in double-double
+};
+// Pairs handed from a build to its bounds-only build (same bases and options,
+// so the same pairs in the same list order; the bounds-only build consumes them).
+struct SharedPairs {
+  std::vector<PairKey> keys;
+  std::vector<int32_t> ab_codes;
+};
+static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, int flags,
+                      SharedPairs* shared, mi_plan** out);
+
 static int compute_pair_bounds(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts,
-                               std::vector<double>& Q, std::vector<double>& V) {
+                               SharedPairs* shared, std::vector<double>& Q, std::vector<double>& V) {
   mi_plan* B = nullptr;
-  int rc = mi_plan_build(ctx, bases, nmol, opts, MI_BUILD_BOUNDS_ONLY | MI_BUILD_DEFER_UPLOAD, &B);
+  int rc = plan_build(ctx, bases, nmol, opts, MI_BUILD_BOUNDS_ONLY | MI_BUILD_DEFER_UPLOAD, shared, &B);
   if (rc) return rc;
   hipStream_t st = ctx->bstream;
   rc = mi_plan_upload(B, st);
@@ -752,6 +769,11 @@ static int compute_pair_bounds(mi_ctx* ctx, const mi_basis* bases, int nmol, con
 }
 
 int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, int flags, mi_plan** out) {
+  return plan_build(ctx, bases, nmol, opts, flags, nullptr, out);
+}
+
+static int plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* opts, int flags,
+                      SharedPairs* shared, mi_plan** out) {
   if (!ctx || !out || !bases || nmol <= 0) return set_err(ctx, MI_EINVAL, "bad plan arguments");
   *out = nullptr;
   int rc;
@@ -764,11 +786,6 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   const int method = opts ? opts->method : MI_METHOD_HGP;
   if (method != MI_METHOD_HGP && method != MI_METHOD_RYS) return set_err(ctx, MI_EINVAL, "unknown method %d", method);
   const bool rys = method == MI_METHOD_RYS;
-  // Bounds first (a device pass) when the plan screens or mixes precision:
-  // pairs are then ordered by bound and screened quartets never enter a task.
-  std::vector<double> kQ, kV;
-  if (!bounds_only && !(flags & MI_BUILD_NO_BOUNDS) && (tau2 > 0.0 || cut2 > 0.0))
-    if ((rc = compute_pair_bounds(ctx, bases, nmol, opts, kQ, kV))) return rc;
   mi_plan* P = new mi_plan();
   P->ctx = ctx;
   P->bounds_only = bounds_only;
@@ -818,17 +835,14 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
   // largest of their shell pair are dropped (22% of them are exactly zero --
   // exp(-ab|AB|^2/zeta) underflows -- and the rest contribute below FP64
   // resolution); the kept (ia, ib) of a pair are listed in pp_ab.
-  struct PairKey {
-    int mol, pc, bucket, pidx, A, B, K;
-    int list;     // list split key: kbucket_fine (unique-array plans) or kbucket (J/K plans)
-    double Q, V;
-    int64_t ab0;  // first kept (ia, ib) code in ab_codes
-    int ext;      // HRR amplification above amp_limit: quartets run in double-double
-  };
   std::vector<PairKey> keys;
   std::vector<int32_t> ab_codes;
   std::vector<double> kk;
-  for (int m = 0; m < nmol; ++m)
+  if (shared) {  // the main build's pairs, already in list order
+    keys = std::move(shared->keys);
+    ab_codes = std::move(shared->ab_codes);
+  }
+  for (int m = 0; m < nmol && !shared; ++m)
     for (int I = mol_sh0[m]; I < mol_sh0[m + 1]; ++I)
       for (int J = mol_sh0[m]; J <= I; ++J) {
         int A, B;
@@ -914,6 +928,16 @@ int mi_plan_build(mi_ctx* ctx, const mi_basis* bases, int nmol, const mi_opts* o
     for_each_list([](std::vector<PairKey>::iterator b, std::vector<PairKey>::iterator e) {
       std::stable_sort(b, e, [](const PairKey& x, const PairKey& y) { return x.A < y.A; });
     });
+  // Bounds (a device pass) when the plan screens or mixes precision: pairs
+  // are then ordered by bound and screened quartets never enter a task.
+  std::vector<double> kQ, kV;
+  if (!bounds_only && !(flags & MI_BUILD_NO_BOUNDS) && (tau2 > 0.0 || cut2 > 0.0)) {
+    SharedPairs sp{keys, ab_codes};
+    if ((rc = compute_pair_bounds(ctx, bases, nmol, opts, &sp, kQ, kV))) {
+      delete P;
+      return rc;
+    }
+  }
   const bool has_bounds = !kQ.empty();
   if (has_bounds) {
     if (kQ.size() != keys.size()) {
